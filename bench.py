@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Throughput of the particle erosion hot path on one MI355X.
+
+Workload (default): 1024x1024 float32 Perlin fBm terrain (8 octaves, base
+frequency 1/1024, lacunarity 2, gain 0.5, seed 1), default reference
+parameters, 65536 drops per cycle.  A "step" of this script is one cycle =
+one World::erode(65536) call (walk + cascade/blend).  The reported value is
+particle-steps/s over exactly --steps timed cycles, inputs resident in HBM.
+
+Prints one JSON line.  Roofline: the drop-walk kernel, HBM-bound,
+algorithmic bytes per particle-step B_STEP (reads: 5 heights, the float4
+{mx, my, d, erf(0.4 d)} and root density = 40 B; commits: int32 height,
+3 x int64 tracks, 1 visit byte = 29 B).  CPU baseline: the oracle's
+sequential restatement of World::erode on the same terrain and spawn stream,
+one core, on a bounded sample of cycles.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B_STEP = 69           # algorithmic bytes per particle-step of the walk kernel
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n: int, drops: int, seed: int, cycles: int) -> dict:
+    import numpy as np
+
+    import oracle
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    P = oracle.Params.default()
+    h = oracle.fbm(seed, n, n)
+    cells = oracle.to_aos(h)
+    steps = 0
+    t0 = time.perf_counter()
+    for c in range(cycles):
+        led = oracle.seq_erode(cells, P, seed, c, drops)
+        steps += led.steps
+    dt = time.perf_counter() - t0
+    return {"value": steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle sequential World::erode, {cycles} cycles x {drops} drops on the "
+                      f"{n}x{n} fBm terrain (seed {seed}), {steps} particle-steps in {dt:.1f} s, "
+                      f"1 thread on {cpu_model()}",
+            "seconds": dt, "particle_steps": steps}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500, help="timed cycles")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed cycles")
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--drops", type=int, default=65536)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--walk-block", type=int, default=0)
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the multi-GPU path (replicated map, drops "
+              "sharded over GPUs, one integer all-reduce of the commit maps per cycle) is "
+              "not built; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import simplehydrology_amd as shy
+
+    n, drops = args.size, args.drops
+    w = shy.World(n, n, seed=args.seed)
+    w.generate_fbm(args.seed, octaves=8)
+    if args.walk_block:
+        w.set_walk_block(args.walk_block)
+    if args.warmup:
+        w.erode_cycles(args.warmup, drops, graph=True)
+    w.synchronize()
+    s0 = w.stats()
+    t0 = time.perf_counter()
+    tm = w.time_cycles(args.steps, drops)  # synchronises at the end
+    wall = time.perf_counter() - t0
+    s1 = w.stats()
+    steps = s1["steps"] - s0["steps"]
+    sec = tm["ms_total"] / 1e3
+    value = steps / sec
+    walk_avg_s = tm["ms_walk"] / 1e3 / args.steps
+    steps_per_launch = steps / args.steps
+    achieved = steps_per_launch * B_STEP / walk_avg_s / 1e9
+    out = {
+        "metric": f"particle-steps/s at {n}x{n}",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tm["ms_total"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain method, single terrain: {n}x{n} fBm seed {args.seed}, "
+                               f"{drops} drops/cycle, default parameters",
+                   "cycles_timed": args.steps, "drops_per_cycle": drops,
+                   "walk_block": args.walk_block or 256},
+        "cycles_per_s": args.steps / sec,
+        "particle_steps": steps,
+        "host_wall_s": wall,
+        "kernel_ms_per_cycle": {"walk": tm["ms_walk"] / args.steps,
+                                "cascade_blend": tm["ms_cascade"] / args.steps},
+        "roofline": {"bound": "hbm", "kernel": "walk_kernel", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_step": B_STEP},
+    }
+    w.close()
+    if not args.no_cpu:
+        cb = cpu_baseline(n, drops, args.seed, args.cpu_cycles)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["speedup_vs_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

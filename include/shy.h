@@ -1,0 +1,181 @@
+/*
+ * shy.h -- C ABI of the MI355X particle hydraulic-erosion engine.
+ *
+ * The reference (weigert/SimpleHydrology) has no FFI; its boundary is the
+ * static C++ API of World and Drop.  Each entry point below names the
+ * reference symbol it stands in for.  include/shy_world.hpp wraps these in a
+ * C++ World/Drop facade with the reference's names.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative shy_status otherwise;
+ *    shy_last_error(ctx) gives a message.  No exception crosses the boundary.
+ *  - Maps are x-major: cell (x, y) lives at x*ny + y, the reference's
+ *    math::flatten (source/include/math.h:11-14).
+ *  - One context owns one terrain's device buffers and one HIP stream.  A
+ *    context is not thread-safe; distinct contexts are independent.
+ *  - Calls are asynchronous on the context's stream unless they return host
+ *    data (download/stats/trace), which synchronise.
+ */
+#ifndef SHY_H
+#define SHY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHY_ABI_VERSION 1
+
+typedef enum shy_status {
+  SHY_OK = 0,
+  SHY_ERR_ARG = -1,      /* invalid argument (size, pointer, parameter) */
+  SHY_ERR_HIP = -2,      /* HIP runtime error */
+  SHY_ERR_NODEV = -3,    /* no GPU visible */
+  SHY_ERR_NONFINITE = -4,/* input map holds NaN or Inf */
+  SHY_ERR_STATE = -5     /* call not valid in the context's current state */
+} shy_status;
+
+/* Parameter set: the reference's static members, same names and defaults.
+   Drop::{evapRate..momentumTransfer}  source/water.h:27-33, 43-50
+   World::{lrate, maxdiff, settling}   source/world.h:26-29, 42-44 */
+typedef struct shy_params {
+  float evapRate;         /* 0.001 */
+  float depositionRate;   /* 0.1   */
+  float minVol;           /* 0.01  */
+  float maxAge;           /* 500   */
+  float entrainment;      /* 10    */
+  float gravity;          /* 1     */
+  float momentumTransfer; /* 1     */
+  float lrate;            /* 0.1   */
+  float maxdiff;          /* 0.01  */
+  float settling;         /* 0.8   */
+} shy_params;
+
+/* Cell fields, in the reference's quad::cell order (source/cellpool.h:207-220). */
+typedef enum shy_field {
+  SHY_HEIGHT = 0,
+  SHY_DISCHARGE = 1,
+  SHY_MOMENTUMX = 2,
+  SHY_MOMENTUMY = 3,
+  SHY_DISCHARGE_TRACK = 4,
+  SHY_MOMENTUMX_TRACK = 5,
+  SHY_MOMENTUMY_TRACK = 6,
+  SHY_ROOTDENSITY = 7,
+  SHY_ERF_DISCHARGE = 8 /* erf(0.4*discharge), the value node::discharge returns */
+} shy_field;
+
+/* Per-step state of one traced drop (Drop fields, source/water.h:18-23). */
+typedef struct shy_drop_step {
+  int32_t age;    /* age before the step */
+  int32_t status; /* 0 continues, 1 out-of-bounds exit */
+  float pos[2];   /* after the step */
+  float speed[2]; /* after the step */
+  float volume;   /* after evaporation */
+  float sediment; /* after evaporation */
+  float erosion;  /* effD*cdiff taken from the start cell */
+  float h2;       /* downstream height used for the equilibrium */
+} shy_drop_step;
+
+/* A drop, as Drop holds it (source/water.h:18-23). */
+typedef struct shy_drop {
+  int32_t age;
+  float pos[2];
+  float speed[2];
+  float volume;
+  float sediment;
+} shy_drop;
+
+/* Counters summed over the cycles run since the last reset. */
+typedef struct shy_stats {
+  int64_t steps;      /* descend() calls that reach the physics update */
+  int64_t walked;     /* spawns that walked */
+  int64_t skipped;    /* spawns rejected by height < 0.1 (world.h:71-72) */
+  int64_t exits;      /* out-of-bounds exits */
+  int64_t ends_age;   /* ended by age > maxAge */
+  int64_t ends_vol;   /* ended by volume < minVol */
+  int64_t export40;   /* sediment carried off the map, 2^-40 units */
+  int64_t gain40;     /* evaporation sediment gain, 2^-40 units */
+  int64_t walk_dh;    /* net height committed by the walk, 2^-20 units */
+  int64_t casc_moves; /* non-zero cascade transfers */
+} shy_stats;
+
+typedef struct shy_ctx shy_ctx;
+
+/* Defaults of the reference statics (water.h:43-50, world.h:42-44). */
+void shy_default_params(shy_params* p);
+
+/* Number of visible GPUs (0 when none). */
+int shy_device_count(void);
+
+/* Create a terrain of nx by ny cells on the current device, all fields zero.
+   Stands in for cellpool.reserve + quad::map::init's allocation
+   (SimpleHydrology.cpp:36-38, source/cellpool.h:323-340); World::SEED
+   (source/world.h:21) is `seed`. */
+int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out);
+int shy_destroy(shy_ctx* ctx);
+const char* shy_last_error(const shy_ctx* ctx); /* never NULL */
+int shy_set_params(shy_ctx* ctx, const shy_params* p);
+int shy_get_params(const shy_ctx* ctx, shy_params* p);
+int shy_size(const shy_ctx* ctx, int* nx, int* ny);
+
+/* Whole-map transfer in the reference's AoS quad::cell layout: 8 floats per
+   cell, x-major (source/cellpool.h:207-220).  Upload rejects NaN/Inf. */
+int shy_upload_cells(shy_ctx* ctx, const float* cells8);
+int shy_download_cells(shy_ctx* ctx, float* cells8);
+/* One field as a dense x-major float array (shy_field). */
+int shy_upload_field(shy_ctx* ctx, int field, const float* src);
+int shy_download_field(shy_ctx* ctx, int field, float* dst);
+
+/* Fill the height field with the seeded 2D Perlin-gradient fBm (normalised
+   to [0,1]); stands in for quad::map::init's noise stack
+   (source/cellpool.h:349-409).  base_freq <= 0 means 1/max(nx, ny). */
+int shy_generate_fbm(shy_ctx* ctx, uint64_t seed, int octaves, float base_freq,
+                     float lacunarity, float gain);
+
+/* World::erode(cycles) (source/world.h:54-88): one cycle of `drops` spawns.
+   Wave-synchronous: every drop reads the map as it stood at the start of the
+   cycle and commits through fixed-point atomics; then the commits are
+   applied, the cascade runs as a stencil over visited cells and the
+   discharge/momentum fields are blended. */
+int shy_erode(shy_ctx* ctx, int drops);
+/* `cycles` consecutive World::erode(drops) calls; use_graph=1 replays them
+   from a captured hipGraph. */
+int shy_erode_cycles(shy_ctx* ctx, int cycles, int drops, int use_graph);
+
+/* One drop traced on the frozen current map (no map update): Drop(pos) then
+   Drop::descend() until it returns false (source/water.h:14, 58-156).
+   Writes up to max_steps states and the number of steps taken. */
+int shy_trace_drop(shy_ctx* ctx, float x, float y, int max_steps, shy_drop_step* out,
+                   int* n_steps);
+
+/* World::cascade(pos) (source/world.h:90-168) at one site, on the live map. */
+int shy_cascade(shy_ctx* ctx, float x, float y);
+/* One Drop::descend() call (source/water.h:58-156) on the live map, with the
+   reference's immediate writes and inline cascade; *alive gets its result. */
+int shy_descend(shy_ctx* ctx, shy_drop* drop, int* alive);
+
+/* Spawn stream state: World::SEED and the number of cycles run. */
+int shy_set_seed(shy_ctx* ctx, uint64_t seed, uint32_t cycle);
+int shy_get_cycle(const shy_ctx* ctx, uint32_t* cycle);
+
+int shy_get_stats(shy_ctx* ctx, shy_stats* out);
+int shy_reset_stats(shy_ctx* ctx);
+int shy_synchronize(shy_ctx* ctx);
+
+/* Launch shape of the drop walk: threads per block (64..1024, multiple of
+   64).  0 restores the default. */
+int shy_set_walk_block(shy_ctx* ctx, int threads);
+
+/* Measurement (no reference counterpart): run `cycles` World::erode(drops)
+   calls eagerly with HIP events on the context's stream around every
+   kernel; returns the wall time of the whole sequence and the summed
+   durations of the walk and cascade/blend kernels, in milliseconds. */
+int shy_time_cycles(shy_ctx* ctx, int cycles, int drops, float* ms_total, float* ms_walk,
+                    float* ms_cascade);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,243 @@
+"""CPU oracle for the particle hydraulic-erosion hot path -- TEST INFRASTRUCTURE.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+(``simplehydrology_amd``) never imports, links or loads it.
+
+ctypes bindings over ``oracle/liboracle.so`` (built from ``shy_oracle.c`` by
+``oracle/Makefile``).  Each function restates the reference:
+
+* :func:`seq_erode` / :func:`seq_descend` / :func:`seq_cascade` --
+  ``World::erode`` (source/world.h:54-88), ``Drop::descend``
+  (source/water.h:58-156), ``World::cascade`` (source/world.h:90-168) on a
+  live AoS ``quad::cell`` map (source/cellpool.h:207-220).
+* :func:`trace_frozen` -- one ``Drop::descend`` trajectory on an immutable map.
+* :func:`wave_cycle` -- the wave-synchronous fixed-point cycle the HIP path
+  reproduces bit for bit.
+* :func:`normal` -- ``quad::_normal`` (source/cellpool.h:181-204).
+* :func:`erf` -- the project erf (A&S 7.1.5 series in double).
+* :func:`fbm` / :func:`spawn_cells` -- the seeded input generators.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class Params(C.Structure):
+    """Reference statics: Drop (water.h:43-50) then World (world.h:42-44)."""
+
+    _fields_ = [(n, C.c_float) for n in (
+        "evapRate", "depositionRate", "minVol", "maxAge", "entrainment", "gravity",
+        "momentumTransfer", "lrate", "maxdiff", "settling")]
+
+    @classmethod
+    def default(cls, **kw):
+        p = cls(0.001, 0.1, 0.01, 500.0, 10.0, 1.0, 1.0, 0.1, 0.01, 0.8)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class Step(C.Structure):
+    _fields_ = [("age", C.c_int32), ("status", C.c_int32), ("pos", C.c_float * 2),
+                ("speed", C.c_float * 2), ("volume", C.c_float), ("sediment", C.c_float),
+                ("erosion", C.c_float), ("h2", C.c_float)]
+
+
+STEP_DTYPE = np.dtype([("age", "<i4"), ("status", "<i4"), ("pos", "<f4", (2,)),
+                       ("speed", "<f4", (2,)), ("volume", "<f4"), ("sediment", "<f4"),
+                       ("erosion", "<f4"), ("h2", "<f4")])
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "steps", "walked", "skipped", "exits", "ends_age", "ends_vol", "export40",
+        "gain40", "walk_dh", "casc_moves")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SeqLedger(C.Structure):
+    _fields_ = [("gain", C.c_double), ("export_", C.c_double), ("steps", C.c_int64),
+                ("walked", C.c_int64), ("skipped", C.c_int64), ("exits", C.c_int64)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        P = C.POINTER(Params)
+        L.orc_erf.argtypes = [C.c_float]
+        L.orc_erf.restype = C.c_float
+        L.orc_erf_many.argtypes = [fp, fp, C.c_int64]
+        L.orc_mix64.argtypes = [C.c_uint64]
+        L.orc_mix64.restype = C.c_uint64
+        L.orc_cycle_key.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_cycle_key.restype = C.c_uint64
+        L.orc_spawn_cell.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_spawn_cell.restype = C.c_uint32
+        L.orc_fbm.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                              C.c_float, fp]
+        L.orc_fbm_perm.argtypes = [C.c_uint64, C.c_int, u8p]
+        L.orc_normal.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
+        L.orc_seq_erode.argtypes = [fp, C.c_int, C.c_int, P, C.c_uint64, C.c_uint32,
+                                    C.c_uint32, C.c_int, C.POINTER(SeqLedger)]
+        L.orc_seq_cascade.argtypes = [fp, C.c_int, C.c_int, P, C.c_float, C.c_float]
+        L.orc_seq_descend.argtypes = [fp, C.c_int, C.c_int, P, C.POINTER(C.c_int32), fp, fp,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_seq_descend.restype = C.c_int
+        L.orc_set_seq_erf.argtypes = [C.c_int]
+        L.orc_trace_frozen.argtypes = [fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, P, C.c_float,
+                                       C.c_float, C.c_int, C.c_int, C.c_void_p,
+                                       C.POINTER(C.c_int)]
+        L.orc_trace_frozen.restype = C.c_int
+        L.orc_wave_cycle.argtypes = [fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, P, C.c_uint64,
+                                     C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Stats),
+                                     C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+# ----------------------------------------------------------------- helpers
+AOS_FIELDS = ("height", "discharge", "momentumx", "momentumy", "discharge_track",
+              "momentumx_track", "momentumy_track", "rootdensity")
+
+
+def erf(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib().orc_erf_many(x.ravel(), y.ravel(), x.size)
+    return y
+
+
+def mix64(z: int) -> int:
+    return lib().orc_mix64(z)
+
+
+def cycle_key(seed: int, terrain: int, cycle: int) -> int:
+    return lib().orc_cycle_key(seed, terrain, cycle)
+
+
+def spawn_cells(seed: int, terrain: int, cycle: int, drops: int, ncells: int) -> np.ndarray:
+    k = cycle_key(seed, terrain, cycle)
+    f = lib().orc_spawn_cell
+    return np.array([f(k, i, ncells) for i in range(drops)], dtype=np.uint32)
+
+
+def fbm(seed: int, nx: int, ny: int, octaves: int = 8, base_freq: float | None = None,
+        lacunarity: float = 2.0, gain: float = 0.5) -> np.ndarray:
+    if base_freq is None:
+        base_freq = np.float32(1.0) / np.float32(max(nx, ny))
+    out = np.empty(nx * ny, np.float32)
+    lib().orc_fbm(seed, nx, ny, octaves, float(base_freq), lacunarity, gain, out)
+    return out.reshape(nx, ny)
+
+
+def fbm_perm(seed: int, octave: int) -> np.ndarray:
+    p = np.empty(256, np.uint8)
+    lib().orc_fbm_perm(seed, octave, p)
+    return p
+
+
+def normal(h: np.ndarray, x: int, y: int) -> np.ndarray:
+    h = np.ascontiguousarray(h, dtype=np.float32)
+    n = np.empty(3, np.float32)
+    lib().orc_normal(h.ravel(), h.shape[0], h.shape[1], x, y, n)
+    return n
+
+
+def to_aos(h, d=None, mx=None, my=None, root=None) -> np.ndarray:
+    nx, ny = h.shape
+    c = np.zeros((nx, ny, 8), np.float32)
+    c[..., 0] = h
+    for i, f in ((1, d), (2, mx), (3, my), (7, root)):
+        if f is not None:
+            c[..., i] = f
+    return c
+
+
+def seq_erode(cells: np.ndarray, params: Params, seed: int, cycle: int, drops: int,
+              terrain: int = 0) -> SeqLedger:
+    """One ``World::erode`` call on AoS cells (nx, ny, 8), in place."""
+    assert cells.dtype == np.float32 and cells.flags.c_contiguous and cells.shape[2] == 8
+    led = SeqLedger()
+    lib().orc_seq_erode(cells.reshape(-1), cells.shape[0], cells.shape[1], C.byref(params),
+                        seed, terrain, cycle, drops, C.byref(led))
+    return led
+
+
+def seq_cascade(cells: np.ndarray, params: Params, x: float, y: float) -> None:
+    lib().orc_seq_cascade(cells.reshape(-1), cells.shape[0], cells.shape[1], C.byref(params),
+                          x, y)
+
+
+def set_seq_erf(project: bool) -> None:
+    """Live path erf: glibc ``erff`` (reference, default) or the project erf."""
+    lib().orc_set_seq_erf(1 if project else 0)
+
+
+def seq_descend(cells, params, drop: dict) -> bool:
+    """One reference ``Drop::descend`` call; ``drop`` holds age/pos/speed/volume/sediment."""
+    age = C.c_int32(drop["age"])
+    pos = np.array(drop["pos"], np.float32)
+    speed = np.array(drop["speed"], np.float32)
+    vol = C.c_float(drop["volume"])
+    sed = C.c_float(drop["sediment"])
+    r = lib().orc_seq_descend(cells.reshape(-1), cells.shape[0], cells.shape[1],
+                              C.byref(params), C.byref(age), pos, speed, C.byref(vol),
+                              C.byref(sed))
+    drop.update(age=age.value, pos=pos, speed=speed, volume=vol.value, sediment=sed.value)
+    return bool(r)
+
+
+def trace_frozen(h, d, mx, my, root, erfd, params, x0, y0, max_steps=600, use_ref_erf=True):
+    """Per-step states of one drop on an immutable map.  Returns (steps, end_reason)."""
+    h = np.ascontiguousarray(h, np.float32)
+    nx, ny = h.shape
+    arrs = [np.ascontiguousarray(a if a is not None else np.zeros_like(h), np.float32).ravel()
+            for a in (h, d, mx, my, root, erfd)]
+    out = np.zeros(max_steps, STEP_DTYPE)
+    er = C.c_int(0)
+    n = lib().orc_trace_frozen(*arrs, nx, ny, C.byref(params), x0, y0, max_steps,
+                               1 if use_ref_erf else 0, out.ctypes.data, C.byref(er))
+    return out[:min(n, max_steps)], er.value
+
+
+def wave_cycle(h, d, mx, my, root, erfd, params, seed, cycle, drops, terrain=0,
+               want_dh=False):
+    """One wave-synchronous cycle in place on SoA float32 (nx, ny) arrays."""
+    nx, ny = h.shape
+    for a in (h, d, mx, my, root, erfd):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.shape == (nx, ny)
+    st = Stats()
+    dh = np.zeros((nx, ny), np.int32) if want_dh else None
+    vis = np.zeros((nx, ny), np.uint8) if want_dh else None
+    lib().orc_wave_cycle(h.reshape(-1), d.reshape(-1), mx.reshape(-1), my.reshape(-1),
+                         root.reshape(-1), erfd.reshape(-1), nx, ny, C.byref(params), seed,
+                         terrain, cycle, drops, C.byref(st),
+                         dh.ctypes.data if want_dh else None,
+                         vis.ctypes.data if want_dh else None)
+    if want_dh:
+        return st, dh, vis
+    return st

@@ -1,0 +1,663 @@
+/*
+ * shy_oracle.c -- CPU oracle for the particle hydraulic-erosion hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): only tests/,
+ * the smoke check and bench.py's CPU-baseline leg load it.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math), the
+ * reference's -O2 (Makefile:3-4).  Float expressions are written in the
+ * evaluation order of the reference's C++/glm source so that every rounding
+ * happens where the reference's does; the glm forms restated here are
+ * length(v) = sqrt(dot(v,v)), normalize(v) = v * (1/sqrt(dot(v,v))),
+ * dot = left-to-right sum of products, cross = (a.y*b.z-b.y*a.z,
+ * a.z*b.x-b.z*a.x, a.x*b.y-b.x*a.y), vec2->ivec2 = truncation toward zero.
+ */
+#include "shy_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAPSCALE 80.0f /* quad::mapscale, cellpool.h:165 */
+#define LOD 1.0f       /* quad::lodsize, cellpool.h:178 (int 1 -> float) */
+#define HQ 1048576.0f  /* height deltas: int32, 2^-20 units */
+#define HQ_INV 9.5367431640625e-07f
+#define TQ 4294967296.0f /* tracks: int64, 2^-32 units */
+#define TQ_INV 2.3283064365386963e-10
+#define SQ40 1099511627776.0 /* ledger sums: 2^-40 units */
+
+void orc_default_params(orc_params* p) {
+  /* water.h:43-50 and world.h:42-44 */
+  p->evapRate = 0.001f;
+  p->depositionRate = 0.1f;
+  p->minVol = 0.01f;
+  p->maxAge = 500.0f;
+  p->entrainment = 10.0f;
+  p->gravity = 1.0f;
+  p->momentumTransfer = 1.0f;
+  p->lrate = 0.1f;
+  p->maxdiff = 0.01f;
+  p->settling = 0.8f;
+}
+
+/* ------------------------------------------------------------------ erf */
+/* erf(x) = 2/sqrt(pi) * sum_k (-1)^k x^(2k+1) / (k! (2k+1))   (A&S 7.1.5),
+   Horner in t = x^2, in double, rounded once to float.  |x| >= 4 gives +-1
+   (erfc(4) = 1.5e-8 < 2^-25, so the float result is exactly 1). */
+#define ERF_N 72
+static double g_erf_a[ERF_N];
+static int g_erf_init = 0;
+
+static void erf_init(void) {
+  const double two_over_sqrtpi = 1.1283791670955126;
+  double f = 1.0;
+  for (int k = 0; k < ERF_N; ++k) {
+    if (k > 0) f = f * (double)k;
+    double a = two_over_sqrtpi / (f * (double)(2 * k + 1));
+    g_erf_a[k] = (k & 1) ? -a : a;
+  }
+  g_erf_init = 1;
+}
+
+float orc_erf(float x) {
+  if (!g_erf_init) erf_init();
+  if (x == 0.0f) return x;
+  double ax = fabs((double)x);
+  float r;
+  if (ax >= 4.0) {
+    r = 1.0f;
+  } else {
+    int n = ax < 0.5 ? 16 : (ax < 2.0 ? 36 : ERF_N);
+    double t = ax * ax;
+    double p = g_erf_a[n - 1];
+    for (int k = n - 2; k >= 0; --k) p = p * t + g_erf_a[k];
+    r = (float)(ax * p);
+  }
+  return x < 0.0f ? -r : r;
+}
+
+void orc_erf_many(const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = orc_erf(x[i]);
+}
+
+/* ------------------------------------------------------------------ rng */
+/* SplitMix64 output function (Steele, Lea, Flood 2014). */
+uint64_t orc_mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t orc_cycle_key(uint64_t seed, uint32_t terrain, uint32_t cycle) {
+  uint64_t k = orc_mix64(seed);
+  k = orc_mix64(k ^ (uint64_t)terrain);
+  return orc_mix64(k ^ ((uint64_t)cycle << 32));
+}
+
+/* Uniform cell index in [0, ncells): Lemire multiply-shift on the high word. */
+uint32_t orc_spawn_cell(uint64_t cycle_key, uint32_t drop, uint32_t ncells) {
+  uint64_t r = orc_mix64(cycle_key ^ (uint64_t)drop);
+  return (uint32_t)(((r >> 32) * (uint64_t)ncells) >> 32);
+}
+
+/* ------------------------------------------------------------------ fBm */
+void orc_fbm_perm(uint64_t seed, int octave, uint8_t* perm) {
+  for (int i = 0; i < 256; ++i) perm[i] = (uint8_t)i;
+  uint64_t key = orc_mix64(orc_mix64(seed ^ 0x5045524c494e4f49ULL) + (uint64_t)octave);
+  for (int i = 255; i > 0; --i) {
+    uint64_t r = orc_mix64(key ^ (uint64_t)i);
+    int j = (int)((r >> 32) % (uint64_t)(i + 1));
+    uint8_t t = perm[i];
+    perm[i] = perm[j];
+    perm[j] = t;
+  }
+}
+
+static float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+
+static float grad2(int h, float x, float y) {
+  switch (h & 7) {
+    case 0: return x + y;
+    case 1: return -x + y;
+    case 2: return x - y;
+    case 3: return -x - y;
+    case 4: return x;
+    case 5: return -x;
+    case 6: return y;
+    default: return -y;
+  }
+}
+
+static float perlin2(const uint8_t* P, float x, float y) {
+  float xf = floorf(x), yf = floorf(y);
+  int xi = ((int)xf) & 255, yi = ((int)yf) & 255;
+  float fx = x - xf, fy = y - yf;
+  float u = fade(fx), v = fade(fy);
+  int xj = (xi + 1) & 255;
+  int aa = P[(P[xi] + yi) & 255], ab = P[(P[xi] + yi + 1) & 255];
+  int ba = P[(P[xj] + yi) & 255], bb = P[(P[xj] + yi + 1) & 255];
+  float n00 = grad2(aa, fx, fy);
+  float n10 = grad2(ba, fx - 1.0f, fy);
+  float n01 = grad2(ab, fx, fy - 1.0f);
+  float n11 = grad2(bb, fx - 1.0f, fy - 1.0f);
+  float x1 = n00 + u * (n10 - n00);
+  float x2 = n01 + u * (n11 - n01);
+  return x1 + v * (x2 - x1);
+}
+
+void orc_fbm(uint64_t seed, int nx, int ny, int octaves, float base_freq,
+             float lacunarity, float gain, float* out) {
+  uint8_t perm[16][256];
+  if (octaves > 16) octaves = 16;
+  for (int o = 0; o < octaves; ++o) orc_fbm_perm(seed, o, perm[o]);
+  float mn = INFINITY, mx = -INFINITY;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      float s = 0.0f, amp = 1.0f, f = base_freq;
+      for (int o = 0; o < octaves; ++o) {
+        s = s + amp * perlin2(perm[o], (float)x * f, (float)y * f);
+        f = f * lacunarity;
+        amp = amp * gain;
+      }
+      out[(size_t)x * ny + y] = s;
+      mn = s < mn ? s : mn;
+      mx = s > mx ? s : mx;
+    }
+  float range = mx - mn;
+  for (size_t i = 0; i < (size_t)nx * ny; ++i)
+    out[i] = range > 0.0f ? (out[i] - mn) / range : 0.0f;
+}
+
+/* ------------------------------------------------------------ accessors */
+static inline int oob(int nx, int ny, int x, int y) {
+  /* quad::map::oob, cellpool.h:413-419 */
+  return x < 0 || y < 0 || x >= nx || y >= ny;
+}
+
+/* quad::_normal, cellpool.h:181-204.  st = float stride between cells. */
+static void normal_st(const float* h, int st, int nx, int ny, int x, int y, float* n) {
+#define HT(X, Y) (oob(nx, ny, (X), (Y)) ? 0.0f : h[((size_t)(X) * ny + (Y)) * st])
+  const float s0 = 1.0f, s1 = MAPSCALE, s2 = 1.0f; /* vec3(1.0, mapscale, 1.0) */
+  float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+  float ax, ay, az, bx, by, bz;
+  const float hp = HT(x, y);
+  if (!oob(nx, ny, x + 1, y + 1)) { /* :187-188 */
+    ax = s0 * 0.0f; ay = s1 * (HT(x, y + 1) - hp); az = s2 * 1.0f;
+    bx = s0 * 1.0f; by = s1 * (HT(x + 1, y) - hp); bz = s2 * 0.0f;
+    n0 = n0 + (ay * bz - by * az); n1 = n1 + (az * bx - bz * ax); n2 = n2 + (ax * by - bx * ay);
+  }
+  if (!oob(nx, ny, x - 1, y - 1)) { /* :190-191 */
+    ax = s0 * 0.0f; ay = s1 * (HT(x, y - 1) - hp); az = s2 * -1.0f;
+    bx = s0 * -1.0f; by = s1 * (HT(x - 1, y) - hp); bz = s2 * 0.0f;
+    n0 = n0 + (ay * bz - by * az); n1 = n1 + (az * bx - bz * ax); n2 = n2 + (ax * by - bx * ay);
+  }
+  if (!oob(nx, ny, x + 1, y - 1)) { /* :194-195 */
+    ax = s0 * 1.0f; ay = s1 * (HT(x + 1, y) - hp); az = s2 * 0.0f;
+    bx = s0 * 0.0f; by = s1 * (HT(x, y - 1) - hp); bz = s2 * -1.0f;
+    n0 = n0 + (ay * bz - by * az); n1 = n1 + (az * bx - bz * ax); n2 = n2 + (ax * by - bx * ay);
+  }
+  if (!oob(nx, ny, x - 1, y + 1)) { /* :197-198 */
+    ax = s0 * -1.0f; ay = s1 * (HT(x - 1, y) - hp); az = s2 * 0.0f;
+    bx = s0 * 0.0f; by = s1 * (HT(x, y + 1) - hp); bz = s2 * 1.0f;
+    n0 = n0 + (ay * bz - by * az); n1 = n1 + (az * bx - bz * ax); n2 = n2 + (ax * by - bx * ay);
+  }
+#undef HT
+  float dd = n0 * n0 + n1 * n1 + n2 * n2; /* dot, left to right */
+  if (sqrtf(dd) > 0.0f) {                 /* length(n) > 0 */
+    float r = 1.0f / sqrtf(dd);           /* glm inversesqrt */
+    n0 = n0 * r; n1 = n1 * r; n2 = n2 * r;
+  }
+  n[0] = n0; n[1] = n1; n[2] = n2;
+}
+
+void orc_normal(const float* h, int nx, int ny, int x, int y, float* n3) {
+  normal_st(h, 1, nx, ny, x, y, n3);
+}
+
+/* ---------------------------------------------------- shared step physics */
+/* water.h:86-111: forces, momentum coupling, fixed step of length sqrt(2).
+   Returns nothing; updates speed and pos. */
+static inline void step_motion(const orc_params* P, const float* n, float vol, float d,
+                               float fx, float fy, float* sx, float* sy, float* px,
+                               float* py) {
+  float k = LOD * P->gravity;
+  *sx = *sx + (k * n[0]) / vol; /* :95 speed += lodsize*gravity*vec2(n.x,n.z)/volume */
+  *sy = *sy + (k * n[2]) / vol;
+  /* :97-99 */
+  if (sqrtf(fx * fx + fy * fy) > 0.0f && sqrtf(*sx * *sx + *sy * *sy) > 0.0f) {
+    float rf = 1.0f / sqrtf(fx * fx + fy * fy);
+    float rs = 1.0f / sqrtf(*sx * *sx + *sy * *sy);
+    float dt = (fx * rf) * (*sx * rs) + (fy * rf) * (*sy * rs);
+    float c = ((LOD * P->momentumTransfer) * dt) / (vol + d);
+    *sx = *sx + c * fx;
+    *sy = *sy + c * fy;
+  }
+  /* :108-109 */
+  if (sqrtf(*sx * *sx + *sy * *sy) > 0.0f) {
+    const float k2 = LOD * sqrtf(2.0f);
+    float r = 1.0f / sqrtf(*sx * *sx + *sy * *sy);
+    *sx = k2 * (*sx * r);
+    *sy = k2 * (*sy * r);
+  }
+  *px = *px + *sx; /* :111 */
+  *py = *py + *sy;
+}
+
+static inline float evap_sed(float sed, float e) { return (float)((double)sed / (1.0 - (double)e)); }
+static inline float evap_vol(float vol, float e) { return (float)((double)vol * (1.0 - (double)e)); }
+
+/* ------------------------------------------------ (a) sequential reference */
+/* 0: glibc erff as the reference; 1: the project erf (to compare bit for bit
+   with a device restatement of the same live semantics). */
+static int g_seq_project_erf = 0;
+void orc_set_seq_erf(int use_project) { g_seq_project_erf = use_project; }
+
+enum { C_H = 0, C_D, C_MX, C_MY, C_DT, C_MXT, C_MYT, C_ROOT, C_N };
+
+void orc_seq_cascade(float* cells, int nx, int ny, const orc_params* P, float fx, float fy) {
+  /* World::cascade, world.h:90-168 */
+  static const int off[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1},
+                                {0, 1},   {1, -1}, {1, 0},  {1, 1}};
+  int px[8], py[8];
+  float sh[8], sd[8];
+  int num = 0;
+  int cx = (int)fx, cy = (int)fy;
+  for (int k = 0; k < 8; ++k) {
+    int x = cx + 1 * off[k][0], y = cy + 1 * off[k][1];
+    if (oob(nx, ny, x, y)) continue;
+    float vx = (float)off[k][0], vy = (float)off[k][1];
+    px[num] = x; py[num] = y;
+    sh[num] = cells[((size_t)x * ny + y) * C_N + C_H];
+    sd[num] = sqrtf(vx * vx + vy * vy); /* length(vec2(nn)) */
+    ++num;
+  }
+  /* std::sort on <= 16 elements: libstdc++ insertion sort (stable for '<') */
+  for (int i = 1; i < num; ++i) {
+    int tx = px[i], ty = py[i];
+    float th = sh[i], td = sd[i];
+    int j = i;
+    while (j > 0 && th < sh[j - 1]) {
+      px[j] = px[j - 1]; py[j] = py[j - 1]; sh[j] = sh[j - 1]; sd[j] = sd[j - 1];
+      --j;
+    }
+    px[j] = tx; py[j] = ty; sh[j] = th; sd[j] = td;
+  }
+  float* hc = &cells[((size_t)cx * ny + cy) * C_N + C_H];
+  for (int i = 0; i < num; ++i) {
+    float diff = *hc - sh[i];
+    if (diff == 0) continue;
+    float excess;
+    if ((double)sh[i] > 0.1)
+      excess = fabsf(diff) - sd[i] * P->maxdiff * LOD;
+    else
+      excess = fabsf(diff);
+    if (excess <= 0) continue;
+    float transfer = P->settling * excess / 2.0f;
+    float* hn = &cells[((size_t)px[i] * ny + py[i]) * C_N + C_H];
+    if (diff > 0) {
+      *hc -= transfer;
+      *hn += transfer;
+    } else {
+      *hc += transfer;
+      *hn -= transfer;
+    }
+  }
+}
+
+/* Drop::descend, water.h:58-156, on the live map.  Returns 0 when the call
+   ends before the physics update (NULL cell, age, volume), 1 for a step that
+   continues, 2 for the out-of-bounds exit step.  *gain receives evaporation's
+   sediment increase of the step (ledger only). */
+static int seq_descend_impl(float* cells, int nx, int ny, const orc_params* P, int32_t* age,
+                            float* pos, float* speed, float* volume, float* sediment,
+                            double* gain) {
+  int ix = (int)pos[0], iy = (int)pos[1];
+  if (oob(nx, ny, ix, iy)) return 0; /* :62-68 */
+  float* c = &cells[((size_t)ix * ny + iy) * C_N];
+  float n[3];
+  normal_st(cells, C_N, nx, ny, ix, iy, n); /* :70 */
+  if ((float)*age > P->maxAge) {            /* :74-77 */
+    c[C_H] += *sediment;
+    return 0;
+  }
+  if (*volume < P->minVol) { /* :79-82 */
+    c[C_H] += *sediment;
+    return 0;
+  }
+  float effD = P->depositionRate * (1.0f - c[C_ROOT]); /* :86-87 */
+  if (effD < 0) effD = 0;
+  float sx = speed[0], sy = speed[1], px = pos[0], py = pos[1];
+  step_motion(P, n, *volume, c[C_D], c[C_MX], c[C_MY], &sx, &sy, &px, &py);
+  c[C_DT] += *volume; /* :115-117 */
+  c[C_MXT] += *volume * sx;
+  c[C_MYT] += *volume * sy;
+  int jx = (int)px, jy = (int)py;
+  float h2;
+  if (oob(nx, ny, jx, jy)) /* :120-124 */
+    h2 = (float)((double)c[C_H] - 0.002);
+  else
+    h2 = cells[((size_t)jx * ny + jy) * C_N + C_H];
+  /* node::discharge, cellpool.h:242-244 */
+  float E = g_seq_project_erf ? orc_erf(0.4f * c[C_D]) : erff(0.4f * c[C_D]);
+  float c_eq = (1.0f + P->entrainment * E) * (c[C_H] - h2); /* :127 */
+  if (c_eq < 0) c_eq = 0;
+  float cdiff = (c_eq - *sediment);
+  *sediment += effD * cdiff; /* :131 */
+  c[C_H] -= effD * cdiff;    /* :132 */
+  float s1 = evap_sed(*sediment, P->evapRate); /* :135 */
+  if (gain) *gain += (double)s1 - (double)*sediment;
+  *sediment = s1;
+  *volume = evap_vol(*volume, P->evapRate); /* :136 */
+  speed[0] = sx; speed[1] = sy; pos[0] = px; pos[1] = py;
+  if (oob(nx, ny, jx, jy)) { /* :139-142 */
+    *volume = 0.0f;
+    return 2;
+  }
+  orc_seq_cascade(cells, nx, ny, P, px, py); /* :151 */
+  *age += 1;                                 /* :153 */
+  return 1;
+}
+
+int orc_seq_descend(float* cells, int nx, int ny, const orc_params* P, int32_t* age,
+                    float* pos, float* speed, float* volume, float* sediment) {
+  return seq_descend_impl(cells, nx, ny, P, age, pos, speed, volume, sediment, NULL) == 1;
+}
+
+void orc_seq_erode(float* cells, int nx, int ny, const orc_params* P, uint64_t seed,
+                   uint32_t terrain, uint32_t cycle, int drops, orc_seq_ledger* led) {
+  /* World::erode, world.h:54-88 (spawn stream: the counter hash, not rand()) */
+  size_t ncell = (size_t)nx * ny;
+  for (size_t i = 0; i < ncell; ++i) { /* :56-61 */
+    cells[i * C_N + C_DT] = 0;
+    cells[i * C_N + C_MXT] = 0;
+    cells[i * C_N + C_MYT] = 0;
+  }
+  uint64_t key = orc_cycle_key(seed, terrain, cycle);
+  for (int i = 0; i < drops; ++i) { /* :64-78 */
+    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)ncell);
+    int x = (int)(cid / (uint32_t)ny), y = (int)(cid % (uint32_t)ny);
+    if ((double)cells[(size_t)cid * C_N + C_H] < 0.1) { /* :71-72 */
+      if (led) led->skipped++;
+      continue;
+    }
+    if (led) led->walked++;
+    int32_t age = 0;
+    float pos[2] = {(float)x, (float)y}, speed[2] = {0.0f, 0.0f};
+    float vol = 1.0f, sed = 0.0f;
+    double gain = 0.0;
+    int64_t steps = 0;
+    for (;;) { /* :76 while(drop.descend()); */
+      int r = seq_descend_impl(cells, nx, ny, P, &age, pos, speed, &vol, &sed, &gain);
+      if (r) ++steps;
+      if (r == 2 && led) {
+        led->exits++;
+        led->export_ += (double)sed;
+      }
+      if (r != 1) break;
+    }
+    if (led) {
+      led->steps += steps;
+      led->gain += gain;
+    }
+  }
+  for (size_t i = 0; i < ncell; ++i) { /* :81-86 */
+    float* c = &cells[i * C_N];
+    c[C_D] = (1.0f - P->lrate) * c[C_D] + P->lrate * c[C_DT];
+    c[C_MX] = (1.0f - P->lrate) * c[C_MX] + P->lrate * c[C_MXT];
+    c[C_MY] = (1.0f - P->lrate) * c[C_MY] + P->lrate * c[C_MYT];
+  }
+}
+
+/* ---------------------------------------------------- fixed-point commits */
+static inline int32_t qh(float v) {
+  float s = v * HQ;
+  if (!(s == s)) s = 0.0f;
+  if (s > 1073741824.0f) s = 1073741824.0f;
+  if (s < -1073741824.0f) s = -1073741824.0f;
+  return (int32_t)rintf(s);
+}
+static inline int64_t qt(float v) {
+  float s = v * TQ;
+  if (!(s == s)) s = 0.0f;
+  if (s > 4.0e18f) s = 4.0e18f;
+  if (s < -4.0e18f) s = -4.0e18f;
+  return (int64_t)rintf(s);
+}
+static inline int64_t q40(double v) {
+  double s = v * SQ40;
+  if (!(s == s)) s = 0.0;
+  if (s > 9.0e18) s = 9.0e18;
+  if (s < -9.0e18) s = -9.0e18;
+  return (int64_t)rint(s);
+}
+
+/* ------------------------------------------------- (b) frozen single drop */
+int orc_trace_frozen(const float* h, const float* d, const float* mx, const float* my,
+                     const float* root, const float* erfd, int nx, int ny,
+                     const orc_params* P, float x0, float y0, int max_steps,
+                     int use_ref_erf, orc_step* out, int* end_reason) {
+  int32_t age = 0;
+  float px = x0, py = y0, sx = 0.0f, sy = 0.0f, vol = 1.0f, sed = 0.0f;
+  int nst = 0;
+  *end_reason = 0;
+  for (;;) {
+    int ix = (int)px, iy = (int)py;
+    if (oob(nx, ny, ix, iy)) { *end_reason = 1; break; }
+    size_t i0 = (size_t)ix * ny + iy;
+    float n[3];
+    normal_st(h, 1, nx, ny, ix, iy, n);
+    if ((float)age > P->maxAge) { *end_reason = 2; break; }
+    if (vol < P->minVol) { *end_reason = 3; break; }
+    float effD = P->depositionRate * (1.0f - root[i0]);
+    if (effD < 0) effD = 0;
+    step_motion(P, n, vol, d[i0], mx[i0], my[i0], &sx, &sy, &px, &py);
+    int jx = (int)px, jy = (int)py;
+    float hc = h[i0];
+    float h2 = oob(nx, ny, jx, jy) ? (float)((double)hc - 0.002) : h[(size_t)jx * ny + jy];
+    float E = use_ref_erf ? erff(0.4f * d[i0]) : erfd[i0];
+    float c_eq = (1.0f + P->entrainment * E) * (hc - h2);
+    if (c_eq < 0) c_eq = 0;
+    float cdiff = (c_eq - sed);
+    float ero = effD * cdiff;
+    sed += ero;
+    sed = evap_sed(sed, P->evapRate);
+    vol = evap_vol(vol, P->evapRate);
+    int ex = oob(nx, ny, jx, jy);
+    if (nst < max_steps) {
+      orc_step* o = &out[nst];
+      o->age = age; o->status = ex;
+      o->pos[0] = px; o->pos[1] = py; o->speed[0] = sx; o->speed[1] = sy;
+      o->volume = vol; o->sediment = sed; o->erosion = ero; o->h2 = h2;
+    }
+    ++nst;
+    if (ex) { *end_reason = 4; break; }
+    ++age;
+    if (nst >= max_steps) { *end_reason = 5; break; }
+  }
+  return nst;
+}
+
+/* --------------------------------------------------- (c) wave-synchronous */
+typedef struct wacc {
+  int32_t* dh;
+  int64_t *td, *tmx, *tmy;
+  uint8_t* visit;
+} wacc;
+
+static void wave_walk(const float* h, const float* d, const float* mx, const float* my,
+                      const float* root, const float* erfd, int nx, int ny,
+                      const orc_params* P, float x0, float y0, wacc* A, orc_stats* st) {
+  int32_t age = 0;
+  float px = x0, py = y0, sx = 0.0f, sy = 0.0f, vol = 1.0f, sed = 0.0f;
+  double gain = 0.0;
+  int64_t steps = 0;
+  for (;;) {
+    int ix = (int)px, iy = (int)py;
+    size_t i0 = (size_t)ix * ny + iy; /* in bounds: spawn or checked below */
+    float n[3];
+    normal_st(h, 1, nx, ny, ix, iy, n);
+    if ((float)age > P->maxAge) {
+      int32_t q = qh(sed);
+      A->dh[i0] += q;
+      st->walk_dh += q;
+      st->ends_age++;
+      break;
+    }
+    if (vol < P->minVol) {
+      int32_t q = qh(sed);
+      A->dh[i0] += q;
+      st->walk_dh += q;
+      st->ends_vol++;
+      break;
+    }
+    ++steps;
+    float effD = P->depositionRate * (1.0f - root[i0]);
+    if (effD < 0) effD = 0;
+    step_motion(P, n, vol, d[i0], mx[i0], my[i0], &sx, &sy, &px, &py);
+    A->td[i0] += qt(vol);
+    A->tmx[i0] += qt(vol * sx);
+    A->tmy[i0] += qt(vol * sy);
+    int jx = (int)px, jy = (int)py;
+    int ex = oob(nx, ny, jx, jy);
+    float hc = h[i0];
+    float h2 = ex ? (float)((double)hc - 0.002) : h[(size_t)jx * ny + jy];
+    float c_eq = (1.0f + P->entrainment * erfd[i0]) * (hc - h2);
+    if (c_eq < 0) c_eq = 0;
+    float cdiff = (c_eq - sed);
+    float ero = effD * cdiff;
+    sed += ero;
+    int32_t q = -qh(ero);
+    A->dh[i0] += q;
+    st->walk_dh += q;
+    float s1 = evap_sed(sed, P->evapRate);
+    gain += (double)s1 - (double)sed;
+    sed = s1;
+    vol = evap_vol(vol, P->evapRate);
+    if (ex) {
+      st->exits++;
+      st->export40 += q40((double)sed);
+      break;
+    }
+    A->visit[(size_t)jx * ny + jy] = 1;
+    ++age;
+  }
+  st->steps += steps;
+  st->gain40 += q40(gain);
+}
+
+/* Cascade of one visited site on the post-walk snapshot H: the reference's
+   sorted, centre-re-reading sequence (world.h:116-166) with each transfer
+   quantised once and committed as +-q. */
+static int64_t wave_cascade_site(const float* H, int nx, int ny, const orc_params* P,
+                                 int cx, int cy, int32_t* C) {
+  static const int off[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1},
+                                {0, 1},   {1, -1}, {1, 0},  {1, 1}};
+  int pidx[8];
+  float sh[8], sd[8];
+  int num = 0;
+  for (int k = 0; k < 8; ++k) {
+    int x = cx + off[k][0], y = cy + off[k][1];
+    if (oob(nx, ny, x, y)) continue;
+    float vx = (float)off[k][0], vy = (float)off[k][1];
+    pidx[num] = x * ny + y;
+    sh[num] = H[(size_t)x * ny + y];
+    sd[num] = sqrtf(vx * vx + vy * vy);
+    ++num;
+  }
+  for (int i = 1; i < num; ++i) {
+    int ti = pidx[i];
+    float th = sh[i], td = sd[i];
+    int j = i;
+    while (j > 0 && th < sh[j - 1]) {
+      pidx[j] = pidx[j - 1]; sh[j] = sh[j - 1]; sd[j] = sd[j - 1];
+      --j;
+    }
+    pidx[j] = ti; sh[j] = th; sd[j] = td;
+  }
+  size_t ci = (size_t)cx * ny + cy;
+  float hc = H[ci];
+  int64_t moves = 0;
+  for (int i = 0; i < num; ++i) {
+    float diff = hc - sh[i];
+    if (diff == 0) continue;
+    float excess;
+    if ((double)sh[i] > 0.1)
+      excess = fabsf(diff) - sd[i] * P->maxdiff * LOD;
+    else
+      excess = fabsf(diff);
+    if (excess <= 0) continue;
+    float transfer = P->settling * excess / 2.0f;
+    int32_t q = qh(transfer);
+    ++moves;
+    if (diff > 0) {
+      hc -= transfer;
+      C[ci] -= q;
+      C[pidx[i]] += q;
+    } else {
+      hc += transfer;
+      C[ci] += q;
+      C[pidx[i]] -= q;
+    }
+  }
+  return moves;
+}
+
+void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
+                    float* erfd, int nx, int ny, const orc_params* P, uint64_t seed,
+                    uint32_t terrain, uint32_t cycle, int drops, orc_stats* st,
+                    int32_t* dh_out, uint8_t* visit_out) {
+  size_t n = (size_t)nx * ny;
+  wacc A;
+  A.dh = (int32_t*)calloc(n, sizeof(int32_t));
+  A.td = (int64_t*)calloc(n, sizeof(int64_t));
+  A.tmx = (int64_t*)calloc(n, sizeof(int64_t));
+  A.tmy = (int64_t*)calloc(n, sizeof(int64_t));
+  A.visit = (uint8_t*)calloc(n, 1);
+  int32_t* C = (int32_t*)calloc(n, sizeof(int32_t));
+  float* H = (float*)malloc(n * sizeof(float));
+  orc_stats local;
+  memset(&local, 0, sizeof(local));
+
+  /* 1. walk every drop against the snapshot (order-free commits) */
+  uint64_t key = orc_cycle_key(seed, terrain, cycle);
+  for (int i = 0; i < drops; ++i) {
+    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
+    if ((double)h[cid] < 0.1) {
+      local.skipped++;
+      continue;
+    }
+    local.walked++;
+    wave_walk(h, d, mx, my, root, erfd, nx, ny, P, (float)(cid / (uint32_t)ny),
+              (float)(cid % (uint32_t)ny), &A, &local);
+  }
+  if (dh_out) memcpy(dh_out, A.dh, n * sizeof(int32_t));
+  if (visit_out) memcpy(visit_out, A.visit, n);
+
+  /* 2. apply the walk's height commits */
+  for (size_t i = 0; i < n; ++i) H[i] = h[i] + (float)A.dh[i] * HQ_INV;
+
+  /* 3. cascade stencil over visited sites, all reading H */
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      if (A.visit[(size_t)x * ny + y]) local.casc_moves += wave_cascade_site(H, nx, ny, P, x, y, C);
+
+  /* 4. apply cascade, 5. blend (world.h:81-86), refresh erf(0.4 d) */
+  for (size_t i = 0; i < n; ++i) {
+    h[i] = H[i] + (float)C[i] * HQ_INV;
+    float t_d = (float)((double)A.td[i] * TQ_INV);
+    float t_x = (float)((double)A.tmx[i] * TQ_INV);
+    float t_y = (float)((double)A.tmy[i] * TQ_INV);
+    d[i] = (1.0f - P->lrate) * d[i] + P->lrate * t_d;
+    mx[i] = (1.0f - P->lrate) * mx[i] + P->lrate * t_x;
+    my[i] = (1.0f - P->lrate) * my[i] + P->lrate * t_y;
+    erfd[i] = orc_erf(0.4f * d[i]);
+  }
+  if (st) {
+    st->steps += local.steps; st->walked += local.walked; st->skipped += local.skipped;
+    st->exits += local.exits; st->ends_age += local.ends_age; st->ends_vol += local.ends_vol;
+    st->export40 += local.export40; st->gain40 += local.gain40;
+    st->walk_dh += local.walk_dh; st->casc_moves += local.casc_moves;
+  }
+  free(A.dh); free(A.td); free(A.tmx); free(A.tmy); free(A.visit); free(C); free(H);
+}

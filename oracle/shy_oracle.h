@@ -1,0 +1,123 @@
+/*
+ * shy_oracle.h -- CPU oracle for the particle hydraulic-erosion hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (simplehydrology_amd/,
+ * include/) includes, links or loads this code; only tests/, the smoke check
+ * and the CPU-baseline leg of bench.py do, and only as the checker.
+ *
+ * Three restatements of the reference's erosion loop
+ * (source/world.h:54-168, source/water.h:58-156, source/cellpool.h:181-250,413-447):
+ *
+ *  (a) orc_seq_erode     -- sequential: every drop sees every earlier edit, the
+ *                           per-step cascade runs inline (reference semantics).
+ *  (b) orc_trace_frozen  -- one drop on an immutable map, per-step state out.
+ *  (c) orc_wave_cycle    -- wave-synchronous: all drops of a cycle read one
+ *                           snapshot and commit through integer fixed point;
+ *                           then apply, cascade stencil over visited cells, blend.
+ *                           This is the exact arithmetic the HIP path must
+ *                           reproduce bit for bit.
+ *
+ * Map arrays are x-major (idx = x*ny + y), the reference's math::flatten
+ * (source/include/math.h:11-14).  Cells in AoS form use the reference's
+ * quad::cell order (source/cellpool.h:207-220).
+ */
+#ifndef SHY_ORACLE_H
+#define SHY_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order as shy_params in include/shy.h. */
+typedef struct orc_params {
+  float evapRate, depositionRate, minVol, maxAge, entrainment, gravity,
+      momentumTransfer;          /* water.h:43-50 */
+  float lrate, maxdiff, settling; /* world.h:42-44 */
+} orc_params;
+
+/* Same field order as shy_drop_step in include/shy.h. */
+typedef struct orc_step {
+  int32_t age;     /* age before the step */
+  int32_t status;  /* 0 continues, 1 out-of-bounds exit */
+  float pos[2];    /* position after the step */
+  float speed[2];  /* speed after the step */
+  float volume;    /* after evaporation */
+  float sediment;  /* after evaporation */
+  float erosion;   /* effD*cdiff removed from the start cell */
+  float h2;        /* downstream height used for c_eq */
+} orc_step;
+
+/* Per-cycle counters.  Fixed-point sums use 2^-40 units (sed40). */
+typedef struct orc_stats {
+  int64_t steps;      /* descend() calls that reach the physics update */
+  int64_t walked;     /* spawned drops that walked */
+  int64_t skipped;    /* spawns rejected by height < 0.1 */
+  int64_t exits;      /* out-of-bounds exits (sediment leaves the map) */
+  int64_t ends_age;   /* terminated by age > maxAge */
+  int64_t ends_vol;   /* terminated by volume < minVol */
+  int64_t export40;   /* sum of sediment carried off the map, 2^-40 units */
+  int64_t gain40;     /* sum of evaporation sediment gains, 2^-40 units */
+  int64_t walk_dh;    /* sum of all walk height commits, 2^-20 units */
+  int64_t casc_moves; /* number of non-zero cascade transfers */
+} orc_stats;
+
+/* Sequential-oracle ledger, accumulated in double. */
+typedef struct orc_seq_ledger {
+  double gain;    /* evaporation gains */
+  double export_; /* sediment lost through the map edge */
+  int64_t steps, walked, skipped, exits;
+} orc_seq_ledger;
+
+void orc_default_params(orc_params* p);
+
+/* project erf: Maclaurin series (Abramowitz & Stegun 7.1.5) in double */
+float orc_erf(float x);
+void orc_erf_many(const float* x, float* y, int64_t n);
+
+/* counter-based spawn stream */
+uint64_t orc_mix64(uint64_t z);
+uint64_t orc_cycle_key(uint64_t seed, uint32_t terrain, uint32_t cycle);
+uint32_t orc_spawn_cell(uint64_t cycle_key, uint32_t drop, uint32_t ncells);
+
+/* seeded 2D Perlin-gradient fBm, normalised to [0,1] */
+void orc_fbm(uint64_t seed, int nx, int ny, int octaves, float base_freq,
+             float lacunarity, float gain, float* out);
+void orc_fbm_perm(uint64_t seed, int octave, uint8_t* perm256);
+
+/* normal of reference _normal (cellpool.h:181-204); h x-major floats */
+void orc_normal(const float* h, int nx, int ny, int x, int y, float* n3);
+
+/* (a) sequential reference: cells are AoS quad::cell (8 floats) x-major */
+void orc_seq_erode(float* cells, int nx, int ny, const orc_params* p,
+                   uint64_t seed, uint32_t terrain, uint32_t cycle, int drops,
+                   orc_seq_ledger* led);
+void orc_seq_cascade(float* cells, int nx, int ny, const orc_params* p,
+                     float x, float y);
+/* erf used by the live (sequential) path: 0 glibc erff, 1 project erf */
+void orc_set_seq_erf(int use_project);
+/* one reference descend() step on a live AoS map; returns descend()'s bool */
+int orc_seq_descend(float* cells, int nx, int ny, const orc_params* p,
+                    int32_t* age, float* pos, float* speed, float* volume,
+                    float* sediment);
+
+/* (b) single drop on a frozen map.  use_ref_erf=1: glibc erff of 0.4*d
+   (reference); 0: the erfd field (project erf, precomputed per cell). */
+int orc_trace_frozen(const float* h, const float* d, const float* mx,
+                     const float* my, const float* root, const float* erfd,
+                     int nx, int ny, const orc_params* p, float x0, float y0,
+                     int max_steps, int use_ref_erf, orc_step* out,
+                     int* end_reason);
+
+/* (c) one wave-synchronous cycle.  Updates h, d, mx, my, erfd in place. */
+void orc_wave_cycle(float* h, float* d, float* mx, float* my,
+                    const float* root, float* erfd, int nx, int ny,
+                    const orc_params* p, uint64_t seed, uint32_t terrain,
+                    uint32_t cycle, int drops, orc_stats* st,
+                    int32_t* dh_out, uint8_t* visit_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,318 @@
+"""MI355X particle hydraulic erosion: Python host over the ``libshy.so`` C ABI.
+
+The reference's interface is the static C++ API of ``World`` and ``Drop``
+(source/world.h:17-36, source/water.h:12-39).  This module mirrors it:
+
+* :class:`Params` -- the static parameters (water.h:43-50, world.h:42-44).
+* :class:`World` -- one terrain on the GPU.  ``World.erode(cycles)`` is
+  ``World::erode`` (world.h:54-88); ``World.cascade(pos)`` is
+  ``World::cascade`` (world.h:90-168); ``World.map`` exposes the post-erode
+  read accessors of ``quad::map`` (cellpool.h:413-447).
+* :class:`Drop` -- ``Drop(pos)`` / ``Drop.descend()`` (water.h:14, 58-156).
+
+Every computation runs in the HIP library; if ``libshy.so`` is missing the
+import of :func:`lib` raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libshy.so")
+
+SHY_OK, SHY_ERR_ARG, SHY_ERR_HIP, SHY_ERR_NODEV, SHY_ERR_NONFINITE, SHY_ERR_STATE = 0, -1, -2, -3, -4, -5
+FIELDS = ("height", "discharge", "momentumx", "momentumy", "discharge_track",
+          "momentumx_track", "momentumy_track", "rootdensity", "erf_discharge")
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+
+
+class ShyError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"shy error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """The reference's statics, same names and defaults."""
+
+    _fields_ = [(n, C.c_float) for n in (
+        "evapRate", "depositionRate", "minVol", "maxAge", "entrainment", "gravity",
+        "momentumTransfer", "lrate", "maxdiff", "settling")]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().shy_default_params(C.byref(self))
+        for k, v in kw.items():
+            if k not in dict(self._fields_):
+                raise TypeError(f"unknown parameter {k}")
+            setattr(self, k, v)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DropStep(C.Structure):
+    _fields_ = [("age", C.c_int32), ("status", C.c_int32), ("pos", C.c_float * 2),
+                ("speed", C.c_float * 2), ("volume", C.c_float), ("sediment", C.c_float),
+                ("erosion", C.c_float), ("h2", C.c_float)]
+
+
+STEP_DTYPE = np.dtype([("age", "<i4"), ("status", "<i4"), ("pos", "<f4", (2,)),
+                       ("speed", "<f4", (2,)), ("volume", "<f4"), ("sediment", "<f4"),
+                       ("erosion", "<f4"), ("h2", "<f4")])
+
+
+class DropState(C.Structure):
+    _fields_ = [("age", C.c_int32), ("pos", C.c_float * 2), ("speed", C.c_float * 2),
+                ("volume", C.c_float), ("sediment", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "steps", "walked", "skipped", "exits", "ends_age", "ends_vol", "export40", "gain40",
+        "walk_dh", "casc_moves")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load ``libshy.so``; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                               "or python -m simplehydrology_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, fp, i32, u32, u64 = C.c_void_p, C.POINTER(C.c_float), C.c_int, C.c_uint32, C.c_uint64
+        sig = {
+            "shy_default_params": (None, [vp]),
+            "shy_device_count": (i32, []),
+            "shy_create": (i32, [vp, i32, i32, u64, C.POINTER(vp)]),
+            "shy_destroy": (i32, [vp]),
+            "shy_last_error": (C.c_char_p, [vp]),
+            "shy_set_params": (i32, [vp, vp]),
+            "shy_get_params": (i32, [vp, vp]),
+            "shy_size": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
+            "shy_upload_cells": (i32, [vp, vp]),
+            "shy_download_cells": (i32, [vp, vp]),
+            "shy_upload_field": (i32, [vp, i32, vp]),
+            "shy_download_field": (i32, [vp, i32, vp]),
+            "shy_generate_fbm": (i32, [vp, u64, i32, C.c_float, C.c_float, C.c_float]),
+            "shy_erode": (i32, [vp, i32]),
+            "shy_erode_cycles": (i32, [vp, i32, i32, i32]),
+            "shy_trace_drop": (i32, [vp, C.c_float, C.c_float, i32, vp, C.POINTER(i32)]),
+            "shy_cascade": (i32, [vp, C.c_float, C.c_float]),
+            "shy_descend": (i32, [vp, vp, C.POINTER(i32)]),
+            "shy_set_seed": (i32, [vp, u64, u32]),
+            "shy_get_cycle": (i32, [vp, C.POINTER(u32)]),
+            "shy_get_stats": (i32, [vp, vp]),
+            "shy_reset_stats": (i32, [vp]),
+            "shy_synchronize": (i32, [vp]),
+            "shy_set_walk_block": (i32, [vp, i32]),
+            "shy_time_cycles": (i32, [vp, i32, i32, fp, fp, fp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    return lib().shy_device_count()
+
+
+def _check(ctx, rc: int):
+    if rc != SHY_OK:
+        msg = lib().shy_last_error(ctx)
+        raise ShyError(rc, msg.decode() if msg else "")
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class MapView:
+    """Read accessors of ``quad::map`` (source/cellpool.h:413-447) over a
+    snapshot downloaded from the device after ``erode``."""
+
+    def __init__(self, world: "World"):
+        self._w = world
+
+    def oob(self, p) -> bool:
+        x, y = int(p[0]), int(p[1])
+        return x < 0 or y < 0 or x >= self._w.nx or y >= self._w.ny
+
+    def height(self, p) -> float:
+        if self.oob(p):
+            return 0.0
+        return float(self._w.field("height")[int(p[0]), int(p[1])])
+
+    def discharge(self, p) -> float:
+        """``erf(0.4 * discharge)``, the value node::discharge returns."""
+        if self.oob(p):
+            return 0.0
+        return float(self._w.field("erf_discharge")[int(p[0]), int(p[1])])
+
+    def getCell(self, p):
+        if self.oob(p):
+            return None
+        c = self._w.cells()
+        return c[int(p[0]), int(p[1])]
+
+
+class World:
+    """One terrain resident in HBM; mirrors the reference ``World``."""
+
+    def __init__(self, nx: int, ny: int | None = None, seed: int = 1,
+                 params: Params | None = None):
+        ny = nx if ny is None else ny
+        self.nx, self.ny = int(nx), int(ny)
+        self._ctx = C.c_void_p()
+        p = params if params is not None else Params()
+        _check(None, lib().shy_create(C.byref(p), self.nx, self.ny, seed, C.byref(self._ctx)))
+        self.map = MapView(self)
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            lib().shy_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- parameters and seed -----------------------------------------------
+    @property
+    def params(self) -> Params:
+        p = Params()
+        _check(self._ctx, lib().shy_get_params(self._ctx, C.byref(p)))
+        return p
+
+    @params.setter
+    def params(self, p: Params):
+        _check(self._ctx, lib().shy_set_params(self._ctx, C.byref(p)))
+
+    def set_seed(self, seed: int, cycle: int = 0):
+        _check(self._ctx, lib().shy_set_seed(self._ctx, seed, cycle))
+
+    @property
+    def cycle(self) -> int:
+        c = C.c_uint32()
+        _check(self._ctx, lib().shy_get_cycle(self._ctx, C.byref(c)))
+        return c.value
+
+    # -- map transfer --------------------------------------------------------
+    def upload(self, field: str, values):
+        a = _f32(values, (self.nx, self.ny))
+        _check(self._ctx, lib().shy_upload_field(self._ctx, FIELD_ID[field], a.ctypes.data))
+
+    def field(self, field: str) -> np.ndarray:
+        out = np.empty((self.nx, self.ny), np.float32)
+        _check(self._ctx, lib().shy_download_field(self._ctx, FIELD_ID[field], out.ctypes.data))
+        return out
+
+    def upload_cells(self, cells):
+        a = _f32(cells, (self.nx, self.ny, 8))
+        _check(self._ctx, lib().shy_upload_cells(self._ctx, a.ctypes.data))
+
+    def cells(self) -> np.ndarray:
+        out = np.empty((self.nx, self.ny, 8), np.float32)
+        _check(self._ctx, lib().shy_download_cells(self._ctx, out.ctypes.data))
+        return out
+
+    def generate_fbm(self, seed: int, octaves: int = 8, base_freq: float = 0.0,
+                     lacunarity: float = 2.0, gain: float = 0.5):
+        _check(self._ctx, lib().shy_generate_fbm(self._ctx, seed, octaves, base_freq,
+                                                  lacunarity, gain))
+
+    # -- the hot path ----------------------------------------------------------
+    def erode(self, cycles: int):
+        """``World::erode(cycles)``: one cycle of ``cycles`` drop spawns."""
+        _check(self._ctx, lib().shy_erode(self._ctx, cycles))
+
+    def erode_cycles(self, n_cycles: int, drops: int, graph: bool = True):
+        _check(self._ctx, lib().shy_erode_cycles(self._ctx, n_cycles, drops, 1 if graph else 0))
+
+    def cascade(self, pos):
+        """``World::cascade(pos)`` on the live map."""
+        _check(self._ctx, lib().shy_cascade(self._ctx, float(pos[0]), float(pos[1])))
+
+    def trace(self, x: float, y: float, max_steps: int = 600) -> np.ndarray:
+        """Per-step states of one drop on the frozen current map."""
+        buf = np.zeros(max_steps, STEP_DTYPE)
+        n = C.c_int(0)
+        _check(self._ctx, lib().shy_trace_drop(self._ctx, x, y, max_steps, buf.ctypes.data,
+                                               C.byref(n)))
+        return buf[:min(n.value, max_steps)]
+
+    def drop(self, pos) -> "Drop":
+        return Drop(self, pos)
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(self._ctx, lib().shy_get_stats(self._ctx, C.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        _check(self._ctx, lib().shy_reset_stats(self._ctx))
+
+    def synchronize(self):
+        _check(self._ctx, lib().shy_synchronize(self._ctx))
+
+    def set_walk_block(self, threads: int):
+        _check(self._ctx, lib().shy_set_walk_block(self._ctx, threads))
+
+    def time_cycles(self, cycles: int, drops: int) -> dict:
+        """Run ``cycles`` erode calls with HIP events around each kernel on the
+        context's stream; returns milliseconds (total, walk, cascade)."""
+        t, w, c = C.c_float(), C.c_float(), C.c_float()
+        _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, drops, C.byref(t), C.byref(w),
+                                                C.byref(c)))
+        return {"ms_total": t.value, "ms_walk": w.value, "ms_cascade": c.value}
+
+
+class Drop:
+    """``Drop`` (source/water.h:12-39): state on the host, steps on the GPU."""
+
+    def __init__(self, world: World, pos):
+        self.world = world
+        self.state = DropState()
+        self.state.age = 0
+        self.state.pos[0], self.state.pos[1] = float(pos[0]), float(pos[1])
+        self.state.speed[0] = self.state.speed[1] = 0.0
+        self.state.volume = 1.0
+        self.state.sediment = 0.0
+
+    age = property(lambda self: self.state.age)
+    pos = property(lambda self: (self.state.pos[0], self.state.pos[1]))
+    speed = property(lambda self: (self.state.speed[0], self.state.speed[1]))
+    volume = property(lambda self: self.state.volume)
+    sediment = property(lambda self: self.state.sediment)
+
+    def descend(self) -> bool:
+        """One ``Drop::descend()`` call on the live map."""
+        alive = C.c_int(0)
+        _check(self.world._ctx, lib().shy_descend(self.world._ctx, C.byref(self.state),
+                                                  C.byref(alive)))
+        return bool(alive.value)

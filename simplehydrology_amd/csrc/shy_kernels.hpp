@@ -1,0 +1,586 @@
+// shy_kernels.hpp -- kernels of the erosion hot path (gfx950, wave64).
+//
+// One cycle (= one World::erode call, source/world.h:54-88) is two launches:
+//
+//  walk_kernel           one drop per lane.  Every drop reads the map as it
+//                        stood at the start of the cycle (heights, discharge,
+//                        momentum, erf(0.4 d), root density) and commits its
+//                        erosion/deposition (int32, 2^-20) and its discharge
+//                        and momentum tracks (int64, 2^-32) with relaxed
+//                        integer atomics, so the result does not depend on the
+//                        order in which drops run.  It marks each cell a drop
+//                        arrives at, where the reference calls World::cascade
+//                        (source/water.h:151).
+//  cascade_blend_kernel  a 2D-tiled stencil: applies the walk's height commits,
+//                        runs the reference cascade (world.h:90-168) of every
+//                        marked cell against those heights (LDS tile with a
+//                        2-cell halo), applies the quantised transfers, blends
+//                        discharge/momentum (world.h:80-86), refreshes
+//                        erf(0.4 d) and clears the other parity's
+//                        accumulators for the next cycle.
+#pragma once
+
+#include "shy_device.hpp"
+
+namespace shy {
+
+enum StatSlot {
+  ST_STEPS = 0, ST_WALKED, ST_SKIPPED, ST_EXITS, ST_ENDS_AGE, ST_ENDS_VOL, ST_EXPORT40,
+  ST_GAIN40, ST_WALK_DH, ST_CASC_MOVES, ST_COUNT
+};
+
+struct DeviceState {
+  long long stats[ST_COUNT];
+  unsigned int cycle;
+  unsigned int pad;
+};
+
+struct WalkArgs {
+  const float* __restrict__ h;
+  const float4* __restrict__ fld;  // {momentumx, momentumy, discharge, erf(0.4*discharge)}
+  const float* __restrict__ root;  // nullptr when the map has no root density
+  int32_t* dh;                     // height commits, 2^-20
+  long long* td;                   // discharge track, 2^-32
+  long long* tmx;
+  long long* tmy;
+  uint8_t* visit;                  // cascade sites
+  DeviceState* state;
+  uint64_t seed;
+  uint32_t terrain;
+  int nx, ny;
+  int drops;
+  shy_params P;
+};
+
+struct CascArgs {
+  const float* __restrict__ h_in;
+  float* __restrict__ h_out;
+  const int32_t* __restrict__ dh;
+  const uint8_t* __restrict__ visit;
+  const long long* __restrict__ td;
+  const long long* __restrict__ tmx;
+  const long long* __restrict__ tmy;
+  int32_t* dh_clear;
+  uint8_t* visit_clear;
+  long long* td_clear;
+  long long* tmx_clear;
+  long long* tmy_clear;
+  float4* fld;
+  const double* __restrict__ erf_a;
+  DeviceState* state;
+  int nx, ny;
+  shy_params P;
+};
+
+// ------------------------------------------------------------ reductions
+template <int NC>
+__device__ inline void block_add_stats(long long (&v)[NC], long long* out) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] += __shfl_xor(v[c], o, 64);
+  }
+  __shared__ long long red[16][NC];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) red[w][c] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    long long s = 0;
+    for (int k = 0; k < nw; ++k) s += red[k][threadIdx.x];
+    if (s != 0)
+      __hip_atomic_fetch_add(out + threadIdx.x, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ------------------------------------------------------------------ walk
+__device__ inline CellView load_cell(const WalkArgs& a, int x, int y, int i) {
+  CellView c;
+  const int ny = a.ny;
+  c.h = a.h[i];
+  c.xm = a.h[x > 0 ? i - ny : i];
+  c.xp = a.h[x + 1 < a.nx ? i + ny : i];
+  c.ym = a.h[y > 0 ? i - 1 : i];
+  c.yp = a.h[y + 1 < ny ? i + 1 : i];
+  const float4 f = a.fld[i];
+  c.mx = f.x;
+  c.my = f.y;
+  c.d = f.z;
+  c.erfd = f.w;
+  c.root = a.root ? a.root[i] : 0.0f;
+  return c;
+}
+
+struct LaneStats {
+  long long steps = 0, exits = 0, ends_age = 0, ends_vol = 0, export40 = 0, gain40 = 0,
+            walk_dh = 0;
+};
+
+__device__ inline void atomic_add_i32(int32_t* p, int32_t v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void atomic_add_i64(long long* p, long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One drop from spawn to termination: Drop(pos) then while(drop.descend())
+// (source/world.h:74-76, source/water.h:58-156).  TRACE records per-step
+// states and commits nothing (the map stays frozen).
+template <bool TRACE>
+__device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
+                          shy_drop_step* tr, int max_tr, int* tr_n) {
+  const shy_params& P = a.P;
+  const int nx = a.nx, ny = a.ny;
+  int ix = (int)x0, iy = (int)y0;
+  int i0 = ix * ny + iy;
+  float px = x0, py = y0, sx = 0.0f, sy = 0.0f, vol = 1.0f, sed = 0.0f;
+  int age = 0;
+  double gain = 0.0;
+  int nrec = 0;
+  CellView c = load_cell(a, ix, iy, i0);
+  for (;;) {
+    float n0, n1, n2;
+    normal_from(c, ix, iy, nx, ny, n0, n1, n2);  // water.h:70
+    (void)n1;
+    if ((float)age > P.maxAge || vol < P.minVol) {  // water.h:74-82
+      if (!TRACE) {
+        const int32_t q = quant_h(sed);
+        atomic_add_i32(&a.dh[i0], q);
+        S.walk_dh += q;
+        if ((float)age > P.maxAge) S.ends_age++; else S.ends_vol++;
+      }
+      break;
+    }
+    S.steps++;
+    float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
+    if (effD < 0) effD = 0;
+    step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);  // water.h:95-111
+    const int jx = (int)px, jy = (int)py;
+    const bool ex = off_map(jx, jy, nx, ny);
+    const int j = jx * ny + jy;
+    if (!TRACE) {  // water.h:115-117
+      atomic_add_i64(&a.td[i0], quant_t(vol));
+      atomic_add_i64(&a.tmx[i0], quant_t(vol * sx));
+      atomic_add_i64(&a.tmy[i0], quant_t(vol * sy));
+    }
+    CellView cn = c;
+    if (!ex) cn = load_cell(a, jx, jy, j);
+    const float h2 = ex ? (float)((double)c.h - 0.002) : cn.h;  // water.h:120-124
+    float c_eq = (1.0f + P.entrainment * c.erfd) * (c.h - h2);  // water.h:127
+    if (c_eq < 0) c_eq = 0;
+    const float cdiff = (c_eq - sed);
+    const float ero = effD * cdiff;
+    sed += ero;  // water.h:131-132
+    if (!TRACE) {
+      const int32_t q = -quant_h(ero);
+      atomic_add_i32(&a.dh[i0], q);
+      S.walk_dh += q;
+    }
+    const float s1 = evap_sed(sed, P.evapRate);  // water.h:135-136
+    gain += (double)s1 - (double)sed;
+    sed = s1;
+    vol = evap_vol(vol, P.evapRate);
+    if (TRACE) {
+      if (nrec < max_tr) {
+        shy_drop_step r;
+        r.age = age; r.status = ex ? 1 : 0;
+        r.pos[0] = px; r.pos[1] = py; r.speed[0] = sx; r.speed[1] = sy;
+        r.volume = vol; r.sediment = sed; r.erosion = ero; r.h2 = h2;
+        tr[nrec] = r;
+      }
+      ++nrec;
+      if (ex || nrec >= max_tr) break;
+    }
+    if (ex) {  // water.h:139-142
+      if (!TRACE) {
+        S.exits++;
+        S.export40 += quant_40((double)sed);
+      }
+      break;
+    }
+    if (!TRACE) a.visit[j] = 1;  // water.h:151 -> cascade site
+    ++age;
+    c = cn;
+    ix = jx; iy = jy; i0 = j;
+  }
+  if (!TRACE) S.gain40 += quant_40(gain);
+  if (TRACE) *tr_n = nrec;
+}
+
+__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  LaneStats S;
+  long long walked = 0, skipped = 0;
+  if (tid < a.drops) {
+    const uint32_t ncell = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint64_t key = cycle_key(a.seed, a.terrain, a.state->cycle);
+    const uint32_t cid = spawn_cell(key, (uint32_t)tid, ncell);
+    if ((double)a.h[cid] < 0.1) {  // world.h:71-72
+      skipped = 1;
+    } else {
+      walked = 1;
+      walk_drop<false>(a, (float)(cid / (uint32_t)a.ny), (float)(cid % (uint32_t)a.ny), S,
+                       nullptr, 0, nullptr);
+    }
+  }
+  long long v[ST_CASC_MOVES] = {S.steps, walked, skipped, S.exits, S.ends_age,
+                                S.ends_vol, S.export40, S.gain40, S.walk_dh};
+  block_add_stats<ST_CASC_MOVES>(v, a.state->stats);
+}
+
+__global__ void trace_kernel(WalkArgs a, float x0, float y0, shy_drop_step* out, int max_steps,
+                             int* n_out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  LaneStats S;
+  walk_drop<true>(a, x0, y0, S, out, max_steps, n_out);
+}
+
+// --------------------------------------------------------------- cascade
+// Neighbour offsets in the reference's enumeration order (world.h:94-103);
+// slot 8 is the centre.
+__constant__ int kOffX[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+__constant__ int kOffY[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+
+// World::cascade (world.h:116-166) of one site against the heights read by
+// get(dx, dy) (offsets from the centre): in-bounds neighbours sorted
+// ascending by height (stable, as libstdc++'s insertion sort for <= 16
+// elements), the centre re-read after every transfer.  emit(slot, dq) gets
+// each quantised transfer; returns the number of transfers.
+template <class Get, class Emit>
+__device__ inline int cascade_site(int cx, int cy, int nx, int ny, const shy_params& P,
+                                   float hc, Get get, Emit emit) {
+  float sh[8], sd[8];
+  int sl[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = k < 3 ? -1 : (k < 5 ? 0 : 1);
+    const int oy = (k == 0 || k == 3 || k == 5) ? -1 : ((k == 1 || k == 6) ? 0 : 1);
+    const bool in = !off_map(cx + ox, cy + oy, nx, ny);
+    const float vx = (float)ox, vy = (float)oy;
+    sh[k] = in ? get(ox, oy) : __builtin_inff();
+    sd[k] = __builtin_sqrtf(vx * vx + vy * vy);  // length(vec2(nn))
+    sl[k] = in ? k : -1;
+  }
+  // stable sort: odd-even transposition (adjacent exchanges on strict >),
+  // absent neighbours carry +inf and settle at the end
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+#pragma unroll
+    for (int i = (r & 1); i + 1 < 8; i += 2) {
+      if (sh[i + 1] < sh[i]) {
+        float t = sh[i]; sh[i] = sh[i + 1]; sh[i + 1] = t;
+        t = sd[i]; sd[i] = sd[i + 1]; sd[i + 1] = t;
+        int u = sl[i]; sl[i] = sl[i + 1]; sl[i + 1] = u;
+      }
+    }
+  }
+  int moves = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (sl[i] < 0) continue;
+    const float diff = hc - sh[i];
+    if (diff == 0) continue;
+    float excess;
+    if ((double)sh[i] > 0.1)
+      excess = __builtin_fabsf(diff) - sd[i] * P.maxdiff * kLod;
+    else
+      excess = __builtin_fabsf(diff);
+    if (excess <= 0) continue;
+    const float transfer = P.settling * excess / 2.0f;
+    ++moves;
+    if (diff > 0) {
+      hc -= transfer;
+      emit(sl[i], transfer, true);
+    } else {
+      hc += transfer;
+      emit(sl[i], transfer, false);
+    }
+  }
+  return moves;
+}
+
+constexpr int kCbY = 32;  // tile extent along y (contiguous)
+constexpr int kCbX = 8;   // tile extent along x
+constexpr int kCbThreads = kCbY * kCbX;
+
+__global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
+  __shared__ float Hs[kCbX + 4][kCbY + 4];   // post-walk heights, 2-cell halo
+  __shared__ uint8_t Vs[kCbX + 2][kCbY + 2]; // cascade sites, 1-cell halo
+  __shared__ int32_t Wd[(kCbX + 2) * (kCbY + 2)][9];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.y * kCbX, y0 = blockIdx.x * kCbY;
+  const int nx = a.nx, ny = a.ny;
+  for (int k = tid; k < (kCbX + 4) * (kCbY + 4); k += kCbThreads) {
+    const int lx = k / (kCbY + 4), ly = k % (kCbY + 4);
+    const int gx = x0 + lx - 2, gy = y0 + ly - 2;
+    float v = 0.0f;
+    if (!off_map(gx, gy, nx, ny)) {
+      const int i = gx * ny + gy;
+      v = a.h_in[i] + (float)a.dh[i] * kHQInv;
+    }
+    Hs[lx][ly] = v;
+  }
+  for (int k = tid; k < (kCbX + 2) * (kCbY + 2); k += kCbThreads) {
+    const int lx = k / (kCbY + 2), ly = k % (kCbY + 2);
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    Vs[lx][ly] = off_map(gx, gy, nx, ny) ? 0 : a.visit[gx * ny + gy];
+  }
+  __syncthreads();
+  long long moves = 0;
+  for (int k = tid; k < (kCbX + 2) * (kCbY + 2); k += kCbThreads) {
+    const int lx = k / (kCbY + 2), ly = k % (kCbY + 2);
+    int32_t* w = Wd[k];
+#pragma unroll
+    for (int s = 0; s < 9; ++s) w[s] = 0;
+    if (!Vs[lx][ly]) continue;
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    const int hx = lx + 1, hy = ly + 1;  // centre in Hs coordinates
+    // halo windows are recomputed by the neighbouring block; count own ones
+    const bool own = lx >= 1 && lx <= kCbX && ly >= 1 && ly <= kCbY;
+    const int m = cascade_site(
+        gx, gy, nx, ny, a.P, Hs[hx][hy],
+        [&](int ox, int oy) { return Hs[hx + ox][hy + oy]; },
+        [&](int slot, float transfer, bool down) {
+          const int32_t q = quant_h(transfer);
+          if (down) { w[8] -= q; w[slot] += q; } else { w[8] += q; w[slot] -= q; }
+        });
+    if (own) moves += m;
+  }
+  __syncthreads();
+  const int tx = tid / kCbY, ty = tid % kCbY;
+  const int gx = x0 + tx, gy = y0 + ty;
+  const bool mine = !off_map(gx, gy, nx, ny);
+  if (mine) {
+    const int i = gx * ny + gy;
+    int32_t tot = 0;
+    // window centred at (gx-ox, gy-oy) sees this cell at offset (ox, oy)
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+      const int ox = s == 8 ? 0 : (s < 3 ? -1 : (s < 5 ? 0 : 1));
+      const int oy = s == 8 ? 0 : ((s == 0 || s == 3 || s == 5) ? -1 : ((s == 1 || s == 6) ? 0 : 1));
+      const int wx = tx - ox + 1, wy = ty - oy + 1;
+      if (Vs[wx][wy]) tot += Wd[wx * (kCbY + 2) + wy][s];
+    }
+    a.h_out[i] = Hs[tx + 2][ty + 2] + (float)tot * kHQInv;
+    // world.h:81-86
+    const float4 f = a.fld[i];
+    const float tD = track_value(a.td[i]);
+    const float tX = track_value(a.tmx[i]);
+    const float tY = track_value(a.tmy[i]);
+    const float keep = 1.0f - a.P.lrate;
+    const float d = keep * f.z + a.P.lrate * tD;
+    const float mx = keep * f.x + a.P.lrate * tX;
+    const float my = keep * f.y + a.P.lrate * tY;
+    a.fld[i] = make_float4(mx, my, d, erf_f32(a.erf_a, 0.4f * d));
+    a.dh_clear[i] = 0;
+    a.visit_clear[i] = 0;
+    a.td_clear[i] = 0;
+    a.tmx_clear[i] = 0;
+    a.tmy_clear[i] = 0;
+  }
+  long long v[1] = {moves};
+  block_add_stats<1>(v, &a.state->stats[ST_CASC_MOVES]);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) a.state->cycle = a.state->cycle + 1u;
+}
+
+// ---------------------------------------------------------- field kernels
+__global__ void refresh_erf_kernel(float4* fld, size_t n, const double* __restrict__ erf_a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 f = fld[i];
+  f.w = erf_f32(erf_a, 0.4f * f.z);
+  fld[i] = f;
+}
+
+// ------------------------------------------------------------------ fBm
+__device__ inline float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+
+__device__ inline float grad2(int h, float x, float y) {
+  switch (h & 7) {
+    case 0: return x + y;
+    case 1: return -x + y;
+    case 2: return x - y;
+    case 3: return -x - y;
+    case 4: return x;
+    case 5: return -x;
+    case 6: return y;
+    default: return -y;
+  }
+}
+
+// Perlin gradient noise in 2D (Perlin 2002 fade curve, 8 gradient
+// directions) over a per-octave 256-entry permutation.
+__device__ inline float perlin2(const uint8_t* __restrict__ P, float x, float y) {
+  const float xf = __builtin_floorf(x), yf = __builtin_floorf(y);
+  const int xi = ((int)xf) & 255, yi = ((int)yf) & 255;
+  const float fx = x - xf, fy = y - yf;
+  const float u = fade(fx), v = fade(fy);
+  const int xj = (xi + 1) & 255;
+  const int aa = P[(P[xi] + yi) & 255], ab = P[(P[xi] + yi + 1) & 255];
+  const int ba = P[(P[xj] + yi) & 255], bb = P[(P[xj] + yi + 1) & 255];
+  const float n00 = grad2(aa, fx, fy);
+  const float n10 = grad2(ba, fx - 1.0f, fy);
+  const float n01 = grad2(ab, fx, fy - 1.0f);
+  const float n11 = grad2(bb, fx - 1.0f, fy - 1.0f);
+  const float x1 = n00 + u * (n10 - n00);
+  const float x2 = n01 + u * (n11 - n01);
+  return x1 + v * (x2 - x1);
+}
+
+__global__ void fbm_kernel(float* out, int nx, int ny, const uint8_t* __restrict__ perms,
+                           int octaves, float base, float lac, float gain) {
+  __shared__ uint8_t P[16 * 256];
+  for (int k = threadIdx.x; k < octaves * 256; k += blockDim.x) P[k] = perms[k];
+  __syncthreads();
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int x = (int)(i / (size_t)ny), y = (int)(i % (size_t)ny);
+  float s = 0.0f, amp = 1.0f, f = base;
+  for (int o = 0; o < octaves; ++o) {
+    s = s + amp * perlin2(P + o * 256, (float)x * f, (float)y * f);
+    f = f * lac;
+    amp = amp * gain;
+  }
+  out[i] = s;
+}
+
+// min/max of a float array: per-block partials then one block folds them.
+__global__ void minmax_partial_kernel(const float* __restrict__ v, size_t n, float2* part) {
+  __shared__ float smn[256], smx[256];
+  float mn = __builtin_inff(), mx = -__builtin_inff();
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float t = v[i];
+    mn = t < mn ? t : mn;
+    mx = t > mx ? t : mx;
+  }
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smn[threadIdx.x] = smn[threadIdx.x + s] < smn[threadIdx.x] ? smn[threadIdx.x + s] : smn[threadIdx.x];
+      smx[threadIdx.x] = smx[threadIdx.x + s] > smx[threadIdx.x] ? smx[threadIdx.x + s] : smx[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_float2(smn[0], smx[0]);
+}
+
+__global__ void normalize_kernel(float* v, size_t n, const float2* part, int nparts) {
+  float mn = __builtin_inff(), mx = -__builtin_inff();
+  for (int k = 0; k < nparts; ++k) {
+    const float2 p = part[k];
+    mn = p.x < mn ? p.x : mn;
+    mx = p.y > mx ? p.y : mx;
+  }
+  const float range = mx - mn;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] = range > 0.0f ? (v[i] - mn) / range : 0.0f;
+}
+
+// ------------------------------------------ single-site reference kernels
+// World::cascade(pos) on the live map (world.h:90-168).
+__device__ void cascade_live(float* h, int nx, int ny, const shy_params& P, float fx, float fy) {
+  const int cx = (int)fx, cy = (int)fy;
+  if (off_map(cx, cy, nx, ny)) return;
+  const int ci = cx * ny + cy;
+  // neighbour heights are captured before the loop (world.h:116-125); the
+  // centre is re-read from the map every iteration (world.h:138)
+  float cap[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = cx + kOffX[k], y = cy + kOffY[k];
+    cap[k] = off_map(x, y, nx, ny) ? 0.0f : h[x * ny + y];
+  }
+  cascade_site(
+      cx, cy, nx, ny, P, h[ci],
+      [&](int ox, int oy) {
+        const int s = ox < 0 ? 1 + oy : (ox == 0 ? (oy < 0 ? 3 : 4) : 6 + oy);
+        return cap[s];
+      },
+      [&](int slot, float transfer, bool down) {
+        const int n = (cx + kOffX[slot]) * ny + (cy + kOffY[slot]);
+        if (down) { h[ci] -= transfer; h[n] += transfer; } else { h[ci] += transfer; h[n] -= transfer; }
+      });
+}
+
+__global__ void cascade_live_kernel(float* h, int nx, int ny, shy_params P, float fx, float fy) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  cascade_live(h, nx, ny, P, fx, fy);
+}
+
+struct LiveArgs {
+  float* h;
+  const float4* fld;
+  const float* root;
+  long long* td;  // the tracks visible until the next cycle
+  long long* tmx;
+  long long* tmy;
+  const double* erf_a;
+  int nx, ny;
+  shy_params P;
+};
+
+// One Drop::descend() call on the live map (water.h:58-156), one thread:
+// writes go straight to the map and the cascade runs inline, as in the
+// reference.  Tracks accumulate in fixed point (2^-32).
+__global__ void descend_live_kernel(LiveArgs a, shy_drop* drop, int* alive) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  shy_drop D = *drop;
+  const shy_params& P = a.P;
+  const int nx = a.nx, ny = a.ny;
+  const int ix = (int)D.pos[0], iy = (int)D.pos[1];
+  *alive = 0;
+  if (off_map(ix, iy, nx, ny)) return;  // water.h:62-68
+  const int i0 = ix * ny + iy;
+  CellView c;
+  c.h = a.h[i0];
+  c.xm = a.h[ix > 0 ? i0 - ny : i0];
+  c.xp = a.h[ix + 1 < nx ? i0 + ny : i0];
+  c.ym = a.h[iy > 0 ? i0 - 1 : i0];
+  c.yp = a.h[iy + 1 < ny ? i0 + 1 : i0];
+  float n0, n1, n2;
+  normal_from(c, ix, iy, nx, ny, n0, n1, n2);
+  (void)n1;
+  if ((float)D.age > P.maxAge || D.volume < P.minVol) {  // water.h:74-82
+    a.h[i0] += D.sediment;
+    *drop = D;
+    return;
+  }
+  const float4 f = a.fld[i0];
+  float effD = P.depositionRate * (1.0f - a.root[i0]);
+  if (effD < 0) effD = 0;
+  float sx = D.speed[0], sy = D.speed[1], px = D.pos[0], py = D.pos[1];
+  step_motion(P, n0, n2, D.volume, f.z, f.x, f.y, sx, sy, px, py);
+  a.td[i0] += quant_t(D.volume);
+  a.tmx[i0] += quant_t(D.volume * sx);
+  a.tmy[i0] += quant_t(D.volume * sy);
+  const int jx = (int)px, jy = (int)py;
+  const bool ex = off_map(jx, jy, nx, ny);
+  const float h2 = ex ? (float)((double)a.h[i0] - 0.002) : a.h[jx * ny + jy];
+  const float E = erf_f32(a.erf_a, 0.4f * f.z);
+  float c_eq = (1.0f + P.entrainment * E) * (a.h[i0] - h2);
+  if (c_eq < 0) c_eq = 0;
+  const float cdiff = (c_eq - D.sediment);
+  D.sediment += effD * cdiff;
+  a.h[i0] -= effD * cdiff;
+  D.sediment = evap_sed(D.sediment, P.evapRate);
+  D.volume = evap_vol(D.volume, P.evapRate);
+  D.speed[0] = sx; D.speed[1] = sy; D.pos[0] = px; D.pos[1] = py;
+  if (ex) {
+    D.volume = 0.0f;
+    *drop = D;
+    return;
+  }
+  cascade_live(a.h, nx, ny, P, px, py);
+  D.age += 1;
+  *drop = D;
+  *alive = 1;
+}
+
+}  // namespace shy
