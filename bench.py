@@ -7,10 +7,12 @@ parameters, 65536 drops per cycle.  A "step" of this script is one cycle =
 one World::erode(65536) call (walk + cascade/blend).  The reported value is
 particle-steps/s over exactly --steps timed cycles, inputs resident in HBM.
 
-Prints one JSON line.  Roofline: the drop-walk kernel, HBM-bound,
-algorithmic bytes per particle-step B_STEP (reads: 5 heights, the float4
-{mx, my, d, erf(0.4 d)} and root density = 40 B; commits: int32 height,
-3 x int64 tracks, 1 visit byte = 29 B).  CPU baseline: the oracle's
+Prints one JSON line.  Roofline: the commit pass of the drop walk (the
+dominant kernel), HBM-bound, algorithmic bytes per particle-step
+B_COMMIT = 69 (reads: 5 heights, the float4 {mx, my, d, erf(0.4 d)}, the
+burst scale = 40 B; commits: int32 height, 3 x int64 tracks, 1 visit byte =
+29 B).  The count pass reads the same 36 B of map and commits one int32
+count: B_COUNT = 40.  CPU baseline: the oracle's
 sequential restatement of World::erode on the same terrain and spawn stream,
 one core, on a bounded sample of cycles.
 """
@@ -24,7 +26,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-B_STEP = 69           # algorithmic bytes per particle-step of the walk kernel
+B_COMMIT = 69         # algorithmic bytes per particle-step, commit walk
+B_COUNT = 40          # algorithmic bytes per particle-step, count walk
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
@@ -100,9 +103,10 @@ def main():
     steps = s1["steps"] - s0["steps"]
     sec = tm["ms_total"] / 1e3
     value = steps / sec
-    walk_avg_s = tm["ms_walk"] / 1e3 / args.steps
+    walk_avg_s = tm["ms_commit_walk"] / 1e3 / args.steps
     steps_per_launch = steps / args.steps
-    achieved = steps_per_launch * B_STEP / walk_avg_s / 1e9
+    achieved = steps_per_launch * B_COMMIT / walk_avg_s / 1e9
+    count_gbs = steps_per_launch * B_COUNT / (tm["ms_count_walk"] / 1e3 / args.steps) / 1e9
     out = {
         "metric": f"particle-steps/s at {n}x{n}",
         "value": value,
@@ -123,11 +127,11 @@ def main():
         "cycles_per_s": args.steps / sec,
         "particle_steps": steps,
         "host_wall_s": wall,
-        "kernel_ms_per_cycle": {"walk": tm["ms_walk"] / args.steps,
-                                "cascade_blend": tm["ms_cascade"] / args.steps},
-        "roofline": {"bound": "hbm", "kernel": "walk_kernel", "achieved": achieved,
+        "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
+        "roofline": {"bound": "hbm", "kernel": "walk_kernel<kCommit>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_step": B_STEP},
+                     "traffic": None, "bytes_per_step": B_COMMIT,
+                     "count_walk_achieved": count_gbs},
     }
     w.close()
     if not args.no_cpu:

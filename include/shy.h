@@ -128,6 +128,11 @@ int shy_download_cells(shy_ctx* ctx, float* cells8);
 int shy_upload_field(shy_ctx* ctx, int field, const float* src);
 int shy_download_field(shy_ctx* ctx, int field, float* dst);
 
+/* quad::map::normal (source/cellpool.h:181-204, 445-447) of every cell,
+   computed on the device from the current heights: 3 floats (x, y, z) per
+   cell, x-major. */
+int shy_download_normals(shy_ctx* ctx, float* xyz);
+
 /* Fill the height field with the seeded 2D Perlin-gradient fBm (normalised
    to [0,1]); stands in for quad::map::init's noise stack
    (source/cellpool.h:349-409).  base_freq <= 0 means 1/max(nx, ny). */
@@ -136,9 +141,12 @@ int shy_generate_fbm(shy_ctx* ctx, uint64_t seed, int octaves, float base_freq,
 
 /* World::erode(cycles) (source/world.h:54-88): one cycle of `drops` spawns.
    Wave-synchronous: every drop reads the map as it stood at the start of the
-   cycle and commits through fixed-point atomics; then the commits are
-   applied, the cascade runs as a stencil over visited cells and the
-   discharge/momentum fields are blended. */
+   cycle.  A counting pass finds how many walk steps start at each cell; the
+   commit pass then applies each drop's erosion scaled so that the k drops
+   through a cell commit what k drops in sequence would, through fixed-point
+   atomics (the result does not depend on drop order); the commits are
+   applied, the cascade runs as a weighted stencil over the cells drops
+   arrived at, and the discharge/momentum fields are blended. */
 int shy_erode(shy_ctx* ctx, int drops);
 /* `cycles` consecutive World::erode(drops) calls; use_graph=1 replays them
    from a captured hipGraph. */
@@ -169,11 +177,11 @@ int shy_synchronize(shy_ctx* ctx);
 int shy_set_walk_block(shy_ctx* ctx, int threads);
 
 /* Measurement (no reference counterpart): run `cycles` World::erode(drops)
-   calls eagerly with HIP events on the context's stream around every
-   kernel; returns the wall time of the whole sequence and the summed
-   durations of the walk and cascade/blend kernels, in milliseconds. */
-int shy_time_cycles(shy_ctx* ctx, int cycles, int drops, float* ms_total, float* ms_walk,
-                    float* ms_cascade);
+   calls eagerly with HIP events on the context's stream between kernels.
+   ms[0] = wall time of the whole sequence; ms[1..4] = summed durations of
+   the count walk, the scale pass, the commit walk and the cascade/blend
+   stencil, in milliseconds. */
+int shy_time_cycles(shy_ctx* ctx, int cycles, int drops, float ms[5]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,202 @@
+/*
+ * shy_world.hpp -- C++ facade with the reference's World / Drop interface
+ * (source/world.h:17-36, source/water.h:12-39) over the C ABI in shy.h.
+ *
+ * A headless host written against the reference swaps its includes for this
+ * header and keeps its loop (SimpleHydrology.cpp:313-325):
+ *
+ *   shy::World::SEED = 42;
+ *   shy::World::map.init(512, 512, shy::World::SEED);   // fBm terrain on the GPU
+ *   for (;;) { shy::World::erode(512); shy::World::map.sync(); ... }
+ *
+ * The static parameters keep the reference's names and defaults; each call
+ * into the engine forwards their current values.  Errors become shy::Error
+ * exceptions on the host side of the boundary (the C ABI itself never throws).
+ */
+#ifndef SHY_WORLD_HPP
+#define SHY_WORLD_HPP
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "shy.h"
+
+namespace shy {
+
+struct vec2 {
+  float x = 0.0f, y = 0.0f;
+  vec2() = default;
+  vec2(float a, float b) : x(a), y(b) {}
+};
+struct ivec2 {
+  int x = 0, y = 0;
+  ivec2() = default;
+  ivec2(int a, int b) : x(a), y(b) {}
+  explicit ivec2(vec2 v) : x((int)v.x), y((int)v.y) {}  // truncation, as glm
+};
+struct vec3 {
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+};
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string& msg) : std::runtime_error(msg), code(code) {}
+  int code;
+};
+
+inline void check(const shy_ctx* ctx, int rc) {
+  if (rc != SHY_OK) throw Error(rc, shy_last_error(ctx));
+}
+
+/* quad::cell (source/cellpool.h:207-220) */
+struct cell {
+  float height, discharge, momentumx, momentumy;
+  float discharge_track, momentumx_track, momentumy_track;
+  float rootdensity;
+};
+
+/* quad::map (source/cellpool.h:319-449): the device terrain plus a host
+   snapshot for the read accessors; sync() refreshes the snapshot. */
+struct map {
+  shy_ctx* ctx = nullptr;
+  int nx = 0, ny = 0;
+  std::vector<cell> cells;
+  std::vector<float> erfd, normals;
+
+  void init(int sx, int sy, unsigned int seed);  // quad::map::init (cellpool.h:323-411)
+  void release() {
+    if (ctx) shy_destroy(ctx);
+    ctx = nullptr;
+  }
+  void sync() {
+    cells.resize((size_t)nx * ny);
+    erfd.resize((size_t)nx * ny);
+    normals.resize((size_t)3 * nx * ny);
+    check(ctx, shy_download_cells(ctx, &cells[0].height));
+    check(ctx, shy_download_field(ctx, SHY_ERF_DISCHARGE, erfd.data()));
+    check(ctx, shy_download_normals(ctx, normals.data()));
+  }
+  void upload() { check(ctx, shy_upload_cells(ctx, &cells[0].height)); }
+
+  bool oob(ivec2 p) const { return p.x < 0 || p.y < 0 || p.x >= nx || p.y >= ny; }
+  cell* getCell(ivec2 p) { return oob(p) ? nullptr : &cells[(size_t)p.x * ny + p.y]; }
+  float height(ivec2 p) const { return oob(p) ? 0.0f : cells[(size_t)p.x * ny + p.y].height; }
+  /* node::discharge: erf(0.4 * discharge) (cellpool.h:242-244) */
+  float discharge(ivec2 p) const { return oob(p) ? 0.0f : erfd[(size_t)p.x * ny + p.y]; }
+  vec3 normal(ivec2 p) const {
+    const float* n = &normals[3 * ((size_t)p.x * ny + p.y)];
+    return vec3{n[0], n[1], n[2]};
+  }
+};
+
+struct Drop;
+
+class World {
+ public:
+  inline static unsigned int SEED = 1;  // world.h:38
+  inline static shy::map map;           // world.h:40
+  inline static float lrate = 0.1f;     // world.h:42-44
+  inline static float maxdiff = 0.01f;
+  inline static float settling = 0.8f;
+
+  static void erode(int cycles);  // world.h:54-88
+  static void cascade(vec2 pos);  // world.h:90-168
+  static shy_params params();
+  static void push_params();
+};
+
+struct Drop {
+  explicit Drop(vec2 p) : pos(p) {}  // water.h:14
+  int age = 0;
+  vec2 pos;
+  vec2 speed = vec2(0.0f, 0.0f);
+  float volume = 1.0f;
+  float sediment = 0.0f;
+
+  inline static float evapRate = 0.001f;  // water.h:43-50
+  inline static float depositionRate = 0.1f;
+  inline static float minVol = 0.01f;
+  inline static float maxAge = 500.0f;
+  inline static float entrainment = 10.0f;
+  inline static float gravity = 1.0f;
+  inline static float momentumTransfer = 1.0f;
+
+  bool descend();  // water.h:58-156, on the live device map
+};
+
+inline shy_params World::params() {
+  shy_params p;
+  p.evapRate = Drop::evapRate;
+  p.depositionRate = Drop::depositionRate;
+  p.minVol = Drop::minVol;
+  p.maxAge = Drop::maxAge;
+  p.entrainment = Drop::entrainment;
+  p.gravity = Drop::gravity;
+  p.momentumTransfer = Drop::momentumTransfer;
+  p.lrate = lrate;
+  p.maxdiff = maxdiff;
+  p.settling = settling;
+  return p;
+}
+
+inline void World::push_params() {
+  // forward the statics only when they changed: a new parameter set
+  // invalidates captured graphs
+  static shy_params last{};
+  static const shy_ctx* last_ctx = nullptr;
+  const shy_params p = params();
+  const bool same = last_ctx == map.ctx && p.evapRate == last.evapRate &&
+                    p.depositionRate == last.depositionRate && p.minVol == last.minVol &&
+                    p.maxAge == last.maxAge && p.entrainment == last.entrainment &&
+                    p.gravity == last.gravity && p.momentumTransfer == last.momentumTransfer &&
+                    p.lrate == last.lrate && p.maxdiff == last.maxdiff && p.settling == last.settling;
+  if (same) return;
+  check(map.ctx, shy_set_params(map.ctx, &p));
+  last = p;
+  last_ctx = map.ctx;
+}
+
+inline void map::init(int sx, int sy, unsigned int seed) {
+  release();
+  nx = sx;
+  ny = sy;
+  const shy_params p = World::params();
+  check(nullptr, shy_create(&p, nx, ny, seed, &ctx));
+  check(ctx, shy_generate_fbm(ctx, seed, 8, 0.0f, 2.0f, 0.5f));
+  sync();
+}
+
+inline void World::erode(int cycles) {
+  push_params();
+  check(map.ctx, shy_erode(map.ctx, cycles));
+}
+
+inline void World::cascade(vec2 pos) {
+  push_params();
+  check(map.ctx, shy_cascade(map.ctx, pos.x, pos.y));
+}
+
+inline bool Drop::descend() {
+  World::push_params();
+  shy_drop d;
+  d.age = age;
+  d.pos[0] = pos.x;
+  d.pos[1] = pos.y;
+  d.speed[0] = speed.x;
+  d.speed[1] = speed.y;
+  d.volume = volume;
+  d.sediment = sediment;
+  int alive = 0;
+  check(World::map.ctx, shy_descend(World::map.ctx, &d, &alive));
+  age = d.age;
+  pos = vec2(d.pos[0], d.pos[1]);
+  speed = vec2(d.speed[0], d.speed[1]);
+  volume = d.volume;
+  sediment = d.sediment;
+  return alive != 0;
+}
+
+}  // namespace shy
+
+#endif

@@ -108,6 +108,8 @@ def lib():
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_seq_descend.restype = C.c_int
         L.orc_set_seq_erf.argtypes = [C.c_int]
+        L.orc_burst_scale.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float]
+        L.orc_burst_scale.restype = C.c_float
         L.orc_trace_frozen.argtypes = [fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, P, C.c_float,
                                        C.c_float, C.c_int, C.c_int, C.c_void_p,
                                        C.POINTER(C.c_int)]

@@ -479,15 +479,44 @@ int orc_trace_frozen(const float* h, const float* d, const float* mx, const floa
 }
 
 /* --------------------------------------------------- (c) wave-synchronous */
+/*
+ * One cycle = one wave.  Every drop reads the map as it stood when the cycle
+ * began, so a drop cannot see the edits of any drop of the same cycle, its own
+ * included.  Committing each drop's full erosion against that stale state
+ * overshoots without bound: one drop moves a cell a fraction
+ * g = effD*(1 + entrainment*erf(0.4 d)) (about 1.1 on rivers) of the way to
+ * its equilibrium, so k >= 2 drops through a cell on the same snapshot
+ * overcorrect, and drops pinned in pits re-erode the same cells up to maxAge
+ * times.  The wave therefore runs in two passes over identical trajectories
+ * (trajectories depend only on the snapshot):
+ *
+ *  pass A  counts k(c), the walk steps that start at cell c;
+ *  scale   s(c) = (1 - (1-gamma)^k) / (gamma k), gamma = min(1, g(c)):
+ *          k drops of identical inputs applied one after another under the
+ *          linear feedback of water.h:127-132 take (1-(1-gamma)^k)/gamma times
+ *          one drop's erosion, so the k wave commits sum to that burst total;
+ *  pass B  commits erosion/deposition q = rint(2^20 effD*cdiff*s(c)) and moves
+ *          the drop's sediment by the same q 2^-20, the full terminal deposit, the
+ *          tracks, and marks arrival cells as cascade sites;
+ *  cascade the reference window sequence (world.h:116-166) at every site,
+ *          on the heights after pass B, each window's transfers divided by
+ *          m(c) = max over its cells of the number of site windows covering
+ *          that cell (a Jacobi sweep that cannot overshoot where windows
+ *          overlap);
+ *  blend   world.h:81-86.
+ */
 typedef struct wacc {
   int32_t* dh;
   int64_t *td, *tmx, *tmy;
   uint8_t* visit;
+  int32_t* count;
+  const float* scale; /* NULL in pass A */
 } wacc;
 
 static void wave_walk(const float* h, const float* d, const float* mx, const float* my,
                       const float* root, const float* erfd, int nx, int ny,
                       const orc_params* P, float x0, float y0, wacc* A, orc_stats* st) {
+  const int commit = A->scale != NULL;
   int32_t age = 0;
   float px = x0, py = y0, sx = 0.0f, sy = 0.0f, vol = 1.0f, sed = 0.0f;
   double gain = 0.0;
@@ -497,27 +526,25 @@ static void wave_walk(const float* h, const float* d, const float* mx, const flo
     size_t i0 = (size_t)ix * ny + iy; /* in bounds: spawn or checked below */
     float n[3];
     normal_st(h, 1, nx, ny, ix, iy, n);
-    if ((float)age > P->maxAge) {
-      int32_t q = qh(sed);
-      A->dh[i0] += q;
-      st->walk_dh += q;
-      st->ends_age++;
-      break;
-    }
-    if (vol < P->minVol) {
-      int32_t q = qh(sed);
-      A->dh[i0] += q;
-      st->walk_dh += q;
-      st->ends_vol++;
+    if ((float)age > P->maxAge || vol < P->minVol) {
+      if (commit) {
+        int32_t q = qh(sed);
+        A->dh[i0] += q;
+        st->walk_dh += q;
+        if ((float)age > P->maxAge) st->ends_age++; else st->ends_vol++;
+      }
       break;
     }
     ++steps;
+    if (!commit) A->count[i0] += 1;
     float effD = P->depositionRate * (1.0f - root[i0]);
     if (effD < 0) effD = 0;
     step_motion(P, n, vol, d[i0], mx[i0], my[i0], &sx, &sy, &px, &py);
-    A->td[i0] += qt(vol);
-    A->tmx[i0] += qt(vol * sx);
-    A->tmy[i0] += qt(vol * sy);
+    if (commit) {
+      A->td[i0] += qt(vol);
+      A->tmx[i0] += qt(vol * sx);
+      A->tmy[i0] += qt(vol * sy);
+    }
     int jx = (int)px, jy = (int)py;
     int ex = oob(nx, ny, jx, jy);
     float hc = h[i0];
@@ -525,32 +552,60 @@ static void wave_walk(const float* h, const float* d, const float* mx, const flo
     float c_eq = (1.0f + P->entrainment * erfd[i0]) * (hc - h2);
     if (c_eq < 0) c_eq = 0;
     float cdiff = (c_eq - sed);
-    float ero = effD * cdiff;
-    sed += ero;
-    int32_t q = -qh(ero);
-    A->dh[i0] += q;
-    st->walk_dh += q;
+    if (commit) {
+      /* one quantised amount moves between map and drop, so the ledger
+         closes in fixed point */
+      int32_t q = qh((effD * cdiff) * A->scale[i0]);
+      sed += (float)q * HQ_INV;
+      A->dh[i0] -= q;
+      st->walk_dh -= q;
+    } else {
+      sed += effD * cdiff;
+    }
     float s1 = evap_sed(sed, P->evapRate);
     gain += (double)s1 - (double)sed;
     sed = s1;
     vol = evap_vol(vol, P->evapRate);
     if (ex) {
-      st->exits++;
-      st->export40 += q40((double)sed);
+      if (commit) {
+        st->exits++;
+        st->export40 += q40((double)sed);
+      }
       break;
     }
-    A->visit[(size_t)jx * ny + jy] = 1;
+    if (commit) A->visit[(size_t)jx * ny + jy] = 1;
     ++age;
   }
-  st->steps += steps;
-  st->gain40 += q40(gain);
+  if (commit) {
+    st->steps += steps;
+    st->gain40 += q40(gain);
+  }
 }
 
-/* Cascade of one visited site on the post-walk snapshot H: the reference's
-   sorted, centre-re-reading sequence (world.h:116-166) with each transfer
-   quantised once and committed as +-q. */
+/* r^k by binary exponentiation, least significant bit first */
+static float powi_f32(float r, int32_t k) {
+  float res = 1.0f, b = r;
+  while (k > 0) {
+    if (k & 1) res = res * b;
+    b = b * b;
+    k >>= 1;
+  }
+  return res;
+}
+
+float orc_burst_scale(int32_t k, float effD, float entrainment, float erfd) {
+  if (k <= 1) return 1.0f;
+  float g = effD * (1.0f + entrainment * erfd);
+  if (!(g > 0.0f)) return 1.0f;
+  if (g > 1.0f) g = 1.0f;
+  return (1.0f - powi_f32(1.0f - g, k)) / (g * (float)k);
+}
+
+/* Cascade of one site on the post-walk heights H: the reference's sorted,
+   centre-re-reading window (world.h:116-166); each transfer divided by the
+   window weight m and quantised once, committed as +-q. */
 static int64_t wave_cascade_site(const float* H, int nx, int ny, const orc_params* P,
-                                 int cx, int cy, int32_t* C) {
+                                 int cx, int cy, int m, int32_t* C) {
   static const int off[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1},
                                 {0, 1},   {1, -1}, {1, 0},  {1, 1}};
   int pidx[8];
@@ -588,7 +643,7 @@ static int64_t wave_cascade_site(const float* H, int nx, int ny, const orc_param
       excess = fabsf(diff);
     if (excess <= 0) continue;
     float transfer = P->settling * excess / 2.0f;
-    int32_t q = qh(transfer);
+    int32_t q = qh(transfer / (float)m);
     ++moves;
     if (diff > 0) {
       hc -= transfer;
@@ -614,13 +669,30 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
   A.tmx = (int64_t*)calloc(n, sizeof(int64_t));
   A.tmy = (int64_t*)calloc(n, sizeof(int64_t));
   A.visit = (uint8_t*)calloc(n, 1);
+  A.count = (int32_t*)calloc(n, sizeof(int32_t));
+  float* scale = (float*)malloc(n * sizeof(float));
   int32_t* C = (int32_t*)calloc(n, sizeof(int32_t));
+  int32_t* wc = (int32_t*)calloc(n, sizeof(int32_t));
   float* H = (float*)malloc(n * sizeof(float));
   orc_stats local;
   memset(&local, 0, sizeof(local));
-
-  /* 1. walk every drop against the snapshot (order-free commits) */
   uint64_t key = orc_cycle_key(seed, terrain, cycle);
+
+  /* pass A: visit counts (no commits) */
+  A.scale = NULL;
+  for (int i = 0; i < drops; ++i) {
+    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
+    if ((double)h[cid] < 0.1) continue;
+    wave_walk(h, d, mx, my, root, erfd, nx, ny, P, (float)(cid / (uint32_t)ny),
+              (float)(cid % (uint32_t)ny), &A, &local);
+  }
+  for (size_t c = 0; c < n; ++c) {
+    float effD = P->depositionRate * (1.0f - root[c]);
+    if (effD < 0) effD = 0;
+    scale[c] = orc_burst_scale(A.count[c], effD, P->entrainment, erfd[c]);
+  }
+  /* pass B: scaled commits */
+  A.scale = scale;
   for (int i = 0; i < drops; ++i) {
     uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
     if ((double)h[cid] < 0.1) {
@@ -634,15 +706,29 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
   if (dh_out) memcpy(dh_out, A.dh, n * sizeof(int32_t));
   if (visit_out) memcpy(visit_out, A.visit, n);
 
-  /* 2. apply the walk's height commits */
+  /* apply the walk's height commits */
   for (size_t i = 0; i < n; ++i) H[i] = h[i] + (float)A.dh[i] * HQ_INV;
 
-  /* 3. cascade stencil over visited sites, all reading H */
+  /* cascade: weighted Jacobi sweep over the sites */
   for (int x = 0; x < nx; ++x)
-    for (int y = 0; y < ny; ++y)
-      if (A.visit[(size_t)x * ny + y]) local.casc_moves += wave_cascade_site(H, nx, ny, P, x, y, C);
+    for (int y = 0; y < ny; ++y) {
+      if (!A.visit[(size_t)x * ny + y]) continue;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          if (!oob(nx, ny, x + a, y + b)) wc[(size_t)(x + a) * ny + (y + b)] += 1;
+    }
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      if (!A.visit[(size_t)x * ny + y]) continue;
+      int m = 1;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          if (!oob(nx, ny, x + a, y + b) && wc[(size_t)(x + a) * ny + (y + b)] > m)
+            m = wc[(size_t)(x + a) * ny + (y + b)];
+      local.casc_moves += wave_cascade_site(H, nx, ny, P, x, y, m, C);
+    }
 
-  /* 4. apply cascade, 5. blend (world.h:81-86), refresh erf(0.4 d) */
+  /* apply the cascade, blend (world.h:81-86), refresh erf(0.4 d) */
   for (size_t i = 0; i < n; ++i) {
     h[i] = H[i] + (float)C[i] * HQ_INV;
     float t_d = (float)((double)A.td[i] * TQ_INV);
@@ -659,5 +745,6 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
     st->export40 += local.export40; st->gain40 += local.gain40;
     st->walk_dh += local.walk_dh; st->casc_moves += local.casc_moves;
   }
-  free(A.dh); free(A.td); free(A.tmx); free(A.tmy); free(A.visit); free(C); free(H);
+  free(A.dh); free(A.td); free(A.tmx); free(A.tmy); free(A.visit); free(A.count);
+  free(scale); free(C); free(wc); free(H);
 }

@@ -12,8 +12,9 @@
  *                           per-step cascade runs inline (reference semantics).
  *  (b) orc_trace_frozen  -- one drop on an immutable map, per-step state out.
  *  (c) orc_wave_cycle    -- wave-synchronous: all drops of a cycle read one
- *                           snapshot and commit through integer fixed point;
- *                           then apply, cascade stencil over visited cells, blend.
+ *                           snapshot; a counting pass, then burst-scaled
+ *                           commits in integer fixed point; then apply, a
+ *                           weighted cascade stencil over arrival cells, blend.
  *                           This is the exact arithmetic the HIP path must
  *                           reproduce bit for bit.
  *
@@ -109,6 +110,9 @@ int orc_trace_frozen(const float* h, const float* d, const float* mx,
                      int nx, int ny, const orc_params* p, float x0, float y0,
                      int max_steps, int use_ref_erf, orc_step* out,
                      int* end_reason);
+
+/* burst scale of a cell visited by k walk steps in one wave */
+float orc_burst_scale(int32_t k, float effD, float entrainment, float erfd);
 
 /* (c) one wave-synchronous cycle.  Updates h, d, mx, my, erfd in place. */
 void orc_wave_cycle(float* h, float* d, float* mx, float* my,

@@ -116,7 +116,8 @@ def lib():
             "shy_reset_stats": (i32, [vp]),
             "shy_synchronize": (i32, [vp]),
             "shy_set_walk_block": (i32, [vp, i32]),
-            "shy_time_cycles": (i32, [vp, i32, i32, fp, fp, fp]),
+            "shy_time_cycles": (i32, [vp, i32, i32, fp]),
+            "shy_download_normals": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -164,6 +165,10 @@ class MapView:
         if self.oob(p):
             return 0.0
         return float(self._w.field("erf_discharge")[int(p[0]), int(p[1])])
+
+    def normal(self, p):
+        """``quad::_normal`` (cellpool.h:181-204), evaluated on the device."""
+        return self._w.normals()[int(p[0]), int(p[1])]
 
     def getCell(self, p):
         if self.oob(p):
@@ -232,6 +237,11 @@ class World:
         _check(self._ctx, lib().shy_download_field(self._ctx, FIELD_ID[field], out.ctypes.data))
         return out
 
+    def normals(self) -> np.ndarray:
+        out = np.empty((self.nx, self.ny, 3), np.float32)
+        _check(self._ctx, lib().shy_download_normals(self._ctx, out.ctypes.data))
+        return out
+
     def upload_cells(self, cells):
         a = _f32(cells, (self.nx, self.ny, 8))
         _check(self._ctx, lib().shy_upload_cells(self._ctx, a.ctypes.data))
@@ -284,12 +294,12 @@ class World:
         _check(self._ctx, lib().shy_set_walk_block(self._ctx, threads))
 
     def time_cycles(self, cycles: int, drops: int) -> dict:
-        """Run ``cycles`` erode calls with HIP events around each kernel on the
-        context's stream; returns milliseconds (total, walk, cascade)."""
-        t, w, c = C.c_float(), C.c_float(), C.c_float()
-        _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, drops, C.byref(t), C.byref(w),
-                                                C.byref(c)))
-        return {"ms_total": t.value, "ms_walk": w.value, "ms_cascade": c.value}
+        """Run ``cycles`` erode calls with HIP events between kernels on the
+        context's stream; returns milliseconds per kernel, summed."""
+        ms = (C.c_float * 5)()
+        _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, drops, ms))
+        return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scale": ms[2],
+                "ms_commit_walk": ms[3], "ms_cascade": ms[4]}
 
 
 class Drop:

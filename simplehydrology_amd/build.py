@@ -37,5 +37,20 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+DEMO = os.path.join(HERE, "world_demo")
+
+
+def build_demo() -> str:
+    """The C++ facade example (include/shy_world.hpp) linked against libshy.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", DEMO + ".tmp",
+           os.path.join(CSRC, "world_demo.cpp"), "-L" + HERE, "-lshy", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    os.replace(DEMO + ".tmp", DEMO)
+    return DEMO
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_demo())

@@ -7,6 +7,8 @@
 //   dh[2]       int32   walk height commits (2^-20), ping-pong
 //   visit[2]    uint8   cascade sites, ping-pong
 //   trk[2][3]   int64   discharge / momentum tracks (2^-32), ping-pong
+//   count       int32   walk steps per start cell (count pass; cleared by scale)
+//   scale       float   burst scale per cell (written by scale, read by commit)
 // Parity p = cycle & 1 selects the buffers a cycle writes; the cascade/blend
 // kernel of cycle p clears parity p^1 for the next cycle, so a cycle needs no
 // memset and two cycles form a replayable hipGraph.
@@ -36,6 +38,8 @@ struct shy_ctx {
   int32_t* dh[2] = {nullptr, nullptr};
   uint8_t* visit[2] = {nullptr, nullptr};
   long long* trk[2] = {nullptr, nullptr};  // 3*n each: d, mx, my
+  int32_t* count = nullptr;
+  float* scale = nullptr;
   DeviceState* state = nullptr;
   double* erf_a = nullptr;
   float2* part = nullptr;  // min/max partials
@@ -116,6 +120,8 @@ WalkArgs walk_args(shy_ctx* c, int drops) {
   a.tmx = c->trk[p] + c->n;
   a.tmy = c->trk[p] + 2 * c->n;
   a.visit = c->visit[p];
+  a.count = c->count;
+  a.scale = nullptr;
   a.state = c->state;
   a.seed = c->seed;
   a.terrain = 0;
@@ -126,7 +132,9 @@ WalkArgs walk_args(shy_ctx* c, int drops) {
   return a;
 }
 
-int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t ev_mid = nullptr) {
+// ev, when given, receives events after the count walk, the scale pass and
+// the commit walk.
+int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t* ev = nullptr) {
   WalkArgs w = walk_args(c, drops);
   w.h = c->h[p];
   w.dh = c->dh[p];
@@ -137,10 +145,25 @@ int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t ev_mid = nullptr) {
   if (drops > 0) {
     const int bs = c->walk_block;
     const int grid = (drops + bs - 1) / bs;
-    walk_kernel<<<grid, bs, 0, c->stream>>>(w);
+    walk_kernel<kCount><<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
-  if (ev_mid) HIPCHK(c, hipEventRecord(ev_mid, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  if (drops > 0) {
+    const size_t sgrid = (c->n + 255) / 256;
+    scale_kernel<<<sgrid, 256, 0, c->stream>>>(c->count, c->scale, c->fld,
+                                              c->has_root ? c->root : nullptr, c->n, c->P);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  if (drops > 0) {
+    const int bs = c->walk_block;
+    const int grid = (drops + bs - 1) / bs;
+    w.scale = c->scale;
+    walk_kernel<kCommit><<<grid, bs, 0, c->stream>>>(w);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   CascArgs a;
   const int q = p ^ 1;
   a.h_in = c->h[p];
@@ -221,7 +244,7 @@ int shy_destroy(shy_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->dh[0], c->dh[1], c->visit[0],
-                  c->visit[1], c->trk[0], c->trk[1], c->state, c->erf_a, c->part,
+                  c->visit[1], c->trk[0], c->trk[1], c->count, c->scale, c->state, c->erf_a, c->part,
                   c->perms, c->trace_buf, c->trace_n, c->drop_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -268,6 +291,8 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
       return bail(e, "hipMalloc trk");
   }
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
+  if ((e = hipMalloc(&c->count, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc count");
+  if ((e = hipMalloc(&c->scale, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc scale");
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->state, sizeof(DeviceState))) != hipSuccess) return bail(e, "hipMalloc state");
   if ((e = hipMalloc(&c->erf_a, kErfTerms * sizeof(double))) != hipSuccess)
@@ -283,6 +308,7 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
     hipMemsetAsync(c->trk[k], 0, 3 * n * sizeof(long long), c->stream);
   }
   hipMemsetAsync(c->fld, 0, n * sizeof(float4), c->stream);
+  hipMemsetAsync(c->count, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->root, 0, n * sizeof(float), c->stream);
   DeviceState st{};
   st.cycle = 0;
@@ -323,10 +349,15 @@ int shy_set_seed(shy_ctx* c, uint64_t seed, uint32_t cycle) {
   c->seed = seed;
   // parity follows the cycle number; move the live buffers if it flips
   if ((cycle & 1u) != (c->cycle & 1u)) {
+    // parity p: h[p] live, dh/visit/trk[p] zero accumulators, trk[p^1] the
+    // last cycle's tracks.  Move everything to parity q = p^1.
     const int p = parity(c), q = p ^ 1;
-    HIPCHK(c, hipMemcpyAsync(c->h[q], c->h[p], c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->trk[q], c->trk[p], 3 * c->n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->trk[p], 0, 3 * c->n * sizeof(long long), c->stream));
+    const size_t n = c->n;
+    HIPCHK(c, hipMemcpyAsync(c->h[q], c->h[p], n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->trk[p], c->trk[q], 3 * n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->trk[q], 0, 3 * n * sizeof(long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dh[q], 0, n * sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->visit[q], 0, n, c->stream));
   }
   c->cycle = cycle;
   HIPCHK(c, hipMemcpyAsync(&c->state->cycle, &cycle, sizeof(cycle), hipMemcpyHostToDevice, c->stream));
@@ -435,6 +466,22 @@ int shy_download_field(shy_ctx* c, int field, float* dst) {
       HIPCHK(c, hipMemcpy(dst, c->root, n * sizeof(float), hipMemcpyDeviceToHost));
       break;
   }
+  return SHY_OK;
+}
+
+int shy_download_normals(shy_ctx* c, float* xyz) {
+  if (!c || !xyz) return fail(c, SHY_ERR_ARG, "shy_download_normals: NULL argument");
+  hipSetDevice(c->device);
+  float* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, 3 * c->n * sizeof(float)));
+  const size_t grid = (c->n + 255) / 256;
+  normals_kernel<<<grid, 256, 0, c->stream>>>(c->h[parity(c)], c->nx, c->ny, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(xyz, d, 3 * c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("shy_download_normals: ") + hipGetErrorString(e));
   return SHY_OK;
 }
 
@@ -648,12 +695,13 @@ int shy_synchronize(shy_ctx* c) {
   return SHY_OK;
 }
 
-int shy_time_cycles(shy_ctx* c, int cycles, int drops, float* ms_total, float* ms_walk,
-                    float* ms_cascade) {
-  if (!c || !ms_total || !ms_walk || !ms_cascade || cycles < 1 || drops < 0)
+int shy_time_cycles(shy_ctx* c, int cycles, int drops, float* ms) {
+  if (!c || !ms || cycles < 1 || drops < 0)
     return fail(c, SHY_ERR_ARG, "shy_time_cycles: bad argument");
   hipSetDevice(c->device);
-  std::vector<hipEvent_t> ev(2 * (size_t)cycles + 1, nullptr);
+  // per cycle: start, after count walk, after scale, after commit walk; + end
+  const size_t ne = 4 * (size_t)cycles + 1;
+  std::vector<hipEvent_t> ev(ne, nullptr);
   auto cleanup = [&]() {
     for (hipEvent_t e : ev)
       if (e) hipEventDestroy(e);
@@ -666,29 +714,29 @@ int shy_time_cycles(shy_ctx* c, int cycles, int drops, float* ms_total, float* m
   }
   hipError_t e = hipEventRecord(ev[0], c->stream);
   for (int k = 0; k < cycles && e == hipSuccess; ++k) {
-    int r = launch_cycle(c, drops, parity(c), ev[2 * k + 1]);
+    int r = launch_cycle(c, drops, parity(c), &ev[4 * k + 1]);
     if (r) {
       cleanup();
       return r;
     }
     c->cycle += 1;
-    e = hipEventRecord(ev[2 * k + 2], c->stream);
+    e = hipEventRecord(ev[4 * k + 4], c->stream);
   }
   if (e == hipSuccess) e = hipEventSynchronize(ev.back());
-  float tot = 0.0f, tw = 0.0f, tc = 0.0f;
+  float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int k = 0; k < cycles && e == hipSuccess; ++k) {
-    float a = 0.0f, b = 0.0f;
-    e = hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]);
-    tw += a;
-    tc += b;
+    for (int s = 0; s < 4 && e == hipSuccess; ++s) {
+      float t = 0.0f;
+      e = hipEventElapsedTime(&t, ev[4 * k + s], ev[4 * k + s + 1]);
+      part[s] += t;
+    }
   }
+  float tot = 0.0f;
   if (e == hipSuccess) e = hipEventElapsedTime(&tot, ev[0], ev.back());
   cleanup();
   if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("shy_time_cycles: ") + hipGetErrorString(e));
-  *ms_total = tot;
-  *ms_walk = tw;
-  *ms_cascade = tc;
+  ms[0] = tot;
+  for (int s = 0; s < 4; ++s) ms[1 + s] = part[s];
   return SHY_OK;
 }
 

@@ -1,22 +1,31 @@
 // shy_kernels.hpp -- kernels of the erosion hot path (gfx950, wave64).
 //
-// One cycle (= one World::erode call, source/world.h:54-88) is two launches:
+// One cycle (= one World::erode call, source/world.h:54-88) is one wave of
+// drops that all read the map as it stood when the cycle began, in four
+// launches:
 //
-//  walk_kernel           one drop per lane.  Every drop reads the map as it
-//                        stood at the start of the cycle (heights, discharge,
-//                        momentum, erf(0.4 d), root density) and commits its
-//                        erosion/deposition (int32, 2^-20) and its discharge
-//                        and momentum tracks (int64, 2^-32) with relaxed
+//  walk_kernel<kCount>   one drop per lane walks its trajectory and counts,
+//                        with relaxed int32 atomics, the steps that start at
+//                        each cell.
+//  scale_kernel          per cell, the burst scale s = (1-(1-gamma)^k)/(gamma k)
+//                        with gamma = min(1, effD (1 + entrainment erf(0.4 d))):
+//                        k drops through one cell on the same snapshot then
+//                        commit together what k drops in sequence would
+//                        (water.h:127-132 as a linear feedback), which keeps the
+//                        wave stable where crowded cells would overshoot.
+//  walk_kernel<kCommit>  the same trajectories again; erosion/deposition
+//                        effD*cdiff*s, quantised once to int32 (2^-20) and
+//                        moved between map and drop as that integer, and the discharge and
+//                        momentum tracks (int64, 2^-32) go through relaxed
 //                        integer atomics, so the result does not depend on the
-//                        order in which drops run.  It marks each cell a drop
-//                        arrives at, where the reference calls World::cascade
-//                        (source/water.h:151).
-//  cascade_blend_kernel  a 2D-tiled stencil: applies the walk's height commits,
-//                        runs the reference cascade (world.h:90-168) of every
-//                        marked cell against those heights (LDS tile with a
-//                        2-cell halo), applies the quantised transfers, blends
-//                        discharge/momentum (world.h:80-86), refreshes
-//                        erf(0.4 d) and clears the other parity's
+//                        order in which drops run; arrival cells (where the
+//                        reference calls World::cascade, water.h:151) are marked.
+//  cascade_blend_kernel  2D-tiled stencil: applies the walk's commits, runs the
+//                        reference cascade (world.h:90-168) of every marked
+//                        cell as a weighted Jacobi sweep (LDS tile, 2-cell
+//                        height halo, 3-cell mark halo), applies the quantised
+//                        transfers, blends discharge/momentum (world.h:80-86),
+//                        refreshes erf(0.4 d) and clears the other parity's
 //                        accumulators for the next cycle.
 #pragma once
 
@@ -44,6 +53,8 @@ struct WalkArgs {
   long long* tmx;
   long long* tmy;
   uint8_t* visit;                  // cascade sites
+  int32_t* count;                  // walk steps starting at each cell (count pass)
+  const float* __restrict__ scale; // burst scale per cell (commit pass)
   DeviceState* state;
   uint64_t seed;
   uint32_t terrain;
@@ -111,6 +122,7 @@ __device__ inline CellView load_cell(const WalkArgs& a, int x, int y, int i) {
   c.d = f.z;
   c.erfd = f.w;
   c.root = a.root ? a.root[i] : 0.0f;
+  c.scale = a.scale ? a.scale[i] : 1.0f;
   return c;
 }
 
@@ -126,12 +138,17 @@ __device__ inline void atomic_add_i64(long long* p, long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+enum WalkMode { kTrace = 0, kCount = 1, kCommit = 2 };
+
 // One drop from spawn to termination: Drop(pos) then while(drop.descend())
-// (source/world.h:74-76, source/water.h:58-156).  TRACE records per-step
-// states and commits nothing (the map stays frozen).
-template <bool TRACE>
+// (source/world.h:74-76, source/water.h:58-156).  kTrace records per-step
+// states of one drop on the frozen map (unscaled, as Drop::descend);
+// kCount only counts steps per start cell; kCommit commits.
+template <int MODE>
 __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
                           shy_drop_step* tr, int max_tr, int* tr_n) {
+  constexpr bool TRACE = MODE == kTrace;
+  constexpr bool COMMIT = MODE == kCommit;
   const shy_params& P = a.P;
   const int nx = a.nx, ny = a.ny;
   int ix = (int)x0, iy = (int)y0;
@@ -146,7 +163,7 @@ __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
     normal_from(c, ix, iy, nx, ny, n0, n1, n2);  // water.h:70
     (void)n1;
     if ((float)age > P.maxAge || vol < P.minVol) {  // water.h:74-82
-      if (!TRACE) {
+      if (COMMIT) {
         const int32_t q = quant_h(sed);
         atomic_add_i32(&a.dh[i0], q);
         S.walk_dh += q;
@@ -155,13 +172,14 @@ __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
       break;
     }
     S.steps++;
+    if (MODE == kCount) atomic_add_i32(&a.count[i0], 1);
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
     step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);  // water.h:95-111
     const int jx = (int)px, jy = (int)py;
     const bool ex = off_map(jx, jy, nx, ny);
     const int j = jx * ny + jy;
-    if (!TRACE) {  // water.h:115-117
+    if (COMMIT) {  // water.h:115-117
       atomic_add_i64(&a.td[i0], quant_t(vol));
       atomic_add_i64(&a.tmx[i0], quant_t(vol * sx));
       atomic_add_i64(&a.tmy[i0], quant_t(vol * sy));
@@ -172,13 +190,17 @@ __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
     float c_eq = (1.0f + P.entrainment * c.erfd) * (c.h - h2);  // water.h:127
     if (c_eq < 0) c_eq = 0;
     const float cdiff = (c_eq - sed);
-    const float ero = effD * cdiff;
-    sed += ero;  // water.h:131-132
-    if (!TRACE) {
-      const int32_t q = -quant_h(ero);
-      atomic_add_i32(&a.dh[i0], q);
-      S.walk_dh += q;
+    float ero;  // water.h:131-132
+    if (COMMIT) {
+      // one quantised amount leaves the map and enters the drop
+      const int32_t q = quant_h((effD * cdiff) * c.scale);
+      ero = (float)q * kHQInv;
+      atomic_add_i32(&a.dh[i0], -q);
+      S.walk_dh -= q;
+    } else {
+      ero = effD * cdiff;
     }
+    sed += ero;
     const float s1 = evap_sed(sed, P.evapRate);  // water.h:135-136
     gain += (double)s1 - (double)sed;
     sed = s1;
@@ -195,21 +217,22 @@ __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
       if (ex || nrec >= max_tr) break;
     }
     if (ex) {  // water.h:139-142
-      if (!TRACE) {
+      if (COMMIT) {
         S.exits++;
         S.export40 += quant_40((double)sed);
       }
       break;
     }
-    if (!TRACE) a.visit[j] = 1;  // water.h:151 -> cascade site
+    if (COMMIT) a.visit[j] = 1;  // water.h:151 -> cascade site
     ++age;
     c = cn;
     ix = jx; iy = jy; i0 = j;
   }
-  if (!TRACE) S.gain40 += quant_40(gain);
+  if (COMMIT) S.gain40 += quant_40(gain);
   if (TRACE) *tr_n = nrec;
 }
 
+template <int MODE>
 __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStats S;
@@ -222,20 +245,53 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a) {
       skipped = 1;
     } else {
       walked = 1;
-      walk_drop<false>(a, (float)(cid / (uint32_t)a.ny), (float)(cid % (uint32_t)a.ny), S,
-                       nullptr, 0, nullptr);
+      walk_drop<MODE>(a, (float)(cid / (uint32_t)a.ny), (float)(cid % (uint32_t)a.ny), S,
+                      nullptr, 0, nullptr);
     }
   }
-  long long v[ST_CASC_MOVES] = {S.steps, walked, skipped, S.exits, S.ends_age,
-                                S.ends_vol, S.export40, S.gain40, S.walk_dh};
-  block_add_stats<ST_CASC_MOVES>(v, a.state->stats);
+  if (MODE == kCommit) {
+    long long v[ST_CASC_MOVES] = {S.steps, walked, skipped, S.exits, S.ends_age,
+                                  S.ends_vol, S.export40, S.gain40, S.walk_dh};
+    block_add_stats<ST_CASC_MOVES>(v, a.state->stats);
+  }
+}
+
+// r^k by binary exponentiation, least significant bit first.
+__device__ inline float powi_f32(float r, int k) {
+  float res = 1.0f, b = r;
+  while (k > 0) {
+    if (k & 1) res = res * b;
+    b = b * b;
+    k >>= 1;
+  }
+  return res;
+}
+
+// Burst scale of a cell whose count pass saw k walk steps start there.
+__device__ inline float burst_scale(int k, float effD, float entrainment, float erfd) {
+  if (k <= 1) return 1.0f;
+  float g = effD * (1.0f + entrainment * erfd);
+  if (!(g > 0.0f)) return 1.0f;
+  if (g > 1.0f) g = 1.0f;
+  return (1.0f - powi_f32(1.0f - g, k)) / (g * (float)k);
+}
+
+__global__ void scale_kernel(int32_t* count, float* scale, const float4* __restrict__ fld,
+                             const float* __restrict__ root, size_t n, shy_params P) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = count[i];
+  float effD = P.depositionRate * (1.0f - (root ? root[i] : 0.0f));
+  if (effD < 0) effD = 0;
+  scale[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
+  count[i] = 0;
 }
 
 __global__ void trace_kernel(WalkArgs a, float x0, float y0, shy_drop_step* out, int max_steps,
                              int* n_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   LaneStats S;
-  walk_drop<true>(a, x0, y0, S, out, max_steps, n_out);
+  walk_drop<kTrace>(a, x0, y0, S, out, max_steps, n_out);
 }
 
 // --------------------------------------------------------------- cascade
@@ -308,7 +364,8 @@ constexpr int kCbThreads = kCbY * kCbX;
 
 __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
   __shared__ float Hs[kCbX + 4][kCbY + 4];   // post-walk heights, 2-cell halo
-  __shared__ uint8_t Vs[kCbX + 2][kCbY + 2]; // cascade sites, 1-cell halo
+  __shared__ uint8_t Vs[kCbX + 6][kCbY + 6]; // cascade sites, 3-cell halo
+  __shared__ uint8_t Wc[kCbX + 4][kCbY + 4]; // site windows covering a cell, 2-cell halo
   __shared__ int32_t Wd[(kCbX + 2) * (kCbY + 2)][9];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.y * kCbX, y0 = blockIdx.x * kCbY;
@@ -323,10 +380,20 @@ __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
     }
     Hs[lx][ly] = v;
   }
-  for (int k = tid; k < (kCbX + 2) * (kCbY + 2); k += kCbThreads) {
-    const int lx = k / (kCbY + 2), ly = k % (kCbY + 2);
-    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+  for (int k = tid; k < (kCbX + 6) * (kCbY + 6); k += kCbThreads) {
+    const int lx = k / (kCbY + 6), ly = k % (kCbY + 6);
+    const int gx = x0 + lx - 3, gy = y0 + ly - 3;
     Vs[lx][ly] = off_map(gx, gy, nx, ny) ? 0 : a.visit[gx * ny + gy];
+  }
+  __syncthreads();
+  for (int k = tid; k < (kCbX + 4) * (kCbY + 4); k += kCbThreads) {
+    const int lx = k / (kCbY + 4), ly = k % (kCbY + 4);  // Vs coords lx+1, ly+1
+    int cnt = 0;
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) cnt += Vs[lx + dx][ly + dy];
+    Wc[lx][ly] = (uint8_t)cnt;
   }
   __syncthreads();
   long long moves = 0;
@@ -335,19 +402,27 @@ __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
     int32_t* w = Wd[k];
 #pragma unroll
     for (int s = 0; s < 9; ++s) w[s] = 0;
-    if (!Vs[lx][ly]) continue;
+    if (!Vs[lx + 2][ly + 2]) continue;
     const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-    const int hx = lx + 1, hy = ly + 1;  // centre in Hs coordinates
+    const int hx = lx + 1, hy = ly + 1;  // centre in Hs / Wc coordinates
+    // window weight: the most windows covering any of its on-map cells
+    int m = 1;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+        if (!off_map(gx + dx, gy + dy, nx, ny) && Wc[hx + dx][hy + dy] > m) m = Wc[hx + dx][hy + dy];
+    const float fm = (float)m;
     // halo windows are recomputed by the neighbouring block; count own ones
     const bool own = lx >= 1 && lx <= kCbX && ly >= 1 && ly <= kCbY;
-    const int m = cascade_site(
+    const int mv = cascade_site(
         gx, gy, nx, ny, a.P, Hs[hx][hy],
         [&](int ox, int oy) { return Hs[hx + ox][hy + oy]; },
         [&](int slot, float transfer, bool down) {
-          const int32_t q = quant_h(transfer);
+          const int32_t q = quant_h(transfer / fm);
           if (down) { w[8] -= q; w[slot] += q; } else { w[8] += q; w[slot] -= q; }
         });
-    if (own) moves += m;
+    if (own) moves += mv;
   }
   __syncthreads();
   const int tx = tid / kCbY, ty = tid % kCbY;
@@ -362,7 +437,7 @@ __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
       const int ox = s == 8 ? 0 : (s < 3 ? -1 : (s < 5 ? 0 : 1));
       const int oy = s == 8 ? 0 : ((s == 0 || s == 3 || s == 5) ? -1 : ((s == 1 || s == 6) ? 0 : 1));
       const int wx = tx - ox + 1, wy = ty - oy + 1;
-      if (Vs[wx][wy]) tot += Wd[wx * (kCbY + 2) + wy][s];
+      if (Vs[wx + 2][wy + 2]) tot += Wd[wx * (kCbY + 2) + wy][s];
     }
     a.h_out[i] = Hs[tx + 2][ty + 2] + (float)tot * kHQInv;
     // world.h:81-86
@@ -387,6 +462,24 @@ __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
 }
 
 // ---------------------------------------------------------- field kernels
+// quad::map::normal (cellpool.h:181-204, 445-447) of every cell, xyz.
+__global__ void normals_kernel(const float* __restrict__ h, int nx, int ny, float* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int x = (int)(i / (size_t)ny), y = (int)(i % (size_t)ny);
+  CellView c;
+  c.h = h[i];
+  c.xm = h[x > 0 ? i - ny : i];
+  c.xp = h[x + 1 < nx ? i + ny : i];
+  c.ym = h[y > 0 ? i - 1 : i];
+  c.yp = h[y + 1 < ny ? i + 1 : i];
+  float n0, n1, n2;
+  normal_from(c, x, y, nx, ny, n0, n1, n2);
+  out[3 * i] = n0;
+  out[3 * i + 1] = n1;
+  out[3 * i + 2] = n2;
+}
+
 __global__ void refresh_erf_kernel(float4* fld, size_t n, const double* __restrict__ erf_a) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

@@ -1,0 +1,361 @@
+"""Pins the CPU oracle to the reference's mathematics (CPU only, no GPU).
+
+The reference ships no tests, fixtures or golden vectors and cannot be built
+here (glm is not installed), so each oracle function is pinned by closed
+forms, invariants and library routines chosen so that a dropped term, a wrong
+sign or index, or a transposed operand fails one of them:
+
+* erf          -- against Python's math.erf (correctly rounded double) cast to
+                  float32: at most 1 ulp, odd symmetry, saturation at |x| >= 4.
+* normal       -- the analytic plane normal normalize(-80a, 1, -80b), and
+                  mirror symmetry of a non-planar bump.
+* descend      -- flat map: 501 steps, fixed position, f64-rounded geometric
+                  volume, exact discharge track sum (B1); inclined plane:
+                  straight steps of length fl(sqrt 2) down -(a, b) (B2); the
+                  first-step equilibrium sediment; momentum coupling sign and
+                  its raw-discharge denominator.
+* cascade      -- hand-evaluated spike transfers, the 0.1 threshold branch,
+                  the ascending visiting order, mass conservation (B4).
+* blend        -- the EMA decay closed form (B3).
+* ledger       -- sum(h) change = evaporation gains - exported sediment, for
+                  the sequential and the wave oracle (B5).
+* wave         -- burst-scale closed forms; a one-drop wave equals the frozen
+                  trace's commits; sequential vs wave terrains agree in
+                  hypsometry and drainage statistics.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+P = O.Params.default()
+F32 = np.float32
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+# ------------------------------------------------------------------- erf
+def test_erf_within_one_ulp_of_math_erf():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([np.linspace(0, 4.5, 60001), rng.uniform(0, 6, 20000),
+                         10.0 ** rng.uniform(-30, 0.7, 20000)]).astype(np.float32)
+    xs = np.concatenate([xs, -xs])
+    got = O.erf(xs)
+    ref = np.array([math.erf(float(x)) for x in xs]).astype(np.float32)
+    d = ulp_diff(got, ref)
+    assert d.max() <= 1
+    assert np.mean(d == 0) > 0.999
+    assert np.array_equal(O.erf(-xs).view(np.uint32), (-got).view(np.uint32))  # odd
+    assert O.erf(np.float32([0.0]))[0] == 0.0
+    assert np.all(O.erf(np.float32([4.0, 5.0, 100.0, 1e30])) == 1.0)
+
+
+# ---------------------------------------------------------------- normal
+@pytest.mark.parametrize("a,b", [(0.01, 0.0), (0.0, -0.02), (0.003, 0.007), (-0.05, 0.02)])
+def test_plane_normal_closed_form(a, b):
+    n = 12
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    h = (0.5 + a * x + b * y).astype(np.float64).astype(np.float32)
+    exp = np.array([-80 * a, 1.0, -80 * b])
+    exp /= np.linalg.norm(exp)
+    for p in [(5, 6), (0, 0), (n - 1, n - 1), (0, n - 1), (n - 1, 3)]:
+        got = O.normal(h, *p)
+        assert np.allclose(got, exp, atol=2e-5), (p, got, exp)
+
+
+def test_normal_mirror_symmetry():
+    n = 9
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    h = np.exp(-((x - 4.0) ** 2 + (y - 4.0) ** 2) / 6.0).astype(np.float32)
+    for dx in range(1, 4):
+        a = O.normal(h, 4 + dx, 4)
+        b = O.normal(h, 4 - dx, 4)
+        assert np.allclose([a[0], a[1], a[2]], [-b[0], b[1], b[2]], atol=1e-6)
+        assert a[0] > 0  # points away from the peak (downhill in +x)
+        c = O.normal(h, 4, 4 + dx)
+        assert np.allclose([c[2], c[1]], [a[0], a[1]], atol=1e-6)  # x <-> y transpose
+
+
+# --------------------------------------------------------------- descend
+def flat_trace(hval=0.5, n=16):
+    h = np.full((n, n), hval, np.float32)
+    z = np.zeros_like(h)
+    return O.trace_frozen(h, z, z, z, z, z, P, 8.0, 8.0, 700)
+
+
+def test_flat_map_closed_form_B1():
+    st, reason = flat_trace()
+    assert len(st) == 501 and reason == 2  # age > maxAge after ages 0..500
+    assert np.all(st["pos"] == 8.0) and np.all(st["speed"] == 0.0)
+    assert np.all(st["sediment"] == 0.0) and np.all(st["erosion"] == 0.0)
+    e = np.float64(np.float32(0.001))
+    v = F32(1.0)
+    for k in range(501):
+        v = F32(np.float64(v) * (1.0 - e))
+        assert st["volume"][k] == v
+    # discharge track of the spawn cell, accumulated in float as water.h:115
+    cells = O.to_aos(np.full((16, 16), 0.5, np.float32))
+    drop = dict(age=0, pos=(8.0, 8.0), speed=(0.0, 0.0), volume=1.0, sediment=0.0)
+    acc, v = F32(0.0), F32(1.0)
+    while O.seq_descend(cells, P, drop):
+        pass
+    for k in range(501):
+        acc = F32(acc + v)
+        v = F32(np.float64(v) * (1.0 - e))
+    assert cells[8, 8, 4] == acc and abs(acc - (1 - 0.999 ** 501) / 0.001) < 0.05
+    assert drop["age"] == 501
+
+
+@pytest.mark.parametrize("a,b", [(0.004, 0.0), (0.002, -0.003), (-0.001, 0.001)])
+def test_plane_trajectory_B2(a, b):
+    n = 64
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    h = (0.5 + a * x + b * y).astype(np.float32)
+    z = np.zeros_like(h)
+    x0, y0 = 31.0, 33.0
+    st, reason = O.trace_frozen(h, z, z, z, z, z, P, x0, y0, 700)
+    assert reason == 4 and st["status"][-1] == 1  # leaves the map
+    u = -np.array([a, b]) / math.hypot(a, b)
+    k = np.arange(1, len(st) + 1)
+    expect = np.stack([x0 + k * math.sqrt(2) * u[0], y0 + k * math.sqrt(2) * u[1]], 1)
+    assert np.allclose(st["pos"], expect, atol=2e-4 * k[:, None] + 1e-4)
+    sp = np.hypot(st["speed"][:, 0].astype(np.float64), st["speed"][:, 1])
+    assert np.allclose(sp, math.sqrt(2), rtol=3e-7)
+
+
+def test_first_step_equilibrium_sediment():
+    """c_eq = (1 + entrainment*erf(0.4 d)) (h - h2), sediment = effD c_eq / (1-e)."""
+    n = 32
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a = 0.01
+    h = (0.5 - a * x).astype(np.float32)
+    for dval in (0.0, 2.5):
+        d = np.full_like(h, dval)
+        z = np.zeros_like(h)
+        st, _ = O.trace_frozen(h, d, z, z, z, O.erf(F32(0.4) * d), P, 10.0, 10.0, 3)
+        assert st["pos"][0][0] > 11.0
+        h2 = h[int(st["pos"][0][0]), int(st["pos"][0][1])]
+        ceq = (1 + 10 * math.erf(0.4 * dval)) * (float(h[10, 10]) - float(h2))
+        assert ceq > 0
+        assert math.isclose(st["erosion"][0], 0.1 * ceq, rel_tol=1e-5)
+        assert math.isclose(st["sediment"][0], 0.1 * ceq / (1 - 0.001), rel_tol=1e-5)
+
+
+def test_momentum_coupling_one_step():
+    """speed += mT * dot(f^, s^) / (volume + d) * f, with the raw discharge d."""
+    n = 16
+    x, _ = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a = 0.005
+    h = (0.5 - a * x).astype(np.float32)
+    f = np.array([3.0, 4.0])
+    for dval in (0.0, 6.0):
+        d = np.full_like(h, dval)
+        mx = np.full_like(h, f[0])
+        my = np.full_like(h, f[1])
+        z = np.zeros_like(h)
+        st, _ = O.trace_frozen(h, d, mx, my, z, z, P, 7.0, 7.0, 1)
+        nv = np.array([80 * a, 1.0, 0.0])
+        nv /= np.linalg.norm(nv)
+        s = np.array([nv[0], nv[2]])  # gravity / volume 1
+        s = s + (np.dot(f / np.linalg.norm(f), s / np.linalg.norm(s)) / (1.0 + dval)) * f
+        s = math.sqrt(2) * s / np.linalg.norm(s)
+        assert np.allclose(st["speed"][0], s, atol=1e-6), (dval, st["speed"][0], s)
+
+
+# --------------------------------------------------------------- cascade
+def test_cascade_spike_hand_values_B4():
+    h = np.full((3, 3), 0.5, np.float32)
+    h[1, 1] = 1.0
+    cells = O.to_aos(h)
+    O.seq_cascade(cells, P, 1.0, 1.0)
+    # all neighbours equal: enumeration order (-1,-1),(-1,0),(-1,1),(0,-1),(0,1),(1,-1),(1,0),(1,1)
+    order = [(0, 0), (0, 1), (0, 2), (1, 0), (1, 2), (2, 0), (2, 1), (2, 2)]
+    hc = 1.0
+    exp = h.astype(np.float64)
+    for (i, j) in order:
+        dist = math.sqrt(2) if (i != 1 and j != 1) else 1.0
+        diff = hc - 0.5
+        t = 0.8 * (abs(diff) - dist * 0.01) / 2
+        hc -= t
+        exp[i, j] += t
+    exp[1, 1] = hc
+    assert np.allclose(cells[..., 0], exp, atol=2e-7)
+    assert abs(cells[..., 0].astype(np.float64).sum() - h.astype(np.float64).sum()) < 1e-6
+
+
+def test_cascade_visits_lowest_neighbour_first_and_threshold():
+    h = np.array([[0.60, 0.40, 0.60],
+                  [0.60, 1.00, 0.05],
+                  [0.60, 0.60, 0.60]], np.float32)
+    cells = O.to_aos(h)
+    O.seq_cascade(cells, P, 1.0, 1.0)
+    # lowest first: (1,2) at 0.05 <= 0.1 -> excess = |diff| (no maxdiff term)
+    t1 = 0.8 * (1.0 - 0.05) / 2
+    hc = 1.0 - t1
+    assert math.isclose(cells[1, 2, 0], 0.05 + t1, rel_tol=1e-6)
+    # then (0,1) at 0.40, edge distance 1
+    t2 = 0.8 * (abs(hc - 0.4) - 0.01) / 2
+    assert math.isclose(cells[0, 1, 0], 0.4 + t2, rel_tol=1e-6)
+    assert abs(cells[..., 0].astype(np.float64).sum() - h.astype(np.float64).sum()) < 1e-6
+
+
+def test_cascade_edge_site_only_in_bounds_neighbours():
+    h = np.full((4, 4), 0.3, np.float32)
+    h[0, 0] = 0.9
+    cells = O.to_aos(h)
+    O.seq_cascade(cells, P, 0.0, 0.0)
+    changed = np.argwhere(cells[..., 0] != h)
+    assert {tuple(c) for c in changed} <= {(0, 0), (0, 1), (1, 0), (1, 1)}
+    assert abs(cells[..., 0].astype(np.float64).sum() - h.astype(np.float64).sum()) < 1e-6
+
+
+# ----------------------------------------------------------------- blend
+def test_blend_ema_decay_B3():
+    n = 16
+    h = np.full((n, n), 0.05, np.float32)  # below 0.1: every spawn is skipped
+    cells = O.to_aos(h, d=np.full_like(h, 4.0))
+    for k in range(1, 6):
+        led = O.seq_erode(cells, P, 1, k, 50)
+        assert led.walked == 0 and led.skipped == 50
+        assert np.allclose(cells[..., 1], 4.0 * 0.9 ** k, rtol=1e-6)
+    z = np.zeros_like(h)
+    d, mx, my, root, erfd = np.full_like(h, 4.0), z.copy(), z.copy(), z.copy(), z.copy()
+    hh = h.copy()
+    for k in range(1, 6):
+        st = O.wave_cycle(hh, d, mx, my, root, erfd, P, 1, k, 50)
+        assert st.walked == 0 and st.skipped == 50
+        assert np.allclose(d, 4.0 * 0.9 ** k, rtol=1e-6)
+    assert np.array_equal(hh, h)
+    assert np.array_equal(erfd, O.erf(F32(0.4) * d))
+
+
+# ---------------------------------------------------------------- ledger
+def test_sequential_mass_ledger_B5():
+    n = 64
+    h = O.fbm(2, n, n)
+    cells = O.to_aos(h)
+    s0 = cells[..., 0].astype(np.float64).sum()
+    gain = exp = 0.0
+    for c in range(6):
+        led = O.seq_erode(cells, P, 2, c, 400)
+        gain += led.gain
+        exp += led.export_
+    s1 = cells[..., 0].astype(np.float64).sum()
+    assert gain > 0 and exp > 0
+    assert abs((s1 - s0) - (gain - exp)) <= 1e-6 * s0
+
+
+def test_wave_mass_ledger_B5():
+    n = 64
+    h = O.fbm(2, n, n)
+    z = np.zeros_like(h)
+    d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+    s0 = h.astype(np.float64).sum()
+    tot = dict(gain40=0, export40=0, walk_dh=0)
+    for c in range(6):
+        st = O.wave_cycle(h, d, mx, my, root, erfd, P, 2, c, 400).as_dict()
+        for k in tot:
+            tot[k] += st[k]
+    s1 = h.astype(np.float64).sum()
+    ledger = (tot["gain40"] - tot["export40"]) * 2.0 ** -40
+    assert abs((s1 - s0) - ledger) <= 1e-6 * s0
+    assert abs((s1 - s0) - tot["walk_dh"] * 2.0 ** -20) <= 1e-6 * s0
+
+
+# ------------------------------------------------------------------ wave
+def test_burst_scale_closed_forms():
+    f = O.lib().orc_burst_scale
+    assert f(0, 0.1, 10.0, 0.5) == 1.0 and f(1, 0.1, 10.0, 1.0) == 1.0
+    for k in (2, 7, 100):
+        assert math.isclose(f(k, 0.1, 10.0, 1.0), 1.0 / k, rel_tol=1e-6)  # g >= 1
+        g = 0.1 * (1 + 10 * 0.05)
+        s = f(k, 0.1, 10.0, 0.05)
+        assert math.isclose(k * s * g, 1 - (1 - g) ** k, rel_tol=1e-5)
+        assert f(k, 0.0, 10.0, 0.5) == 1.0
+    assert f(5, 0.1, 10.0, 0.0) > f(50, 0.1, 10.0, 0.0)
+
+
+def test_one_drop_wave_matches_frozen_trace():
+    """Two code paths of the oracle: a single-drop wave's commits are the
+    trace's per-step erosion at each start cell (no cell is started twice),
+    to one 2^-20 unit (the wave moves sediment in whole units)."""
+    n = 64
+    h = O.fbm(5, n, n)
+    z = np.zeros_like(h)
+    key = O.cycle_key(9, 0, 0)
+    cid = O.lib().orc_spawn_cell(key, 0, n * n)
+    x0, y0 = divmod(cid, n)
+    st, _ = O.trace_frozen(h, z, z, z, z, z, P, float(x0), float(y0), 700, use_ref_erf=False)
+    cells = [(x0, y0)] + [tuple(int(v) for v in p) for p in st["pos"][:-1]]
+    if len(set(cells)) != len(cells) or h[x0, y0] < 0.1:
+        pytest.skip("drop revisits a cell")
+    hh = h.copy()
+    d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+    ws, dh, vis = O.wave_cycle(hh, d, mx, my, root, erfd, P, 9, 0, 1, want_dh=True)
+    assert ws.steps == len(st)
+    for k, c in enumerate(cells):
+        q = -int(np.rint(np.float32(st["erosion"][k]) * np.float32(2 ** 20)))
+        if k == len(cells) - 1 and st["status"][-1] == 0:
+            continue
+        assert abs(int(dh[c]) - q) <= 1, (k, c)
+
+
+def test_spawn_stream_uniform_and_skip_rule():
+    n = 64
+    cells = O.spawn_cells(4, 0, 0, 20000, n * n)
+    assert cells.max() < n * n
+    hist = np.bincount(cells * 16 // (n * n), minlength=16)
+    chi2 = ((hist - 1250.0) ** 2 / 1250.0).sum()
+    assert chi2 < 40  # 15 dof, p ~ 5e-4
+    h = O.fbm(4, n, n)
+    z = np.zeros_like(h)
+    d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+    spawned = O.spawn_cells(4, 0, 0, 500, n * n)
+    st = O.wave_cycle(h.copy(), d, mx, my, root, erfd, P, 4, 0, 500)
+    assert st.skipped == int((h.ravel()[spawned].astype(np.float64) < 0.1).sum())
+
+
+def test_fbm_generator_properties():
+    a = O.fbm(1, 64, 48)
+    b = O.fbm(1, 64, 48)
+    c = O.fbm(2, 64, 48)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert a.min() == 0.0 and a.max() == 1.0 and a.shape == (64, 48)
+    for o in range(8):
+        assert sorted(O.fbm_perm(1, o).tolist()) == list(range(256))
+
+
+def _hyps(h):
+    return np.quantile(h, np.linspace(0, 1, 21))
+
+
+def _drain(d):
+    return np.histogram(np.log10(np.maximum(d, 1e-3)), bins=np.linspace(-3, 3.5, 14))[0] / d.size
+
+
+def test_sequential_vs_wave_terrain_statistics():
+    """Sequential reference vs wave-synchronous terrains, 128x128, 1 drop per
+    16 cells, 40 cycles: hypsometric curve, drainage-area histogram (of the
+    discharge field), height correlation; both stay bounded."""
+    n, cyc = 128, 40
+    drops = n * n // 16
+    h0 = O.fbm(3, n, n)
+    cells = O.to_aos(h0)
+    for k in range(cyc):
+        O.seq_erode(cells, P, 1, k, drops)
+    hs, ds = cells[..., 0], cells[..., 1]
+    h = h0.copy()
+    z = np.zeros_like(h)
+    d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+    for k in range(cyc):
+        O.wave_cycle(h, d, mx, my, root, erfd, P, 1, k, drops)
+    assert -0.5 < h.min() and h.max() < 1.5 and -0.5 < hs.min() and hs.max() < 1.5
+    assert np.abs(_hyps(h) - _hyps(hs)).mean() < 0.05
+    assert np.abs(_drain(d) - _drain(ds)).sum() < 0.3
+    assert np.corrcoef(h.ravel(), hs.ravel())[0, 1] > 0.9
