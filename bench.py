@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--walk-block", type=int, default=0)
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (HBM counter "
+                         "calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,6 +89,16 @@ def main():
         sys.exit(2)
 
     import simplehydrology_amd as shy
+
+    if args.calib_copy_mib:
+        import torch
+        nel = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     n, drops = args.size, args.drops
     w = shy.World(n, n, seed=args.seed)

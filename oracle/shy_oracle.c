@@ -22,8 +22,8 @@
 #define LOD 1.0f       /* quad::lodsize, cellpool.h:178 (int 1 -> float) */
 #define HQ 1048576.0f  /* height deltas: int32, 2^-20 units */
 #define HQ_INV 9.5367431640625e-07f
-#define TQ 4294967296.0f /* tracks: int64, 2^-32 units */
-#define TQ_INV 2.3283064365386963e-10
+#define TQ 16777216.0f /* track commits: int32 in 2^-24 units, summed in int64 */
+#define TQ_INV 5.9604644775390625e-08
 #define SQ40 1099511627776.0 /* ledger sums: 2^-40 units */
 
 void orc_default_params(orc_params* p) {
@@ -417,12 +417,12 @@ static inline int32_t qh(float v) {
   if (s < -1073741824.0f) s = -1073741824.0f;
   return (int32_t)rintf(s);
 }
-static inline int64_t qt(float v) {
+static inline int32_t qt(float v) {
   float s = v * TQ;
   if (!(s == s)) s = 0.0f;
-  if (s > 4.0e18f) s = 4.0e18f;
-  if (s < -4.0e18f) s = -4.0e18f;
-  return (int64_t)rintf(s);
+  if (s > 2.0e9f) s = 2.0e9f;
+  if (s < -2.0e9f) s = -2.0e9f;
+  return (int32_t)rintf(s);
 }
 static inline int64_t q40(double v) {
   double s = v * SQ40;

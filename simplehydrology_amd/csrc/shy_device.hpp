@@ -17,8 +17,8 @@ constexpr float kMapScale = 80.0f;  // quad::mapscale, cellpool.h:165
 constexpr float kLod = 1.0f;        // quad::lodsize, cellpool.h:178
 constexpr float kHQ = 1048576.0f;   // height commits: int32 in 2^-20 units
 constexpr float kHQInv = 9.5367431640625e-07f;
-constexpr float kTQ = 4294967296.0f;  // track commits: int64 in 2^-32 units
-constexpr double kTQInv = 2.3283064365386963e-10;
+constexpr float kTQ = 16777216.0f;  // track commits: int32 in 2^-24 units
+constexpr double kTQInv = 5.9604644775390625e-08;
 constexpr double kSQ40 = 1099511627776.0;  // ledger sums in 2^-40 units
 constexpr int kErfTerms = 72;
 
@@ -53,12 +53,12 @@ __device__ inline int32_t quant_h(float v) {
   if (s < -1073741824.0f) s = -1073741824.0f;
   return (int32_t)__builtin_rintf(s);
 }
-__device__ inline long long quant_t(float v) {
+__device__ inline int32_t quant_t(float v) {
   float s = v * kTQ;
   if (!(s == s)) s = 0.0f;
-  if (s > 4.0e18f) s = 4.0e18f;
-  if (s < -4.0e18f) s = -4.0e18f;
-  return (long long)__builtin_rintf(s);
+  if (s > 2.0e9f) s = 2.0e9f;
+  if (s < -2.0e9f) s = -2.0e9f;
+  return (int32_t)__builtin_rintf(s);
 }
 __device__ inline long long quant_40(double v) {
   double s = v * kSQ40;

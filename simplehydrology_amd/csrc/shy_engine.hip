@@ -4,14 +4,21 @@
 //   h[2]        float   heights, ping-pong (the cascade stencil reads a halo)
 //   fld         float4  {momentumx, momentumy, discharge, erf(0.4 discharge)}
 //   root        float   root density
-//   dh[2]       int32   walk height commits (2^-20), ping-pong
 //   visit[2]    uint8   cascade sites, ping-pong
-//   trk[2][3]   int64   discharge / momentum tracks (2^-32), ping-pong
-//   count       int32   walk steps per start cell (count pass; cleared by scale)
-//   scale       float   burst scale per cell (written by scale, read by commit)
-// Parity p = cycle & 1 selects the buffers a cycle writes; the cascade/blend
-// kernel of cycle p clears parity p^1 for the next cycle, so a cycle needs no
-// memset and two cycles form a replayable hipGraph.
+//   count       int32   walk steps per start cell (tickets); cleared by reduce
+//   offset      int32   bucket start per cell (prefix sum of count)
+//   scale       float   burst scale per cell
+//   dh          int32   summed walk commits per cell, 2^-20 (written by reduce)
+//   trk[3]      int64   summed tracks per cell, 2^-24 (written by reduce; stay
+//                       readable until the next cycle, as the reference's
+//                       *_track fields do)
+//   deposit     int32   terminal deposits, 2^-20; cleared by reduce
+// and per wave (sized by drops x (maxAge+1) steps):
+//   ticket      int32   [step][drop] ticket log of the count pass
+//   rec         int4    per-step commit records, bucketed by start cell
+// Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle
+// p clears visit[p^1] for the next cycle, so a cycle needs no memset and two
+// cycles form a replayable hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,11 +42,17 @@ struct shy_ctx {
   float4* fld = nullptr;
   float* root = nullptr;
   bool has_root = false;
-  int32_t* dh[2] = {nullptr, nullptr};
   uint8_t* visit[2] = {nullptr, nullptr};
-  long long* trk[2] = {nullptr, nullptr};  // 3*n each: d, mx, my
   int32_t* count = nullptr;
+  int32_t* offset = nullptr;
   float* scale = nullptr;
+  int32_t* dh = nullptr;
+  long long* trk = nullptr;  // 3*n: d, mx, my
+  int32_t* deposit = nullptr;
+  int32_t* bsum = nullptr;
+  int32_t* ticket = nullptr;
+  int4* rec = nullptr;
+  size_t wave_cap = 0;  // steps the ticket log and buckets can hold
   DeviceState* state = nullptr;
   double* erf_a = nullptr;
   float2* part = nullptr;  // min/max partials
@@ -115,13 +128,13 @@ WalkArgs walk_args(shy_ctx* c, int drops) {
   a.h = c->h[p];
   a.fld = c->fld;
   a.root = c->has_root ? c->root : nullptr;
-  a.dh = c->dh[p];
-  a.td = c->trk[p];
-  a.tmx = c->trk[p] + c->n;
-  a.tmy = c->trk[p] + 2 * c->n;
-  a.visit = c->visit[p];
   a.count = c->count;
+  a.ticket = c->ticket;
+  a.offset = c->offset;
   a.scale = nullptr;
+  a.rec = c->rec;
+  a.deposit = c->deposit;
+  a.visit = c->visit[p];
   a.state = c->state;
   a.seed = c->seed;
   a.terrain = 0;
@@ -132,60 +145,82 @@ WalkArgs walk_args(shy_ctx* c, int drops) {
   return a;
 }
 
-// ev, when given, receives events after the count walk, the scale pass and
-// the commit walk.
+void drop_graph(shy_ctx* c);
+
+size_t wave_steps(const shy_ctx* c, int drops) {
+  return (size_t)drops * ((size_t)c->P.maxAge + 1);
+}
+
+// Size the ticket log and buckets for `drops` (never inside a capture).
+int ensure_wave(shy_ctx* c, int drops) {
+  const size_t need = wave_steps(c, drops);
+  if (need > (size_t)INT32_MAX)
+    return fail(c, SHY_ERR_ARG, "drops x (maxAge+1) exceeds 2^31 steps per wave");
+  if (need <= c->wave_cap) return SHY_OK;
+  drop_graph(c);  // a captured graph holds the old buffers
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->ticket) hipFree(c->ticket);
+  if (c->rec) hipFree(c->rec);
+  c->ticket = nullptr;
+  c->rec = nullptr;
+  c->wave_cap = 0;
+  HIPCHK(c, hipMalloc(&c->ticket, need * sizeof(int32_t)));
+  HIPCHK(c, hipMalloc(&c->rec, need * sizeof(int4)));
+  c->wave_cap = need;
+  return SHY_OK;
+}
+
+// ev, when given, receives events after the count walk, the scan and the
+// commit walk.  The wave buffers must already hold wave_steps(c, drops).
 int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t* ev = nullptr) {
   WalkArgs w = walk_args(c, drops);
   w.h = c->h[p];
-  w.dh = c->dh[p];
-  w.td = c->trk[p];
-  w.tmx = c->trk[p] + c->n;
-  w.tmy = c->trk[p] + 2 * c->n;
   w.visit = c->visit[p];
+  const int bs = c->walk_block;
+  const int grid = (drops + bs - 1) / bs;
+  const size_t n = c->n;
+  const int nb = (int)((n + kScanBlock - 1) / kScanBlock);
   if (drops > 0) {
-    const int bs = c->walk_block;
-    const int grid = (drops + bs - 1) / bs;
     walk_kernel<kCount><<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   if (drops > 0) {
-    const size_t sgrid = (c->n + 255) / 256;
-    scale_kernel<<<sgrid, 256, 0, c->stream>>>(c->count, c->scale, c->fld,
-                                              c->has_root ? c->root : nullptr, c->n, c->P);
+    scan_sums_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, n, c->bsum);
+    scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->bsum, nb);
+    scan_cells_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, c->bsum, n, c->offset, c->scale,
+                                                       c->fld, c->has_root ? c->root : nullptr,
+                                                       c->P);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   if (drops > 0) {
-    const int bs = c->walk_block;
-    const int grid = (drops + bs - 1) / bs;
     w.scale = c->scale;
     walk_kernel<kCommit><<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  reduce_kernel<<<(n + kReduceCells - 1) / kReduceCells, kReduceCells, 0, c->stream>>>(
+      c->count, c->offset, c->rec, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
+  HIPCHK(c, hipGetLastError());
   CascArgs a;
   const int q = p ^ 1;
   a.h_in = c->h[p];
   a.h_out = c->h[q];
-  a.dh = c->dh[p];
+  a.dh = c->dh;
   a.visit = c->visit[p];
-  a.td = c->trk[p];
-  a.tmx = c->trk[p] + c->n;
-  a.tmy = c->trk[p] + 2 * c->n;
-  a.dh_clear = c->dh[q];
+  a.td = c->trk;
+  a.tmx = c->trk + n;
+  a.tmy = c->trk + 2 * n;
   a.visit_clear = c->visit[q];
-  a.td_clear = c->trk[q];
-  a.tmx_clear = c->trk[q] + c->n;
-  a.tmy_clear = c->trk[q] + 2 * c->n;
   a.fld = c->fld;
   a.erf_a = c->erf_a;
   a.state = c->state;
   a.nx = c->nx;
   a.ny = c->ny;
   a.P = c->P;
-  dim3 grid((c->ny + kCbY - 1) / kCbY, (c->nx + kCbX - 1) / kCbX);
-  cascade_blend_kernel<<<grid, kCbThreads, 0, c->stream>>>(a);
+  dim3 cgrid((c->ny + kCbY - 1) / kCbY, (c->nx + kCbX - 1) / kCbX);
+  cascade_blend_kernel<<<cgrid, kCbThreads, 0, c->stream>>>(a);
   HIPCHK(c, hipGetLastError());
   return SHY_OK;
 }
@@ -243,9 +278,10 @@ int shy_destroy(shy_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->dh[0], c->dh[1], c->visit[0],
-                  c->visit[1], c->trk[0], c->trk[1], c->count, c->scale, c->state, c->erf_a, c->part,
-                  c->perms, c->trace_buf, c->trace_n, c->drop_buf};
+  void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
+                  c->offset, c->scale, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
+                  c->state, c->erf_a, c->part, c->perms, c->trace_buf, c->trace_n,
+                  c->drop_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -283,16 +319,19 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
   if ((e = hipGetDevice(&c->device)) != hipSuccess) return bail(e, "hipGetDevice");
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(e, "hipStreamCreate");
+  const size_t nb = (n + kScanBlock - 1) / kScanBlock;
   for (int k = 0; k < 2; ++k) {
     if ((e = hipMalloc(&c->h[k], n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc h");
-    if ((e = hipMalloc(&c->dh[k], n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
     if ((e = hipMalloc(&c->visit[k], n)) != hipSuccess) return bail(e, "hipMalloc visit");
-    if ((e = hipMalloc(&c->trk[k], 3 * n * sizeof(long long))) != hipSuccess)
-      return bail(e, "hipMalloc trk");
   }
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
   if ((e = hipMalloc(&c->count, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc count");
+  if ((e = hipMalloc(&c->offset, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc offset");
   if ((e = hipMalloc(&c->scale, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc scale");
+  if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
+  if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
+  if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");
+  if ((e = hipMalloc(&c->bsum, nb * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc bsum");
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->state, sizeof(DeviceState))) != hipSuccess) return bail(e, "hipMalloc state");
   if ((e = hipMalloc(&c->erf_a, kErfTerms * sizeof(double))) != hipSuccess)
@@ -303,13 +342,15 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
   if ((e = hipMalloc(&c->trace_n, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc trace_n");
   for (int k = 0; k < 2; ++k) {
     hipMemsetAsync(c->h[k], 0, n * sizeof(float), c->stream);
-    hipMemsetAsync(c->dh[k], 0, n * sizeof(int32_t), c->stream);
     hipMemsetAsync(c->visit[k], 0, n, c->stream);
-    hipMemsetAsync(c->trk[k], 0, 3 * n * sizeof(long long), c->stream);
   }
   hipMemsetAsync(c->fld, 0, n * sizeof(float4), c->stream);
-  hipMemsetAsync(c->count, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->root, 0, n * sizeof(float), c->stream);
+  hipMemsetAsync(c->count, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->offset, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->dh, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
+  hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);
   DeviceState st{};
   st.cycle = 0;
   double coef[kErfTerms];
@@ -349,14 +390,10 @@ int shy_set_seed(shy_ctx* c, uint64_t seed, uint32_t cycle) {
   c->seed = seed;
   // parity follows the cycle number; move the live buffers if it flips
   if ((cycle & 1u) != (c->cycle & 1u)) {
-    // parity p: h[p] live, dh/visit/trk[p] zero accumulators, trk[p^1] the
-    // last cycle's tracks.  Move everything to parity q = p^1.
+    // parity p: h[p] live, visit[p] the zero accumulator.  Move to q = p^1.
     const int p = parity(c), q = p ^ 1;
     const size_t n = c->n;
     HIPCHK(c, hipMemcpyAsync(c->h[q], c->h[p], n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->trk[p], c->trk[q], 3 * n * sizeof(long long), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->trk[q], 0, 3 * n * sizeof(long long), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->dh[q], 0, n * sizeof(int32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(c->visit[q], 0, n, c->stream));
   }
   c->cycle = cycle;
@@ -405,12 +442,11 @@ int shy_upload_field(shy_ctx* c, int field, const float* src) {
     case SHY_DISCHARGE_TRACK:
     case SHY_MOMENTUMX_TRACK:
     case SHY_MOMENTUMY_TRACK: {
-      // tracks are per-cycle scratch (world.h:56-61); keep them visible until
-      // the next cycle as the reference does, in the last cycle's buffer
+      // tracks are per-cycle scratch (world.h:56-61): they stay visible until
+      // the next cycle overwrites them, as the reference's do
       std::vector<long long> q(n);
-      for (size_t i = 0; i < n; ++i) q[i] = (long long)std::llrint((double)src[i] * 4294967296.0);
-      const int last = p ^ 1;
-      HIPCHK(c, hipMemcpy(c->trk[last] + (size_t)(field - SHY_DISCHARGE_TRACK) * n, q.data(),
+      for (size_t i = 0; i < n; ++i) q[i] = (long long)std::llrint((double)src[i] * 16777216.0);
+      HIPCHK(c, hipMemcpy(c->trk + (size_t)(field - SHY_DISCHARGE_TRACK) * n, q.data(),
                           n * sizeof(long long), hipMemcpyHostToDevice));
       break;
     }
@@ -456,8 +492,7 @@ int shy_download_field(shy_ctx* c, int field, float* dst) {
     case SHY_MOMENTUMX_TRACK:
     case SHY_MOMENTUMY_TRACK: {
       std::vector<long long> q(n);
-      const int last = p ^ 1;
-      HIPCHK(c, hipMemcpy(q.data(), c->trk[last] + (size_t)(field - SHY_DISCHARGE_TRACK) * n,
+      HIPCHK(c, hipMemcpy(q.data(), c->trk + (size_t)(field - SHY_DISCHARGE_TRACK) * n,
                           n * sizeof(long long), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i) dst[i] = (float)((double)q[i] * kTQInv);
       break;
@@ -526,7 +561,9 @@ int shy_generate_fbm(shy_ctx* c, uint64_t seed, int octaves, float base_freq, fl
   HIPCHK(c, hipGetLastError());
   minmax_partial_kernel<<<kPartBlocks, 256, 0, c->stream>>>(out, c->n, c->part);
   HIPCHK(c, hipGetLastError());
-  normalize_kernel<<<grid, bs, 0, c->stream>>>(out, c->n, c->part, kPartBlocks);
+  minmax_fold_kernel<<<1, 256, 0, c->stream>>>(c->part, kPartBlocks);
+  HIPCHK(c, hipGetLastError());
+  normalize_kernel<<<grid, bs, 0, c->stream>>>(out, c->n, c->part);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SHY_OK;
@@ -536,7 +573,9 @@ int shy_erode(shy_ctx* c, int drops) {
   if (!c) return SHY_ERR_ARG;
   if (drops < 0) return fail(c, SHY_ERR_ARG, "shy_erode: drops must be >= 0");
   hipSetDevice(c->device);
-  int r = launch_cycle(c, drops, parity(c));
+  int r = ensure_wave(c, drops);
+  if (r) return r;
+  r = launch_cycle(c, drops, parity(c));
   if (r) return r;
   c->cycle += 1;
   return SHY_OK;
@@ -559,6 +598,8 @@ int shy_erode_cycles(shy_ctx* c, int cycles, int drops, int use_graph) {
     --cycles;
   }
   if (cycles >= 2) {
+    int rw = ensure_wave(c, drops);
+    if (rw) return rw;
     if (!c->graph || c->graph_drops != drops) {
       drop_graph(c);
       hipGraph_t g;
@@ -638,13 +679,13 @@ int shy_descend(shy_ctx* c, shy_drop* drop, int* alive) {
   hipSetDevice(c->device);
   if (!c->drop_buf) HIPCHK(c, hipMalloc(&c->drop_buf, sizeof(shy_drop) + sizeof(int)));
   LiveArgs a;
-  const int p = parity(c), last = p ^ 1;
+  const int p = parity(c);
   a.h = c->h[p];
   a.fld = c->fld;
   a.root = c->root;
-  a.td = c->trk[last];
-  a.tmx = c->trk[last] + c->n;
-  a.tmy = c->trk[last] + 2 * c->n;
+  a.td = c->trk;
+  a.tmx = c->trk + c->n;
+  a.tmy = c->trk + 2 * c->n;
   a.erf_a = c->erf_a;
   a.nx = c->nx;
   a.ny = c->ny;
@@ -699,7 +740,9 @@ int shy_time_cycles(shy_ctx* c, int cycles, int drops, float* ms) {
   if (!c || !ms || cycles < 1 || drops < 0)
     return fail(c, SHY_ERR_ARG, "shy_time_cycles: bad argument");
   hipSetDevice(c->device);
-  // per cycle: start, after count walk, after scale, after commit walk; + end
+  int rw = ensure_wave(c, drops);
+  if (rw) return rw;
+  // per cycle: start, after count walk, after scan, after commit walk; + end
   const size_t ne = 4 * (size_t)cycles + 1;
   std::vector<hipEvent_t> ev(ne, nullptr);
   auto cleanup = [&]() {

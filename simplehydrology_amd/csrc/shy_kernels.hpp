@@ -1,32 +1,36 @@
 // shy_kernels.hpp -- kernels of the erosion hot path (gfx950, wave64).
 //
 // One cycle (= one World::erode call, source/world.h:54-88) is one wave of
-// drops that all read the map as it stood when the cycle began, in four
-// launches:
+// drops that all read the map as it stood when the cycle began:
 //
-//  walk_kernel<kCount>   one drop per lane walks its trajectory and counts,
-//                        with relaxed int32 atomics, the steps that start at
-//                        each cell.
-//  scale_kernel          per cell, the burst scale s = (1-(1-gamma)^k)/(gamma k)
-//                        with gamma = min(1, effD (1 + entrainment erf(0.4 d))):
-//                        k drops through one cell on the same snapshot then
-//                        commit together what k drops in sequence would
-//                        (water.h:127-132 as a linear feedback), which keeps the
-//                        wave stable where crowded cells would overshoot.
-//  walk_kernel<kCommit>  the same trajectories again; erosion/deposition
-//                        effD*cdiff*s, quantised once to int32 (2^-20) and
-//                        moved between map and drop as that integer, and the discharge and
-//                        momentum tracks (int64, 2^-32) go through relaxed
-//                        integer atomics, so the result does not depend on the
-//                        order in which drops run; arrival cells (where the
-//                        reference calls World::cascade, water.h:151) are marked.
+//  walk_kernel<kCount>   one drop per lane walks its trajectory; for every
+//                        step it takes a ticket (relaxed int32 atomic with
+//                        return) on the step's start cell and logs it.
+//  scan_*_kernel         exclusive prefix sum of the per-cell step counts
+//                        (bucket offsets), fused with the per-cell burst scale
+//                        s = (1-(1-gamma)^k)/(gamma k), gamma = min(1, effD (1 +
+//                        entrainment erf(0.4 d))): the k drops through one cell
+//                        on the same snapshot commit together what k drops in
+//                        sequence would (water.h:127-132 as a linear feedback),
+//                        which keeps the wave stable where crowded cells would
+//                        overshoot.
+//  walk_kernel<kCommit>  the same trajectories again; each step's commit --
+//                        erosion/deposition effD*cdiff*s quantised once to
+//                        int32 (2^-20, moved between map and drop as that
+//                        integer) and the discharge/momentum tracks (int32,
+//                        2^-24) -- goes with one plain 16-byte store into the
+//                        start cell's bucket at its ticket: no read-modify-write
+//                        of the map per step.
+//  reduce_kernel         per cell, sums its bucket in integers (so the result
+//                        does not depend on the order drops ran) plus the
+//                        terminal deposits.
 //  cascade_blend_kernel  2D-tiled stencil: applies the walk's commits, runs the
-//                        reference cascade (world.h:90-168) of every marked
-//                        cell as a weighted Jacobi sweep (LDS tile, 2-cell
-//                        height halo, 3-cell mark halo), applies the quantised
-//                        transfers, blends discharge/momentum (world.h:80-86),
-//                        refreshes erf(0.4 d) and clears the other parity's
-//                        accumulators for the next cycle.
+//                        reference cascade (world.h:90-168) of every cell a drop
+//                        arrived at (water.h:151) as a weighted Jacobi sweep
+//                        (LDS tile, 2-cell height halo, 3-cell mark halo),
+//                        applies the quantised transfers, blends
+//                        discharge/momentum (world.h:80-86), refreshes
+//                        erf(0.4 d) and clears the other parity's marks.
 #pragma once
 
 #include "shy_device.hpp"
@@ -48,13 +52,13 @@ struct WalkArgs {
   const float* __restrict__ h;
   const float4* __restrict__ fld;  // {momentumx, momentumy, discharge, erf(0.4*discharge)}
   const float* __restrict__ root;  // nullptr when the map has no root density
-  int32_t* dh;                     // height commits, 2^-20
-  long long* td;                   // discharge track, 2^-32
-  long long* tmx;
-  long long* tmy;
+  int32_t* count;                  // steps starting at each cell (tickets)
+  int32_t* ticket;                 // [step][drop] ticket log
+  const int32_t* __restrict__ offset;  // bucket start per cell
+  const float* __restrict__ scale;     // burst scale per cell (commit pass)
+  int4* rec;                       // buckets: {-dh, volume, vol*sx, vol*sy} per step
+  int32_t* deposit;                // terminal deposits, 2^-20
   uint8_t* visit;                  // cascade sites
-  int32_t* count;                  // walk steps starting at each cell (count pass)
-  const float* __restrict__ scale; // burst scale per cell (commit pass)
   DeviceState* state;
   uint64_t seed;
   uint32_t terrain;
@@ -66,16 +70,12 @@ struct WalkArgs {
 struct CascArgs {
   const float* __restrict__ h_in;
   float* __restrict__ h_out;
-  const int32_t* __restrict__ dh;
+  const int32_t* __restrict__ dh;     // walk commits per cell, 2^-20
   const uint8_t* __restrict__ visit;
-  const long long* __restrict__ td;
+  const long long* __restrict__ td;   // tracks per cell, 2^-24
   const long long* __restrict__ tmx;
   const long long* __restrict__ tmy;
-  int32_t* dh_clear;
   uint8_t* visit_clear;
-  long long* td_clear;
-  long long* tmx_clear;
-  long long* tmy_clear;
   float4* fld;
   const double* __restrict__ erf_a;
   DeviceState* state;
@@ -145,7 +145,7 @@ enum WalkMode { kTrace = 0, kCount = 1, kCommit = 2 };
 // states of one drop on the frozen map (unscaled, as Drop::descend);
 // kCount only counts steps per start cell; kCommit commits.
 template <int MODE>
-__device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
+__device__ void walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, LaneStats& S,
                           shy_drop_step* tr, int max_tr, int* tr_n) {
   constexpr bool TRACE = MODE == kTrace;
   constexpr bool COMMIT = MODE == kCommit;
@@ -165,25 +165,27 @@ __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
     if ((float)age > P.maxAge || vol < P.minVol) {  // water.h:74-82
       if (COMMIT) {
         const int32_t q = quant_h(sed);
-        atomic_add_i32(&a.dh[i0], q);
+        atomic_add_i32(&a.deposit[i0], q);
         S.walk_dh += q;
         if ((float)age > P.maxAge) S.ends_age++; else S.ends_vol++;
       }
       break;
     }
+    // step k of this drop (k = age) holds ticket t on its start cell
+    const size_t klog = (size_t)age * a.drops + lane_drop;
+    if (MODE == kCount) {
+      a.ticket[klog] =
+          __hip_atomic_fetch_add(&a.count[i0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    int slot = 0;
+    if (COMMIT) slot = a.offset[i0] + a.ticket[klog];
     S.steps++;
-    if (MODE == kCount) atomic_add_i32(&a.count[i0], 1);
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
     step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);  // water.h:95-111
     const int jx = (int)px, jy = (int)py;
     const bool ex = off_map(jx, jy, nx, ny);
     const int j = jx * ny + jy;
-    if (COMMIT) {  // water.h:115-117
-      atomic_add_i64(&a.td[i0], quant_t(vol));
-      atomic_add_i64(&a.tmx[i0], quant_t(vol * sx));
-      atomic_add_i64(&a.tmy[i0], quant_t(vol * sy));
-    }
     CellView cn = c;
     if (!ex) cn = load_cell(a, jx, jy, j);
     const float h2 = ex ? (float)((double)c.h - 0.002) : cn.h;  // water.h:120-124
@@ -192,10 +194,11 @@ __device__ void walk_drop(const WalkArgs& a, float x0, float y0, LaneStats& S,
     const float cdiff = (c_eq - sed);
     float ero;  // water.h:131-132
     if (COMMIT) {
-      // one quantised amount leaves the map and enters the drop
+      // one quantised amount leaves the map and enters the drop; the step's
+      // tracks (water.h:115-117) ride in the same bucket record
       const int32_t q = quant_h((effD * cdiff) * c.scale);
       ero = (float)q * kHQInv;
-      atomic_add_i32(&a.dh[i0], -q);
+      a.rec[slot] = make_int4(-q, quant_t(vol), quant_t(vol * sx), quant_t(vol * sy));
       S.walk_dh -= q;
     } else {
       ero = effD * cdiff;
@@ -245,7 +248,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a) {
       skipped = 1;
     } else {
       walked = 1;
-      walk_drop<MODE>(a, (float)(cid / (uint32_t)a.ny), (float)(cid % (uint32_t)a.ny), S,
+      walk_drop<MODE>(a, tid, (float)(cid / (uint32_t)a.ny), (float)(cid % (uint32_t)a.ny), S,
                       nullptr, 0, nullptr);
     }
   }
@@ -276,22 +279,149 @@ __device__ inline float burst_scale(int k, float effD, float entrainment, float 
   return (1.0f - powi_f32(1.0f - g, k)) / (g * (float)k);
 }
 
-__global__ void scale_kernel(int32_t* count, float* scale, const float4* __restrict__ fld,
-                             const float* __restrict__ root, size_t n, shy_params P) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// ----------------------------------------------------- bucket offsets
+constexpr int kScanBlock = 1024;
+
+// exclusive scan of v over the block's kScanBlock threads (Hillis-Steele in
+// LDS); total receives the block sum
+__device__ inline int block_exclusive_scan(int v, int* tmp, int& total) {
+  const int t = threadIdx.x;
+  tmp[t] = v;
+  __syncthreads();
+  for (int o = 1; o < kScanBlock; o <<= 1) {
+    const int add = t >= o ? tmp[t - o] : 0;
+    __syncthreads();
+    tmp[t] += add;
+    __syncthreads();
+  }
+  total = tmp[kScanBlock - 1];
+  const int incl = tmp[t];
+  __syncthreads();
+  return incl - v;
+}
+
+// per-block sums of the step counts
+__global__ void __launch_bounds__(kScanBlock) scan_sums_kernel(const int32_t* __restrict__ count,
+                                                               size_t n, int32_t* bsum) {
+  __shared__ int tmp[kScanBlock];
+  const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
+  int total;
+  block_exclusive_scan(i < n ? count[i] : 0, tmp, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// exclusive scan of the block sums in one block (each thread a run of them)
+__global__ void __launch_bounds__(kScanBlock) scan_top_kernel(int32_t* bsum, int nb) {
+  __shared__ int tmp[kScanBlock];
+  const int per = (nb + kScanBlock - 1) / kScanBlock;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int b = b0; b < b0 + per && b < nb; ++b) s += bsum[b];
+  int total;
+  int run = block_exclusive_scan(s, tmp, total);
+  for (int b = b0; b < b0 + per && b < nb; ++b) {
+    const int v = bsum[b];
+    bsum[b] = run;
+    run += v;
+  }
+}
+
+// bucket offsets and burst scale per cell
+__global__ void __launch_bounds__(kScanBlock) scan_cells_kernel(
+    const int32_t* __restrict__ count, const int32_t* __restrict__ bsum, size_t n,
+    int32_t* offset, float* scale, const float4* __restrict__ fld,
+    const float* __restrict__ root, shy_params P) {
+  __shared__ int tmp[kScanBlock];
+  const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
+  const int k = i < n ? count[i] : 0;
+  int total;
+  const int ex = block_exclusive_scan(k, tmp, total);
   if (i >= n) return;
-  const int k = count[i];
+  offset[i] = bsum[blockIdx.x] + ex;
   float effD = P.depositionRate * (1.0f - (root ? root[i] : 0.0f));
   if (effD < 0) effD = 0;
   scale[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
-  count[i] = 0;
+}
+
+// Per cell: sum its bucket and terminal deposits; clear the tickets.  The
+// buckets of a block's kReduceCells cells are one contiguous record range.
+// It is staged through LDS in chunks with coalesced loads; each thread then
+// sums an equal slice of the chunk, flushing a run sum into the cell's LDS
+// accumulator whenever it crosses a bucket boundary, so a cell holding
+// thousands of records (a river) is shared by the whole block.
+constexpr int kReduceCells = 256;
+constexpr int kReduceChunk = 2048;  // records per LDS chunk (32 KiB)
+
+__global__ void __launch_bounds__(kReduceCells) reduce_kernel(
+    int32_t* count, const int32_t* __restrict__ offset, const int4* __restrict__ rec,
+    int32_t* deposit, size_t n, int32_t* dh, long long* td, long long* tmx, long long* tmy) {
+  __shared__ int off[kReduceCells + 1];
+  __shared__ unsigned long long acc[4][kReduceCells];
+  __shared__ int4 buf[kReduceChunk];
+  const int t = threadIdx.x;
+  const size_t c0 = (size_t)blockIdx.x * kReduceCells;
+  const int m = (int)((n - c0) < (size_t)kReduceCells ? (n - c0) : (size_t)kReduceCells);
+  if (t < m) off[t] = offset[c0 + t];
+  if (t == m - 1) off[m] = offset[c0 + t] + count[c0 + t];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) acc[f][t] = 0;
+  __syncthreads();
+  const int r0 = off[0], r1 = off[m];
+  constexpr int kPer = kReduceChunk / kReduceCells;
+  for (int base = r0; base < r1; base += kReduceChunk) {
+    const int len = r1 - base < kReduceChunk ? r1 - base : kReduceChunk;
+    for (int k = t; k < len; k += kReduceCells) buf[k] = rec[base + k];
+    __syncthreads();
+    int k = t * kPer;
+    const int kend = k + kPer < len ? k + kPer : len;
+    if (k < kend) {
+      const int r = base + k;
+      // bucket j holding record r: the last j with off[j] <= r
+      int lo = 0, hi = m - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= r) lo = mid; else hi = mid - 1;
+      }
+      int j = lo;
+      long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      for (; k < kend; ++k) {
+        while (base + k >= off[j + 1]) {
+          atomicAdd(&acc[0][j], (unsigned long long)s0);
+          atomicAdd(&acc[1][j], (unsigned long long)s1);
+          atomicAdd(&acc[2][j], (unsigned long long)s2);
+          atomicAdd(&acc[3][j], (unsigned long long)s3);
+          s0 = s1 = s2 = s3 = 0;
+          ++j;
+        }
+        const int4 v = buf[k];
+        s0 += v.x;
+        s1 += v.y;
+        s2 += v.z;
+        s3 += v.w;
+      }
+      atomicAdd(&acc[0][j], (unsigned long long)s0);
+      atomicAdd(&acc[1][j], (unsigned long long)s1);
+      atomicAdd(&acc[2][j], (unsigned long long)s2);
+      atomicAdd(&acc[3][j], (unsigned long long)s3);
+    }
+    __syncthreads();
+  }
+  if (t < m) {
+    const size_t i = c0 + t;
+    dh[i] = (int32_t)((long long)acc[0][t] + deposit[i]);
+    td[i] = (long long)acc[1][t];
+    tmx[i] = (long long)acc[2][t];
+    tmy[i] = (long long)acc[3][t];
+    count[i] = 0;
+    deposit[i] = 0;
+  }
 }
 
 __global__ void trace_kernel(WalkArgs a, float x0, float y0, shy_drop_step* out, int max_steps,
                              int* n_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   LaneStats S;
-  walk_drop<kTrace>(a, x0, y0, S, out, max_steps, n_out);
+  walk_drop<kTrace>(a, 0, x0, y0, S, out, max_steps, n_out);
 }
 
 // --------------------------------------------------------------- cascade
@@ -450,11 +580,7 @@ __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
     const float mx = keep * f.x + a.P.lrate * tX;
     const float my = keep * f.y + a.P.lrate * tY;
     a.fld[i] = make_float4(mx, my, d, erf_f32(a.erf_a, 0.4f * d));
-    a.dh_clear[i] = 0;
     a.visit_clear[i] = 0;
-    a.td_clear[i] = 0;
-    a.tmx_clear[i] = 0;
-    a.tmy_clear[i] = 0;
   }
   long long v[1] = {moves};
   block_add_stats<1>(v, &a.state->stats[ST_CASC_MOVES]);
@@ -563,17 +689,34 @@ __global__ void minmax_partial_kernel(const float* __restrict__ v, size_t n, flo
   if (threadIdx.x == 0) part[blockIdx.x] = make_float2(smn[0], smx[0]);
 }
 
-__global__ void normalize_kernel(float* v, size_t n, const float2* part, int nparts) {
+// fold the per-block partials into part[0] (one block)
+__global__ void minmax_fold_kernel(float2* part, int nparts) {
+  __shared__ float smn[256], smx[256];
   float mn = __builtin_inff(), mx = -__builtin_inff();
-  for (int k = 0; k < nparts; ++k) {
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
     const float2 p = part[k];
     mn = p.x < mn ? p.x : mn;
     mx = p.y > mx ? p.y : mx;
   }
-  const float range = mx - mn;
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smn[threadIdx.x] = smn[threadIdx.x + s] < smn[threadIdx.x] ? smn[threadIdx.x + s] : smn[threadIdx.x];
+      smx[threadIdx.x] = smx[threadIdx.x + s] > smx[threadIdx.x] ? smx[threadIdx.x + s] : smx[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[0] = make_float2(smn[0], smx[0]);
+}
+
+__global__ void normalize_kernel(float* v, size_t n, const float2* part) {
+  const float2 p = part[0];
+  const float range = p.y - p.x;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  v[i] = range > 0.0f ? (v[i] - mn) / range : 0.0f;
+  v[i] = range > 0.0f ? (v[i] - p.x) / range : 0.0f;
 }
 
 // ------------------------------------------ single-site reference kernels
