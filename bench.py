@@ -66,6 +66,62 @@ def cpu_baseline(n: int, drops: int, seed: int, cycles: int) -> dict:
             "seconds": dt, "particle_steps": steps}
 
 
+def ensemble_specs(shy, n_terrains: int, seed: int):
+    """64 terrains, sides uniform over {128,...,1024} from the seed, each with
+    its own fBm seed, evapRate in [0.0005, 0.002], depositionRate in
+    [0.05, 0.2] and 1 drop per 16 cells."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = (128, 256, 384, 512, 768, 1024)
+    specs = []
+    for t in range(n_terrains):
+        s = int(rng.choice(sides))
+        p = shy.Params(evapRate=float(rng.uniform(0.0005, 0.002)),
+                       depositionRate=float(rng.uniform(0.05, 0.2)))
+        specs.append(dict(nx=s, ny=s, drops=s * s // 16, seed=int(rng.integers(1, 2**62)),
+                          params=p))
+    return specs
+
+
+def run_ensemble(shy, args) -> dict:
+    specs = ensemble_specs(shy, args.terrains, args.seed)
+    e = shy.Ensemble(specs)
+    for t, s in enumerate(specs):
+        e.generate_fbm(t, s["seed"], 8)
+    tuned, tuned_ms = None, None
+    if not args.no_autotune:
+        tuned, tuned_ms = e.autotune(2)
+    if args.warmup:
+        e.erode(args.warmup, graph=True)
+    e.synchronize()
+    s0 = e.stats()["steps"]
+    tm = e.time_cycles(args.steps)
+    steps = e.stats()["steps"] - s0
+    sec = tm["ms_total"] / 1e3
+    cells = sum(s["nx"] * s["ny"] for s in specs)
+    drops = sum(s["drops"] for s in specs)
+    steps_per_launch = steps / args.steps
+    achieved = steps_per_launch * B_COMMIT / (tm["ms_commit_walk"] / 1e3 / args.steps) / 1e9
+    e.close()
+    return {
+        "metric": "particle-steps/s, ensemble of uneven terrains",
+        "value": steps / sec, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": tm["ms_total"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched ensemble: {len(specs)} fBm terrains, sides in "
+                               "{128..1024}, per-terrain params, 1 drop per 16 cells",
+                   "cells": cells, "drops_per_cycle": drops, "tuning": tuned,
+                   "tuned_ms_per_cycle": tuned_ms},
+        "cycles_per_s": args.steps / sec, "particle_steps": steps,
+        "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
+        "roofline": {"bound": "hbm", "kernel": "walk_kernel<kCommit>", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_step": B_COMMIT},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +133,11 @@ def main():
     ap.add_argument("--walk-block", type=int, default=0)
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("single", "ensemble"), default="single",
+                    help="single: one 1024^2 terrain; ensemble: 64 uneven terrains in one "
+                         "launch per kernel")
+    ap.add_argument("--terrains", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -99,6 +160,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.workload == "ensemble":
+        print(json.dumps(run_ensemble(shy, args)))
+        return
 
     n, drops = args.size, args.drops
     w = shy.World(n, n, seed=args.seed)

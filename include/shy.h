@@ -101,6 +101,26 @@ typedef struct shy_stats {
   int64_t casc_moves; /* non-zero cascade transfers */
 } shy_stats;
 
+/* One terrain of an ensemble: its size, its drops per cycle (World::erode's
+   argument), its World::SEED and its parameter set. */
+typedef struct shy_terrain_desc {
+  int nx, ny;
+  int drops;
+  uint64_t seed;
+  shy_params params;
+} shy_terrain_desc;
+
+/* Launch shape (see shy_autotune):
+   walk_block  threads per block of the walk kernels (64..1024, multiple of 64)
+   walk_grid   0: one lane per drop; > 0: a persistent grid of that many blocks
+               walking the wave's drops grid-stride
+   tile        cascade/blend stencil tile: 0 = 32x8, 1 = 64x4, 2 = 16x16 cells */
+typedef struct shy_tuning {
+  int walk_block;
+  int walk_grid;
+  int tile;
+} shy_tuning;
+
 typedef struct shy_ctx shy_ctx;
 
 /* Defaults of the reference statics (water.h:43-50, world.h:42-44). */
@@ -114,6 +134,26 @@ int shy_device_count(void);
    (SimpleHydrology.cpp:36-38, source/cellpool.h:323-340); World::SEED
    (source/world.h:21) is `seed`. */
 int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out);
+/* An ensemble of terrains of any sizes and parameter sets in one context;
+   every cycle runs all of them in one launch per kernel.  No reference
+   counterpart: the reference holds one static World (world.h:17-44).
+   Terrain t spawns from the hash of (seed_t, t, cycle).  The single-terrain
+   entry points below are valid only when nter == 1 (else SHY_ERR_STATE); the
+   shy_terrain_* ones take a terrain index. */
+int shy_ensemble_create(const shy_terrain_desc* terrains, int nter, shy_ctx** out);
+int shy_ensemble_count(const shy_ctx* ctx, int* nter);
+int shy_terrain_size(const shy_ctx* ctx, int terrain, int* nx, int* ny, int* drops);
+int shy_terrain_set_params(shy_ctx* ctx, int terrain, const shy_params* p);
+int shy_terrain_set_drops(shy_ctx* ctx, int terrain, int drops);
+int shy_terrain_upload_field(shy_ctx* ctx, int terrain, int field, const float* src);
+int shy_terrain_download_field(shy_ctx* ctx, int terrain, int field, float* dst);
+int shy_terrain_generate_fbm(shy_ctx* ctx, int terrain, uint64_t seed, int octaves,
+                             float base_freq, float lacunarity, float gain);
+/* particle-steps per terrain since the last shy_reset_stats */
+int shy_terrain_steps(shy_ctx* ctx, int64_t* steps);
+/* `cycles` World::erode calls of every terrain (each with its own drops);
+   use_graph=1 replays two-cycle hipGraphs. */
+int shy_ensemble_erode(shy_ctx* ctx, int cycles, int use_graph);
 int shy_destroy(shy_ctx* ctx);
 const char* shy_last_error(const shy_ctx* ctx); /* never NULL */
 int shy_set_params(shy_ctx* ctx, const shy_params* p);
@@ -166,6 +206,7 @@ int shy_descend(shy_ctx* ctx, shy_drop* drop, int* alive);
 
 /* Spawn stream state: World::SEED and the number of cycles run. */
 int shy_set_seed(shy_ctx* ctx, uint64_t seed, uint32_t cycle);
+int shy_set_cycle(shy_ctx* ctx, uint32_t cycle);
 int shy_get_cycle(const shy_ctx* ctx, uint32_t* cycle);
 
 int shy_get_stats(shy_ctx* ctx, shy_stats* out);
@@ -175,11 +216,20 @@ int shy_synchronize(shy_ctx* ctx);
 /* Launch shape of the drop walk: threads per block (64..1024, multiple of
    64).  0 restores the default. */
 int shy_set_walk_block(shy_ctx* ctx, int threads);
+int shy_set_tuning(shy_ctx* ctx, const shy_tuning* t);
+int shy_get_tuning(const shy_ctx* ctx, shy_tuning* t);
+/* Autotuner (no reference counterpart): times `cycles` cycles of every
+   candidate launch shape (4 walk block sizes x one-lane-per-drop or
+   persistent x 3 stencil tiles) from a copy of the live state, restores the
+   state after each, keeps the fastest and returns it with its ms per cycle.
+   Results are bit-identical under every shape. */
+int shy_autotune(shy_ctx* ctx, int cycles, shy_tuning* chosen, float* best_ms);
 
-/* Measurement (no reference counterpart): run `cycles` World::erode(drops)
-   calls eagerly with HIP events on the context's stream between kernels.
-   ms[0] = wall time of the whole sequence; ms[1..4] = summed durations of
-   the count walk, the scale pass, the commit walk and the cascade/blend
+/* Measurement (no reference counterpart): run `cycles` cycles eagerly with
+   HIP events on the context's stream between kernels.  drops >= 0 sets the
+   drops of a single-terrain context first; drops < 0 keeps every terrain's
+   own.  ms[0] = wall time of the whole sequence; ms[1..4] = summed durations
+   of the count walk, the scan, the commit walk and the reduce + cascade/blend
    stencil, in milliseconds. */
 int shy_time_cycles(shy_ctx* ctx, int cycles, int drops, float ms[5]);
 

@@ -70,6 +70,18 @@ class DropState(C.Structure):
                 ("volume", C.c_float), ("sediment", C.c_float)]
 
 
+class TerrainDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("drops", C.c_int), ("seed", C.c_uint64),
+                ("params", Params)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("walk_block", C.c_int), ("walk_grid", C.c_int), ("tile", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "steps", "walked", "skipped", "exits", "ends_age", "ends_vol", "export40", "gain40",
@@ -117,6 +129,21 @@ def lib():
             "shy_synchronize": (i32, [vp]),
             "shy_set_walk_block": (i32, [vp, i32]),
             "shy_time_cycles": (i32, [vp, i32, i32, fp]),
+            "shy_ensemble_create": (i32, [vp, i32, C.POINTER(vp)]),
+            "shy_ensemble_count": (i32, [vp, C.POINTER(i32)]),
+            "shy_terrain_size": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+            "shy_terrain_set_params": (i32, [vp, i32, vp]),
+            "shy_terrain_set_drops": (i32, [vp, i32, i32]),
+            "shy_terrain_upload_field": (i32, [vp, i32, i32, vp]),
+            "shy_terrain_download_field": (i32, [vp, i32, i32, vp]),
+            "shy_terrain_generate_fbm": (i32, [vp, i32, u64, i32, C.c_float, C.c_float,
+                                              C.c_float]),
+            "shy_terrain_steps": (i32, [vp, vp]),
+            "shy_ensemble_erode": (i32, [vp, i32, i32]),
+            "shy_set_cycle": (i32, [vp, u32]),
+            "shy_set_tuning": (i32, [vp, vp]),
+            "shy_get_tuning": (i32, [vp, vp]),
+            "shy_autotune": (i32, [vp, i32, vp, fp]),
             "shy_download_normals": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -293,13 +320,28 @@ class World:
     def set_walk_block(self, threads: int):
         _check(self._ctx, lib().shy_set_walk_block(self._ctx, threads))
 
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(self._ctx, lib().shy_get_tuning(self._ctx, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, walk_block=256, walk_grid=0, tile=0):
+        t = Tuning(walk_block, walk_grid, tile)
+        _check(self._ctx, lib().shy_set_tuning(self._ctx, C.byref(t)))
+
+    def autotune(self, cycles: int = 3):
+        """Pick the fastest launch shape (results are identical under all)."""
+        t, ms = Tuning(), C.c_float()
+        _check(self._ctx, lib().shy_autotune(self._ctx, cycles, C.byref(t), C.byref(ms)))
+        return t.as_dict(), ms.value
+
     def time_cycles(self, cycles: int, drops: int) -> dict:
         """Run ``cycles`` erode calls with HIP events between kernels on the
         context's stream; returns milliseconds per kernel, summed."""
         ms = (C.c_float * 5)()
         _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, drops, ms))
-        return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scale": ms[2],
-                "ms_commit_walk": ms[3], "ms_cascade": ms[4]}
+        return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scan": ms[2],
+                "ms_commit_walk": ms[3], "ms_reduce_cascade": ms[4]}
 
 
 class Drop:
@@ -326,3 +368,54 @@ class Drop:
         _check(self.world._ctx, lib().shy_descend(self.world._ctx, C.byref(self.state),
                                                   C.byref(alive)))
         return bool(alive.value)
+
+
+class Ensemble(World):
+    """Many terrains of uneven sizes and parameter sets in one context: every
+    cycle runs all of them in one launch per kernel (hipGraph-replayable)."""
+
+    def __init__(self, terrains):
+        """terrains: iterable of dicts {nx, ny, drops, seed, params (Params)}."""
+        ts = list(terrains)
+        arr = (TerrainDesc * len(ts))()
+        for i, t in enumerate(ts):
+            arr[i].nx, arr[i].ny = int(t["nx"]), int(t.get("ny", t["nx"]))
+            arr[i].drops, arr[i].seed = int(t["drops"]), int(t["seed"])
+            arr[i].params = t.get("params") or Params()
+        self._ctx = C.c_void_p()
+        _check(None, lib().shy_ensemble_create(arr, len(ts), C.byref(self._ctx)))
+        self.nter = len(ts)
+        self.sizes = [(arr[i].nx, arr[i].ny) for i in range(len(ts))]
+        self.nx, self.ny = self.sizes[0]
+        self.map = None
+
+    def generate_fbm(self, terrain: int, seed: int, octaves: int = 8, base_freq: float = 0.0,
+                     lacunarity: float = 2.0, gain: float = 0.5):
+        _check(self._ctx, lib().shy_terrain_generate_fbm(self._ctx, terrain, seed, octaves,
+                                                          base_freq, lacunarity, gain))
+
+    def field(self, terrain: int, field: str) -> np.ndarray:
+        nx, ny = self.sizes[terrain]
+        out = np.empty((nx, ny), np.float32)
+        _check(self._ctx, lib().shy_terrain_download_field(self._ctx, terrain, FIELD_ID[field],
+                                                            out.ctypes.data))
+        return out
+
+    def upload(self, terrain: int, field: str, values):
+        a = _f32(values, self.sizes[terrain])
+        _check(self._ctx, lib().shy_terrain_upload_field(self._ctx, terrain, FIELD_ID[field],
+                                                          a.ctypes.data))
+
+    def erode(self, cycles: int = 1, graph: bool = True):
+        _check(self._ctx, lib().shy_ensemble_erode(self._ctx, cycles, 1 if graph else 0))
+
+    def terrain_steps(self) -> np.ndarray:
+        out = np.zeros(self.nter, np.int64)
+        _check(self._ctx, lib().shy_terrain_steps(self._ctx, out.ctypes.data))
+        return out
+
+    def time_cycles(self, cycles: int) -> dict:
+        ms = (C.c_float * 5)()
+        _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, -1, ms))
+        return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scan": ms[2],
+                "ms_commit_walk": ms[3], "ms_reduce_cascade": ms[4]}

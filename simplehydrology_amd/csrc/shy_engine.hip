@@ -1,6 +1,8 @@
 // shy_engine.hip -- C ABI (include/shy.h) over the gfx950 kernels.
 //
-// A context holds one terrain in HBM as structure-of-arrays, x-major:
+// A context holds an ensemble of terrains (a single-terrain context is an
+// ensemble of one) in HBM as concatenated structure-of-arrays, each terrain
+// x-major in its own slice:
 //   h[2]        float   heights, ping-pong (the cascade stencil reads a halo)
 //   fld         float4  {momentumx, momentumy, discharge, erf(0.4 discharge)}
 //   root        float   root density
@@ -13,14 +15,17 @@
 //                       readable until the next cycle, as the reference's
 //                       *_track fields do)
 //   deposit     int32   terminal deposits, 2^-20; cleared by reduce
-// and per wave (sized by drops x (maxAge+1) steps):
+// and per wave (sized by total drops x (max maxAge + 1) steps):
 //   ticket      int32   [step][drop] ticket log of the count pass
 //   rec         int4    per-step commit records, bucketed by start cell
+// A device table of Terrain entries maps drops, cells and cascade tiles to
+// their terrain, so every kernel covers the whole ensemble in one launch.
 // Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle
 // p clears visit[p^1] for the next cycle, so a cycle needs no memset and two
 // cycles form a replayable hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,15 +38,15 @@ using namespace shy;
 
 struct shy_ctx {
   int device = 0;
-  int nx = 0, ny = 0;
-  size_t n = 0;
-  shy_params P{};
-  uint64_t seed = 1;
-  uint32_t cycle = 0;  // host mirror of state->cycle
+  std::vector<Terrain> ter;  // host copy of the table
+  Terrain* d_ter = nullptr;
+  size_t n = 0;              // cells, all terrains
+  int total_drops = 0;
+  int total_tiles = 0;
+  uint32_t cycle = 0;        // host mirror of state->cycle
   float* h[2] = {nullptr, nullptr};
   float4* fld = nullptr;
   float* root = nullptr;
-  bool has_root = false;
   uint8_t* visit[2] = {nullptr, nullptr};
   int32_t* count = nullptr;
   int32_t* offset = nullptr;
@@ -53,6 +58,7 @@ struct shy_ctx {
   int32_t* ticket = nullptr;
   int4* rec = nullptr;
   size_t wave_cap = 0;  // steps the ticket log and buckets can hold
+  long long* tsteps = nullptr;
   DeviceState* state = nullptr;
   double* erf_a = nullptr;
   float2* part = nullptr;  // min/max partials
@@ -62,9 +68,8 @@ struct shy_ctx {
   int* trace_n = nullptr;
   void* drop_buf = nullptr;
   hipStream_t stream = nullptr;
-  int walk_block = 256;
+  shy_tuning tune{256, 0, 0};
   hipGraphExec_t graph = nullptr;
-  int graph_drops = -1;
   std::string err;
 };
 
@@ -72,6 +77,9 @@ namespace {
 
 constexpr int kPartBlocks = 1024;
 constexpr float kMaxAgeLimit = 1.0e6f;
+constexpr int kTileY[3] = {32, 64, 16};
+constexpr int kTileX[3] = {8, 4, 16};
+constexpr int kWaveDrops = 64;  // drop ranges are padded to whole wave64s
 
 std::string g_noctx_err;
 
@@ -85,6 +93,12 @@ int fail(shy_ctx* c, int code, const std::string& msg) {
     hipError_t e_ = (call);                                                             \
     if (e_ != hipSuccess)                                                               \
       return fail((ctx), SHY_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define RETURN_IF(r)   \
+  do {                 \
+    int r_ = (r);      \
+    if (r_) return r_; \
   } while (0)
 
 void erf_coefficients(double* a) {
@@ -120,40 +134,51 @@ bool params_ok(const shy_params* p) {
   return p->evapRate < 1.0f && p->maxAge >= 0.0f && p->maxAge <= kMaxAgeLimit;
 }
 
+const char* kParamsMsg = "parameter not finite, evapRate >= 1 or maxAge outside [0, 1e6]";
+
 int parity(const shy_ctx* c) { return (int)(c->cycle & 1u); }
 
-WalkArgs walk_args(shy_ctx* c, int drops) {
-  const int p = parity(c);
-  WalkArgs a;
-  a.h = c->h[p];
-  a.fld = c->fld;
-  a.root = c->has_root ? c->root : nullptr;
-  a.count = c->count;
-  a.ticket = c->ticket;
-  a.offset = c->offset;
-  a.scale = nullptr;
-  a.rec = c->rec;
-  a.deposit = c->deposit;
-  a.visit = c->visit[p];
-  a.state = c->state;
-  a.seed = c->seed;
-  a.terrain = 0;
-  a.nx = c->nx;
-  a.ny = c->ny;
-  a.drops = drops;
-  a.P = c->P;
-  return a;
+void drop_graph(shy_ctx* c) {
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
 }
 
-void drop_graph(shy_ctx* c);
-
-size_t wave_steps(const shy_ctx* c, int drops) {
-  return (size_t)drops * ((size_t)c->P.maxAge + 1);
+// Offsets of every terrain's cells, drops and cascade tiles; uploads the table.
+int layout(shy_ctx* c) {
+  long long cells = 0, drops = 0, tiles = 0;
+  const int ty = kTileY[c->tune.tile], tx = kTileX[c->tune.tile];
+  for (Terrain& t : c->ter) {
+    t.cell_off = cells;
+    t.drop_off = (int)drops;
+    t.tiles_y = (t.ny + ty - 1) / ty;
+    t.tile_off = (int)tiles;
+    cells += (long long)t.nx * t.ny;
+    // whole waves per terrain: the walk reads its terrain entry once per wave
+    drops += ((long long)t.drops + kWaveDrops - 1) / kWaveDrops * kWaveDrops;
+    tiles += (long long)t.tiles_y * ((t.nx + tx - 1) / tx);
+    if (drops > INT32_MAX || tiles > INT32_MAX)
+      return fail(c, SHY_ERR_ARG, "ensemble too large: drops and cascade tiles must stay below 2^31");
+  }
+  c->total_drops = (int)drops;
+  c->total_tiles = (int)tiles;
+  drop_graph(c);
+  if (c->d_ter) {
+    HIPCHK(c, hipMemcpyAsync(c->d_ter, c->ter.data(), c->ter.size() * sizeof(Terrain),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return SHY_OK;
 }
 
-// Size the ticket log and buckets for `drops` (never inside a capture).
-int ensure_wave(shy_ctx* c, int drops) {
-  const size_t need = wave_steps(c, drops);
+size_t wave_steps(const shy_ctx* c) {
+  float age = 0.0f;
+  for (const Terrain& t : c->ter) age = std::max(age, t.P.maxAge);
+  return (size_t)c->total_drops * ((size_t)age + 1);
+}
+
+// Size the ticket log and buckets for the current drops (never inside a capture).
+int ensure_wave(shy_ctx* c) {
+  const size_t need = wave_steps(c);
   if (need > (size_t)INT32_MAX)
     return fail(c, SHY_ERR_ARG, "drops x (maxAge+1) exceeds 2^31 steps per wave");
   if (need <= c->wave_cap) return SHY_OK;
@@ -170,31 +195,62 @@ int ensure_wave(shy_ctx* c, int drops) {
   return SHY_OK;
 }
 
-// ev, when given, receives events after the count walk, the scan and the
-// commit walk.  The wave buffers must already hold wave_steps(c, drops).
-int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t* ev = nullptr) {
-  WalkArgs w = walk_args(c, drops);
+WaveArgs wave_args(shy_ctx* c, int p) {
+  WaveArgs w;
+  w.ter = c->d_ter;
+  w.nter = (int)c->ter.size();
+  w.total_drops = c->total_drops;
   w.h = c->h[p];
+  w.fld = c->fld;
+  w.root = c->root;
+  w.count = c->count;
+  w.ticket = c->ticket;
+  w.offset = c->offset;
+  w.scale = nullptr;
+  w.rec = c->rec;
+  w.deposit = c->deposit;
   w.visit = c->visit[p];
-  const int bs = c->walk_block;
-  const int grid = (drops + bs - 1) / bs;
+  w.state = c->state;
+  w.tsteps = c->tsteps;
+  return w;
+}
+
+int launch_cascade(shy_ctx* c, const CascArgs& a) {
+  switch (c->tune.tile) {
+    case 1: cascade_blend_kernel<64, 4><<<c->total_tiles, 256, 0, c->stream>>>(a); break;
+    case 2: cascade_blend_kernel<16, 16><<<c->total_tiles, 256, 0, c->stream>>>(a); break;
+    default: cascade_blend_kernel<32, 8><<<c->total_tiles, 256, 0, c->stream>>>(a); break;
+  }
+  HIPCHK(c, hipGetLastError());
+  return SHY_OK;
+}
+
+// One cycle of every terrain.  ev, when given, receives events after the
+// count walk, the scan and the commit walk.  The wave buffers must already
+// hold wave_steps(c).
+int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
+  WaveArgs w = wave_args(c, p);
+  const int bs = c->tune.walk_block;
+  const int need = (c->total_drops + bs - 1) / bs;
+  const int grid = c->tune.walk_grid > 0 ? std::min(c->tune.walk_grid, need) : need;
   const size_t n = c->n;
   const int nb = (int)((n + kScanBlock - 1) / kScanBlock);
-  if (drops > 0) {
+  const int nter = (int)c->ter.size();
+  const bool walk = c->total_drops > 0;
+  if (walk) {
     walk_kernel<kCount><<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-  if (drops > 0) {
+  if (walk) {
     scan_sums_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, n, c->bsum);
     scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->bsum, nb);
     scan_cells_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, c->bsum, n, c->offset, c->scale,
-                                                       c->fld, c->has_root ? c->root : nullptr,
-                                                       c->P);
+                                                       c->fld, c->root, c->d_ter, nter);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  if (drops > 0) {
+  if (walk) {
     w.scale = c->scale;
     walk_kernel<kCommit><<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
@@ -205,6 +261,8 @@ int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t* ev = nullptr) {
   HIPCHK(c, hipGetLastError());
   CascArgs a;
   const int q = p ^ 1;
+  a.ter = c->d_ter;
+  a.nter = nter;
   a.h_in = c->h[p];
   a.h_out = c->h[q];
   a.dh = c->dh;
@@ -216,19 +274,7 @@ int launch_cycle(shy_ctx* c, int drops, int p, hipEvent_t* ev = nullptr) {
   a.fld = c->fld;
   a.erf_a = c->erf_a;
   a.state = c->state;
-  a.nx = c->nx;
-  a.ny = c->ny;
-  a.P = c->P;
-  dim3 cgrid((c->ny + kCbY - 1) / kCbY, (c->nx + kCbX - 1) / kCbX);
-  cascade_blend_kernel<<<cgrid, kCbThreads, 0, c->stream>>>(a);
-  HIPCHK(c, hipGetLastError());
-  return SHY_OK;
-}
-
-void drop_graph(shy_ctx* c) {
-  if (c->graph) hipGraphExecDestroy(c->graph);
-  c->graph = nullptr;
-  c->graph_drops = -1;
+  return launch_cascade(c, a);
 }
 
 int refresh_erf(shy_ctx* c) {
@@ -243,6 +289,105 @@ bool all_finite(const float* v, size_t n) {
   for (size_t i = 0; i < n; ++i)
     if (!std::isfinite(v[i])) return false;
   return true;
+}
+
+int check_terrain(shy_ctx* c, int t) {
+  if (!c) return SHY_ERR_ARG;
+  if (t < 0 || t >= (int)c->ter.size()) return fail(c, SHY_ERR_ARG, "terrain index out of range");
+  return SHY_OK;
+}
+
+int need_single(shy_ctx* c, const char* what) {
+  if (!c) return SHY_ERR_ARG;
+  if (c->ter.size() != 1)
+    return fail(c, SHY_ERR_STATE, std::string(what) + ": only for single-terrain contexts");
+  return SHY_OK;
+}
+
+// One cycle eagerly (wave buffers sized first).
+int erode_once(shy_ctx* c, hipEvent_t* ev = nullptr) {
+  RETURN_IF(ensure_wave(c));
+  RETURN_IF(launch_cycle(c, parity(c), ev));
+  c->cycle += 1;
+  return SHY_OK;
+}
+
+int erode_many(shy_ctx* c, int cycles, bool use_graph) {
+  if (!use_graph) {
+    for (int k = 0; k < cycles; ++k) RETURN_IF(erode_once(c));
+    return SHY_OK;
+  }
+  if (cycles > 0 && parity(c) == 1) {
+    RETURN_IF(erode_once(c));
+    --cycles;
+  }
+  if (cycles >= 2) {
+    RETURN_IF(ensure_wave(c));
+    if (!c->graph) {
+      hipGraph_t g;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r0 = launch_cycle(c, 0);
+      int r1 = r0 ? r0 : launch_cycle(c, 1);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (r1) return r1;
+      if (e != hipSuccess)
+        return fail(c, SHY_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->graph = nullptr;
+        return fail(c, SHY_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    for (int k = 0; k + 1 < cycles; k += 2) {
+      HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+      c->cycle += 2;
+    }
+  }
+  if (cycles & 1) return erode_once(c);
+  return SHY_OK;
+}
+
+int time_cycles(shy_ctx* c, int cycles, float* ms) {
+  RETURN_IF(ensure_wave(c));
+  const size_t ne = 4 * (size_t)cycles + 1;
+  std::vector<hipEvent_t> ev(ne, nullptr);
+  auto cleanup = [&]() {
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
+  };
+  for (auto& e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      cleanup();
+      return fail(c, SHY_ERR_HIP, "shy_time_cycles: hipEventCreate failed");
+    }
+  }
+  hipError_t e = hipEventRecord(ev[0], c->stream);
+  for (int k = 0; k < cycles && e == hipSuccess; ++k) {
+    int r = launch_cycle(c, parity(c), &ev[4 * k + 1]);
+    if (r) {
+      cleanup();
+      return r;
+    }
+    c->cycle += 1;
+    e = hipEventRecord(ev[4 * k + 4], c->stream);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(ev.back());
+  float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < cycles && e == hipSuccess; ++k) {
+    for (int s = 0; s < 4 && e == hipSuccess; ++s) {
+      float t = 0.0f;
+      e = hipEventElapsedTime(&t, ev[4 * k + s], ev[4 * k + s + 1]);
+      part[s] += t;
+    }
+  }
+  float tot = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&tot, ev[0], ev.back());
+  cleanup();
+  if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("shy_time_cycles: ") + hipGetErrorString(e));
+  ms[0] = tot;
+  for (int s = 0; s < 4; ++s) ms[1 + s] = part[s];
+  return SHY_OK;
 }
 
 }  // namespace
@@ -280,8 +425,8 @@ int shy_destroy(shy_ctx* c) {
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
                   c->offset, c->scale, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
-                  c->state, c->erf_a, c->part, c->perms, c->trace_buf, c->trace_n,
-                  c->drop_buf};
+                  c->tsteps, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
+                  c->trace_n, c->drop_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -289,26 +434,36 @@ int shy_destroy(shy_ctx* c) {
   return SHY_OK;
 }
 
-int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out) {
-  if (!out) return fail(nullptr, SHY_ERR_ARG, "shy_create: out is NULL");
+int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
+  if (!out) return fail(nullptr, SHY_ERR_ARG, "shy_ensemble_create: out is NULL");
   *out = nullptr;
-  if (nx < 1 || ny < 1 || (long long)nx * ny > (1LL << 31) - 1)
-    return fail(nullptr, SHY_ERR_ARG, "shy_create: nx, ny must be >= 1 with nx*ny < 2^31");
+  if (!desc || nter < 1) return fail(nullptr, SHY_ERR_ARG, "shy_ensemble_create: need >= 1 terrain");
+  long long cells = 0;
+  for (int t = 0; t < nter; ++t) {
+    const shy_terrain_desc& d = desc[t];
+    if (d.nx < 1 || d.ny < 1 || (long long)d.nx * d.ny > (1LL << 31) - 1 || d.drops < 0)
+      return fail(nullptr, SHY_ERR_ARG,
+                  "shy_create: nx, ny must be >= 1 with nx*ny < 2^31, drops >= 0");
+    if (!params_ok(&d.params)) return fail(nullptr, SHY_ERR_ARG, std::string("shy_create: ") + kParamsMsg);
+    cells += (long long)d.nx * d.ny;
+  }
+  if (cells > (1LL << 31) - 1)
+    return fail(nullptr, SHY_ERR_ARG, "shy_ensemble_create: total cells must stay below 2^31");
   if (shy_device_count() < 1) return fail(nullptr, SHY_ERR_NODEV, "shy_create: no GPU visible");
   shy_ctx* c = new shy_ctx();
-  if (p) {
-    if (!params_ok(p)) {
-      delete c;
-      return fail(nullptr, SHY_ERR_ARG, "shy_create: parameter not finite, evapRate >= 1 or maxAge outside [0, 1e6]");
-    }
-    c->P = *p;
-  } else {
-    shy_default_params(&c->P);
+  c->ter.resize(nter);
+  for (int t = 0; t < nter; ++t) {
+    Terrain& T = c->ter[t];
+    std::memset(&T, 0, sizeof(T));
+    T.nx = desc[t].nx;
+    T.ny = desc[t].ny;
+    T.drops = desc[t].drops;
+    T.seed = desc[t].seed;
+    T.id = (unsigned)t;
+    T.has_root = 0;
+    T.P = desc[t].params;
   }
-  c->nx = nx;
-  c->ny = ny;
-  c->n = (size_t)nx * ny;
-  c->seed = seed;
+  c->n = (size_t)cells;
   const size_t n = c->n;
   auto bail = [&](hipError_t e, const char* what) {
     g_noctx_err = std::string("shy_create: ") + what + ": " + hipGetErrorString(e);
@@ -319,12 +474,18 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
   if ((e = hipGetDevice(&c->device)) != hipSuccess) return bail(e, "hipGetDevice");
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(e, "hipStreamCreate");
+  if (layout(c)) {
+    g_noctx_err = c->err;
+    shy_destroy(c);
+    return SHY_ERR_ARG;
+  }
   const size_t nb = (n + kScanBlock - 1) / kScanBlock;
   for (int k = 0; k < 2; ++k) {
     if ((e = hipMalloc(&c->h[k], n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc h");
     if ((e = hipMalloc(&c->visit[k], n)) != hipSuccess) return bail(e, "hipMalloc visit");
   }
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
+  if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->count, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc count");
   if ((e = hipMalloc(&c->offset, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc offset");
   if ((e = hipMalloc(&c->scale, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc scale");
@@ -332,12 +493,11 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");
   if ((e = hipMalloc(&c->bsum, nb * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc bsum");
-  if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
+  if ((e = hipMalloc(&c->tsteps, nter * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc tsteps");
+  if ((e = hipMalloc(&c->d_ter, nter * sizeof(Terrain))) != hipSuccess) return bail(e, "hipMalloc terrains");
   if ((e = hipMalloc(&c->state, sizeof(DeviceState))) != hipSuccess) return bail(e, "hipMalloc state");
-  if ((e = hipMalloc(&c->erf_a, kErfTerms * sizeof(double))) != hipSuccess)
-    return bail(e, "hipMalloc erf");
-  if ((e = hipMalloc(&c->part, kPartBlocks * sizeof(float2))) != hipSuccess)
-    return bail(e, "hipMalloc part");
+  if ((e = hipMalloc(&c->erf_a, kErfTerms * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc erf");
+  if ((e = hipMalloc(&c->part, kPartBlocks * sizeof(float2))) != hipSuccess) return bail(e, "hipMalloc part");
   if ((e = hipMalloc(&c->perms, 16 * 256)) != hipSuccess) return bail(e, "hipMalloc perms");
   if ((e = hipMalloc(&c->trace_n, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc trace_n");
   for (int k = 0; k < 2; ++k) {
@@ -351,50 +511,92 @@ int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out
   hipMemsetAsync(c->dh, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
   hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->tsteps, 0, nter * sizeof(long long), c->stream);
   DeviceState st{};
   st.cycle = 0;
   double coef[kErfTerms];
   erf_coefficients(coef);
   hipMemcpyAsync(c->state, &st, sizeof(st), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(c->erf_a, coef, sizeof(coef), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(c->d_ter, c->ter.data(), nter * sizeof(Terrain), hipMemcpyHostToDevice, c->stream);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "init");
   *out = c;
   return SHY_OK;
 }
 
-int shy_set_params(shy_ctx* c, const shy_params* p) {
-  if (!c || !p) return fail(c, SHY_ERR_ARG, "shy_set_params: NULL argument");
-  if (!params_ok(p)) return fail(c, SHY_ERR_ARG, "shy_set_params: parameter not finite, evapRate >= 1 or maxAge outside [0, 1e6]");
-  c->P = *p;
-  drop_graph(c);
+int shy_create(const shy_params* p, int nx, int ny, uint64_t seed, shy_ctx** out) {
+  shy_terrain_desc d;
+  d.nx = nx;
+  d.ny = ny;
+  d.drops = 0;
+  d.seed = seed;
+  if (p) d.params = *p; else shy_default_params(&d.params);
+  return shy_ensemble_create(&d, 1, out);
+}
+
+int shy_ensemble_count(const shy_ctx* c, int* nter) {
+  if (!c || !nter) return SHY_ERR_ARG;
+  *nter = (int)c->ter.size();
   return SHY_OK;
 }
 
-int shy_get_params(const shy_ctx* c, shy_params* p) {
-  if (!c || !p) return SHY_ERR_ARG;
-  *p = c->P;
+int shy_terrain_size(const shy_ctx* c, int t, int* nx, int* ny, int* drops) {
+  if (!c || !nx || !ny || !drops || t < 0 || t >= (int)c->ter.size()) return SHY_ERR_ARG;
+  *nx = c->ter[t].nx;
+  *ny = c->ter[t].ny;
+  *drops = c->ter[t].drops;
   return SHY_OK;
 }
 
 int shy_size(const shy_ctx* c, int* nx, int* ny) {
   if (!c || !nx || !ny) return SHY_ERR_ARG;
-  *nx = c->nx;
-  *ny = c->ny;
+  *nx = c->ter[0].nx;
+  *ny = c->ter[0].ny;
   return SHY_OK;
 }
 
-int shy_set_seed(shy_ctx* c, uint64_t seed, uint32_t cycle) {
+int shy_terrain_set_params(shy_ctx* c, int t, const shy_params* p) {
+  RETURN_IF(check_terrain(c, t));
+  if (!p) return fail(c, SHY_ERR_ARG, "shy_set_params: NULL argument");
+  if (!params_ok(p)) return fail(c, SHY_ERR_ARG, std::string("shy_set_params: ") + kParamsMsg);
+  hipSetDevice(c->device);
+  c->ter[t].P = *p;
+  return layout(c);
+}
+
+int shy_set_params(shy_ctx* c, const shy_params* p) {
+  if (!c) return SHY_ERR_ARG;
+  if (!p) return fail(c, SHY_ERR_ARG, "shy_set_params: NULL argument");
+  if (!params_ok(p)) return fail(c, SHY_ERR_ARG, std::string("shy_set_params: ") + kParamsMsg);
+  for (Terrain& t : c->ter) t.P = *p;
+  hipSetDevice(c->device);
+  return layout(c);
+}
+
+int shy_get_params(const shy_ctx* c, shy_params* p) {
+  if (!c || !p) return SHY_ERR_ARG;
+  *p = c->ter[0].P;
+  return SHY_OK;
+}
+
+int shy_terrain_set_drops(shy_ctx* c, int t, int drops) {
+  RETURN_IF(check_terrain(c, t));
+  if (drops < 0) return fail(c, SHY_ERR_ARG, "drops must be >= 0");
+  if (c->ter[t].drops == drops) return SHY_OK;
+  hipSetDevice(c->device);
+  c->ter[t].drops = drops;
+  return layout(c);
+}
+
+int shy_set_cycle(shy_ctx* c, uint32_t cycle) {
   if (!c) return SHY_ERR_ARG;
   hipSetDevice(c->device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->seed = seed;
-  // parity follows the cycle number; move the live buffers if it flips
   if ((cycle & 1u) != (c->cycle & 1u)) {
     // parity p: h[p] live, visit[p] the zero accumulator.  Move to q = p^1.
     const int p = parity(c), q = p ^ 1;
-    const size_t n = c->n;
-    HIPCHK(c, hipMemcpyAsync(c->h[q], c->h[p], n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->visit[q], 0, n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h[q], c->h[p], c->n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->visit[q], 0, c->n, c->stream));
   }
   c->cycle = cycle;
   HIPCHK(c, hipMemcpyAsync(&c->state->cycle, &cycle, sizeof(cycle), hipMemcpyHostToDevice, c->stream));
@@ -403,40 +605,46 @@ int shy_set_seed(shy_ctx* c, uint64_t seed, uint32_t cycle) {
   return SHY_OK;
 }
 
+int shy_set_seed(shy_ctx* c, uint64_t seed, uint32_t cycle) {
+  RETURN_IF(need_single(c, "shy_set_seed"));
+  c->ter[0].seed = seed;
+  RETURN_IF(layout(c));
+  return shy_set_cycle(c, cycle);
+}
+
 int shy_get_cycle(const shy_ctx* c, uint32_t* cycle) {
   if (!c || !cycle) return SHY_ERR_ARG;
   *cycle = c->cycle;
   return SHY_OK;
 }
 
-int shy_upload_field(shy_ctx* c, int field, const float* src) {
-  if (!c || !src) return fail(c, SHY_ERR_ARG, "shy_upload_field: NULL argument");
+int shy_terrain_upload_field(shy_ctx* c, int t, int field, const float* src) {
+  RETURN_IF(check_terrain(c, t));
+  if (!src) return fail(c, SHY_ERR_ARG, "shy_upload_field: NULL argument");
   if (field < 0 || field > SHY_ROOTDENSITY)
     return fail(c, SHY_ERR_ARG, "shy_upload_field: field must be a quad::cell member (0..7)");
-  if (!all_finite(src, c->n)) return fail(c, SHY_ERR_NONFINITE, "shy_upload_field: NaN or Inf in input");
+  Terrain& T = c->ter[t];
+  const size_t n = (size_t)T.nx * T.ny, o = (size_t)T.cell_off;
+  if (!all_finite(src, n)) return fail(c, SHY_ERR_NONFINITE, "shy_upload_field: NaN or Inf in input");
   hipSetDevice(c->device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int p = parity(c);
-  const size_t n = c->n;
   switch (field) {
     case SHY_HEIGHT:
-      HIPCHK(c, hipMemcpy(c->h[p], src, n * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->h[p] + o, src, n * sizeof(float), hipMemcpyHostToDevice));
       break;
     case SHY_DISCHARGE:
     case SHY_MOMENTUMX:
     case SHY_MOMENTUMY: {
       std::vector<float4> f(n);
-      HIPCHK(c, hipMemcpy(f.data(), c->fld, n * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(f.data(), c->fld + o, n * sizeof(float4), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i) {
         if (field == SHY_DISCHARGE) f[i].z = src[i];
         else if (field == SHY_MOMENTUMX) f[i].x = src[i];
         else f[i].y = src[i];
       }
-      HIPCHK(c, hipMemcpy(c->fld, f.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-      if (field == SHY_DISCHARGE) {
-        int r = refresh_erf(c);
-        if (r) return r;
-      }
+      HIPCHK(c, hipMemcpy(c->fld + o, f.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+      if (field == SHY_DISCHARGE) RETURN_IF(refresh_erf(c));
       break;
     }
     case SHY_DISCHARGE_TRACK:
@@ -446,16 +654,18 @@ int shy_upload_field(shy_ctx* c, int field, const float* src) {
       // the next cycle overwrites them, as the reference's do
       std::vector<long long> q(n);
       for (size_t i = 0; i < n; ++i) q[i] = (long long)std::llrint((double)src[i] * 16777216.0);
-      HIPCHK(c, hipMemcpy(c->trk + (size_t)(field - SHY_DISCHARGE_TRACK) * n, q.data(),
+      HIPCHK(c, hipMemcpy(c->trk + (size_t)(field - SHY_DISCHARGE_TRACK) * c->n + o, q.data(),
                           n * sizeof(long long), hipMemcpyHostToDevice));
       break;
     }
     case SHY_ROOTDENSITY: {
       bool any = false;
       for (size_t i = 0; i < n && !any; ++i) any = src[i] != 0.0f;
-      HIPCHK(c, hipMemcpy(c->root, src, n * sizeof(float), hipMemcpyHostToDevice));
-      if (any != c->has_root) drop_graph(c);
-      c->has_root = any;
+      HIPCHK(c, hipMemcpy(c->root + o, src, n * sizeof(float), hipMemcpyHostToDevice));
+      if ((int)any != T.has_root) {
+        T.has_root = any ? 1 : 0;
+        RETURN_IF(layout(c));
+      }
       break;
     }
   }
@@ -463,24 +673,26 @@ int shy_upload_field(shy_ctx* c, int field, const float* src) {
   return SHY_OK;
 }
 
-int shy_download_field(shy_ctx* c, int field, float* dst) {
-  if (!c || !dst) return fail(c, SHY_ERR_ARG, "shy_download_field: NULL argument");
+int shy_terrain_download_field(shy_ctx* c, int t, int field, float* dst) {
+  RETURN_IF(check_terrain(c, t));
+  if (!dst) return fail(c, SHY_ERR_ARG, "shy_download_field: NULL argument");
   if (field < 0 || field > SHY_ERF_DISCHARGE)
     return fail(c, SHY_ERR_ARG, "shy_download_field: unknown field");
+  const Terrain& T = c->ter[t];
+  const size_t n = (size_t)T.nx * T.ny, o = (size_t)T.cell_off;
   hipSetDevice(c->device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int p = parity(c);
-  const size_t n = c->n;
   switch (field) {
     case SHY_HEIGHT:
-      HIPCHK(c, hipMemcpy(dst, c->h[p], n * sizeof(float), hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(dst, c->h[p] + o, n * sizeof(float), hipMemcpyDeviceToHost));
       break;
     case SHY_DISCHARGE:
     case SHY_MOMENTUMX:
     case SHY_MOMENTUMY:
     case SHY_ERF_DISCHARGE: {
       std::vector<float4> f(n);
-      HIPCHK(c, hipMemcpy(f.data(), c->fld, n * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(f.data(), c->fld + o, n * sizeof(float4), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i)
         dst[i] = field == SHY_DISCHARGE ? f[i].z
                  : field == SHY_MOMENTUMX ? f[i].x
@@ -492,25 +704,37 @@ int shy_download_field(shy_ctx* c, int field, float* dst) {
     case SHY_MOMENTUMX_TRACK:
     case SHY_MOMENTUMY_TRACK: {
       std::vector<long long> q(n);
-      HIPCHK(c, hipMemcpy(q.data(), c->trk + (size_t)(field - SHY_DISCHARGE_TRACK) * n,
+      HIPCHK(c, hipMemcpy(q.data(), c->trk + (size_t)(field - SHY_DISCHARGE_TRACK) * c->n + o,
                           n * sizeof(long long), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i) dst[i] = (float)((double)q[i] * kTQInv);
       break;
     }
     case SHY_ROOTDENSITY:
-      HIPCHK(c, hipMemcpy(dst, c->root, n * sizeof(float), hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(dst, c->root + o, n * sizeof(float), hipMemcpyDeviceToHost));
       break;
   }
   return SHY_OK;
 }
 
+int shy_upload_field(shy_ctx* c, int field, const float* src) {
+  RETURN_IF(need_single(c, "shy_upload_field"));
+  return shy_terrain_upload_field(c, 0, field, src);
+}
+
+int shy_download_field(shy_ctx* c, int field, float* dst) {
+  RETURN_IF(need_single(c, "shy_download_field"));
+  return shy_terrain_download_field(c, 0, field, dst);
+}
+
 int shy_download_normals(shy_ctx* c, float* xyz) {
-  if (!c || !xyz) return fail(c, SHY_ERR_ARG, "shy_download_normals: NULL argument");
+  RETURN_IF(need_single(c, "shy_download_normals"));
+  if (!xyz) return fail(c, SHY_ERR_ARG, "shy_download_normals: NULL argument");
   hipSetDevice(c->device);
+  const Terrain& T = c->ter[0];
   float* d = nullptr;
   HIPCHK(c, hipMalloc((void**)&d, 3 * c->n * sizeof(float)));
   const size_t grid = (c->n + 255) / 256;
-  normals_kernel<<<grid, 256, 0, c->stream>>>(c->h[parity(c)], c->nx, c->ny, d);
+  normals_kernel<<<grid, 256, 0, c->stream>>>(c->h[parity(c)], T.nx, T.ny, d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
     e = hipMemcpyAsync(xyz, d, 3 * c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
@@ -521,120 +745,96 @@ int shy_download_normals(shy_ctx* c, float* xyz) {
 }
 
 int shy_upload_cells(shy_ctx* c, const float* cells8) {
-  if (!c || !cells8) return fail(c, SHY_ERR_ARG, "shy_upload_cells: NULL argument");
+  RETURN_IF(need_single(c, "shy_upload_cells"));
+  if (!cells8) return fail(c, SHY_ERR_ARG, "shy_upload_cells: NULL argument");
   std::vector<float> f(c->n);
   for (int field = 0; field < 8; ++field) {
     for (size_t i = 0; i < c->n; ++i) f[i] = cells8[i * 8 + field];
-    int r = shy_upload_field(c, field, f.data());
-    if (r) return r;
+    RETURN_IF(shy_upload_field(c, field, f.data()));
   }
   return SHY_OK;
 }
 
 int shy_download_cells(shy_ctx* c, float* cells8) {
-  if (!c || !cells8) return fail(c, SHY_ERR_ARG, "shy_download_cells: NULL argument");
+  RETURN_IF(need_single(c, "shy_download_cells"));
+  if (!cells8) return fail(c, SHY_ERR_ARG, "shy_download_cells: NULL argument");
   std::vector<float> f(c->n);
   for (int field = 0; field < 8; ++field) {
-    int r = shy_download_field(c, field, f.data());
-    if (r) return r;
+    RETURN_IF(shy_download_field(c, field, f.data()));
     for (size_t i = 0; i < c->n; ++i) cells8[i * 8 + field] = f[i];
   }
   return SHY_OK;
 }
 
-int shy_generate_fbm(shy_ctx* c, uint64_t seed, int octaves, float base_freq, float lacunarity,
-                     float gain) {
-  if (!c) return SHY_ERR_ARG;
+int shy_terrain_generate_fbm(shy_ctx* c, int t, uint64_t seed, int octaves, float base_freq,
+                             float lacunarity, float gain) {
+  RETURN_IF(check_terrain(c, t));
   if (octaves < 1 || octaves > 16) return fail(c, SHY_ERR_ARG, "shy_generate_fbm: octaves must be 1..16");
   if (!std::isfinite(base_freq) || !std::isfinite(lacunarity) || !std::isfinite(gain))
     return fail(c, SHY_ERR_ARG, "shy_generate_fbm: non-finite argument");
-  if (base_freq <= 0.0f) base_freq = 1.0f / (float)(c->nx > c->ny ? c->nx : c->ny);
+  const Terrain& T = c->ter[t];
+  if (base_freq <= 0.0f) base_freq = 1.0f / (float)(T.nx > T.ny ? T.nx : T.ny);
   hipSetDevice(c->device);
   std::vector<uint8_t> perms(16 * 256);
   for (int o = 0; o < octaves; ++o) fbm_perm(seed, o, perms.data() + o * 256);
   HIPCHK(c, hipMemcpyAsync(c->perms, perms.data(), octaves * 256, hipMemcpyHostToDevice, c->stream));
-  float* out = c->h[parity(c)];
+  const size_t n = (size_t)T.nx * T.ny;
+  float* out = c->h[parity(c)] + T.cell_off;
   const int bs = 256;
-  const size_t grid = (c->n + bs - 1) / bs;
-  fbm_kernel<<<grid, bs, 0, c->stream>>>(out, c->nx, c->ny, c->perms, octaves, base_freq,
-                                         lacunarity, gain);
+  const size_t grid = (n + bs - 1) / bs;
+  fbm_kernel<<<grid, bs, 0, c->stream>>>(out, T.nx, T.ny, c->perms, octaves, base_freq, lacunarity,
+                                         gain);
   HIPCHK(c, hipGetLastError());
-  minmax_partial_kernel<<<kPartBlocks, 256, 0, c->stream>>>(out, c->n, c->part);
+  minmax_partial_kernel<<<kPartBlocks, 256, 0, c->stream>>>(out, n, c->part);
   HIPCHK(c, hipGetLastError());
   minmax_fold_kernel<<<1, 256, 0, c->stream>>>(c->part, kPartBlocks);
   HIPCHK(c, hipGetLastError());
-  normalize_kernel<<<grid, bs, 0, c->stream>>>(out, c->n, c->part);
+  normalize_kernel<<<grid, bs, 0, c->stream>>>(out, n, c->part);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SHY_OK;
 }
 
+int shy_generate_fbm(shy_ctx* c, uint64_t seed, int octaves, float base_freq, float lacunarity,
+                     float gain) {
+  RETURN_IF(need_single(c, "shy_generate_fbm"));
+  return shy_terrain_generate_fbm(c, 0, seed, octaves, base_freq, lacunarity, gain);
+}
+
 int shy_erode(shy_ctx* c, int drops) {
-  if (!c) return SHY_ERR_ARG;
+  RETURN_IF(need_single(c, "shy_erode"));
   if (drops < 0) return fail(c, SHY_ERR_ARG, "shy_erode: drops must be >= 0");
   hipSetDevice(c->device);
-  int r = ensure_wave(c, drops);
-  if (r) return r;
-  r = launch_cycle(c, drops, parity(c));
-  if (r) return r;
-  c->cycle += 1;
-  return SHY_OK;
+  RETURN_IF(shy_terrain_set_drops(c, 0, drops));
+  return erode_once(c);
 }
 
 int shy_erode_cycles(shy_ctx* c, int cycles, int drops, int use_graph) {
-  if (!c) return SHY_ERR_ARG;
+  RETURN_IF(need_single(c, "shy_erode_cycles"));
   if (cycles < 0 || drops < 0) return fail(c, SHY_ERR_ARG, "shy_erode_cycles: negative count");
   hipSetDevice(c->device);
-  if (!use_graph) {
-    for (int k = 0; k < cycles; ++k) {
-      int r = shy_erode(c, drops);
-      if (r) return r;
-    }
-    return SHY_OK;
-  }
-  if (cycles > 0 && parity(c) == 1) {
-    int r = shy_erode(c, drops);
-    if (r) return r;
-    --cycles;
-  }
-  if (cycles >= 2) {
-    int rw = ensure_wave(c, drops);
-    if (rw) return rw;
-    if (!c->graph || c->graph_drops != drops) {
-      drop_graph(c);
-      hipGraph_t g;
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int r0 = launch_cycle(c, drops, 0);
-      int r1 = r0 ? r0 : launch_cycle(c, drops, 1);
-      hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (r1) return r1;
-      if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      if (e != hipSuccess) {
-        c->graph = nullptr;
-        return fail(c, SHY_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-      }
-      c->graph_drops = drops;
-    }
-    for (int k = 0; k + 1 < cycles; k += 2) {
-      HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-      c->cycle += 2;
-    }
-  }
-  if (cycles & 1) return shy_erode(c, drops);
-  return SHY_OK;
+  RETURN_IF(shy_terrain_set_drops(c, 0, drops));
+  return erode_many(c, cycles, use_graph != 0);
+}
+
+int shy_ensemble_erode(shy_ctx* c, int cycles, int use_graph) {
+  if (!c) return SHY_ERR_ARG;
+  if (cycles < 0) return fail(c, SHY_ERR_ARG, "shy_ensemble_erode: negative count");
+  hipSetDevice(c->device);
+  return erode_many(c, cycles, use_graph != 0);
 }
 
 int shy_trace_drop(shy_ctx* c, float x, float y, int max_steps, shy_drop_step* out, int* n_steps) {
-  if (!c || !out || !n_steps) return fail(c, SHY_ERR_ARG, "shy_trace_drop: NULL argument");
+  RETURN_IF(need_single(c, "shy_trace_drop"));
+  if (!out || !n_steps) return fail(c, SHY_ERR_ARG, "shy_trace_drop: NULL argument");
   if (max_steps < 1) return fail(c, SHY_ERR_ARG, "shy_trace_drop: max_steps must be >= 1");
+  const Terrain& T = c->ter[0];
   if (!std::isfinite(x) || !std::isfinite(y) || std::fabs(x) > 1e9f || std::fabs(y) > 1e9f) {
     *n_steps = 0;
     return SHY_OK;
   }
   const int ix = (int)x, iy = (int)y;
-  if (ix < 0 || iy < 0 || ix >= c->nx || iy >= c->ny) {
+  if (ix < 0 || iy < 0 || ix >= T.nx || iy >= T.ny) {
     *n_steps = 0;
     return SHY_OK;  // Drop::descend returns false on a NULL node (water.h:62-64)
   }
@@ -646,7 +846,17 @@ int shy_trace_drop(shy_ctx* c, float x, float y, int max_steps, shy_drop_step* o
     HIPCHK(c, hipMalloc(&c->trace_buf, (size_t)max_steps * sizeof(shy_drop_step)));
     c->trace_cap = max_steps;
   }
-  WalkArgs a = walk_args(c, 1);
+  WalkArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.h = c->h[parity(c)];
+  a.fld = c->fld;
+  a.root = T.has_root ? c->root : nullptr;
+  a.state = c->state;
+  a.seed = T.seed;
+  a.nx = T.nx;
+  a.ny = T.ny;
+  a.drops = 1;
+  a.P = T.P;
   trace_kernel<<<1, 64, 0, c->stream>>>(a, x, y, c->trace_buf, max_steps, c->trace_n);
   HIPCHK(c, hipGetLastError());
   int n = 0;
@@ -659,18 +869,22 @@ int shy_trace_drop(shy_ctx* c, float x, float y, int max_steps, shy_drop_step* o
 }
 
 int shy_cascade(shy_ctx* c, float x, float y) {
-  if (!c) return SHY_ERR_ARG;
+  RETURN_IF(need_single(c, "shy_cascade"));
+  const Terrain& T = c->ter[0];
+  if (!std::isfinite(x) || !std::isfinite(y) || std::fabs(x) > 1e9f || std::fabs(y) > 1e9f)
+    return fail(c, SHY_ERR_ARG, "shy_cascade: position off the map");
   const int ix = (int)x, iy = (int)y;
-  if (!std::isfinite(x) || !std::isfinite(y) || ix < 0 || iy < 0 || ix >= c->nx || iy >= c->ny)
+  if (ix < 0 || iy < 0 || ix >= T.nx || iy >= T.ny)
     return fail(c, SHY_ERR_ARG, "shy_cascade: position off the map");
   hipSetDevice(c->device);
-  cascade_live_kernel<<<1, 64, 0, c->stream>>>(c->h[parity(c)], c->nx, c->ny, c->P, x, y);
+  cascade_live_kernel<<<1, 64, 0, c->stream>>>(c->h[parity(c)], T.nx, T.ny, T.P, x, y);
   HIPCHK(c, hipGetLastError());
   return SHY_OK;
 }
 
 int shy_descend(shy_ctx* c, shy_drop* drop, int* alive) {
-  if (!c || !drop || !alive) return fail(c, SHY_ERR_ARG, "shy_descend: NULL argument");
+  RETURN_IF(need_single(c, "shy_descend"));
+  if (!drop || !alive) return fail(c, SHY_ERR_ARG, "shy_descend: NULL argument");
   if (!std::isfinite(drop->pos[0]) || !std::isfinite(drop->pos[1]) ||
       std::fabs(drop->pos[0]) > 1e9f || std::fabs(drop->pos[1]) > 1e9f) {
     *alive = 0;
@@ -678,18 +892,18 @@ int shy_descend(shy_ctx* c, shy_drop* drop, int* alive) {
   }
   hipSetDevice(c->device);
   if (!c->drop_buf) HIPCHK(c, hipMalloc(&c->drop_buf, sizeof(shy_drop) + sizeof(int)));
+  const Terrain& T = c->ter[0];
   LiveArgs a;
-  const int p = parity(c);
-  a.h = c->h[p];
+  a.h = c->h[parity(c)];
   a.fld = c->fld;
   a.root = c->root;
   a.td = c->trk;
   a.tmx = c->trk + c->n;
   a.tmy = c->trk + 2 * c->n;
   a.erf_a = c->erf_a;
-  a.nx = c->nx;
-  a.ny = c->ny;
-  a.P = c->P;
+  a.nx = T.nx;
+  a.ny = T.ny;
+  a.P = T.P;
   shy_drop* dd = (shy_drop*)c->drop_buf;
   int* da = (int*)((char*)c->drop_buf + sizeof(shy_drop));
   HIPCHK(c, hipMemcpyAsync(dd, drop, sizeof(shy_drop), hipMemcpyHostToDevice, c->stream));
@@ -720,11 +934,21 @@ int shy_get_stats(shy_ctx* c, shy_stats* out) {
   return SHY_OK;
 }
 
+int shy_terrain_steps(shy_ctx* c, int64_t* steps) {
+  if (!c || !steps) return SHY_ERR_ARG;
+  hipSetDevice(c->device);
+  HIPCHK(c, hipMemcpyAsync(steps, c->tsteps, c->ter.size() * sizeof(long long),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SHY_OK;
+}
+
 int shy_reset_stats(shy_ctx* c) {
   if (!c) return SHY_ERR_ARG;
   hipSetDevice(c->device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemsetAsync(c->state->stats, 0, sizeof(c->state->stats), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->tsteps, 0, c->ter.size() * sizeof(long long), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SHY_OK;
 }
@@ -736,60 +960,102 @@ int shy_synchronize(shy_ctx* c) {
   return SHY_OK;
 }
 
-int shy_time_cycles(shy_ctx* c, int cycles, int drops, float* ms) {
-  if (!c || !ms || cycles < 1 || drops < 0)
-    return fail(c, SHY_ERR_ARG, "shy_time_cycles: bad argument");
+int shy_set_tuning(shy_ctx* c, const shy_tuning* t) {
+  if (!c || !t) return SHY_ERR_ARG;
+  shy_tuning v = *t;
+  if (v.walk_block == 0) v.walk_block = 256;
+  if (v.walk_block < 64 || v.walk_block > 1024 || v.walk_block % 64)
+    return fail(c, SHY_ERR_ARG, "walk_block must be a multiple of 64 in 64..1024");
+  if (v.walk_grid < 0) return fail(c, SHY_ERR_ARG, "walk_grid must be >= 0");
+  if (v.tile < 0 || v.tile > 2) return fail(c, SHY_ERR_ARG, "tile must be 0 (32x8), 1 (64x4) or 2 (16x16)");
   hipSetDevice(c->device);
-  int rw = ensure_wave(c, drops);
-  if (rw) return rw;
-  // per cycle: start, after count walk, after scan, after commit walk; + end
-  const size_t ne = 4 * (size_t)cycles + 1;
-  std::vector<hipEvent_t> ev(ne, nullptr);
-  auto cleanup = [&]() {
-    for (hipEvent_t e : ev)
-      if (e) hipEventDestroy(e);
-  };
-  for (auto& e : ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      cleanup();
-      return fail(c, SHY_ERR_HIP, "shy_time_cycles: hipEventCreate failed");
-    }
-  }
-  hipError_t e = hipEventRecord(ev[0], c->stream);
-  for (int k = 0; k < cycles && e == hipSuccess; ++k) {
-    int r = launch_cycle(c, drops, parity(c), &ev[4 * k + 1]);
-    if (r) {
-      cleanup();
-      return r;
-    }
-    c->cycle += 1;
-    e = hipEventRecord(ev[4 * k + 4], c->stream);
-  }
-  if (e == hipSuccess) e = hipEventSynchronize(ev.back());
-  float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k = 0; k < cycles && e == hipSuccess; ++k) {
-    for (int s = 0; s < 4 && e == hipSuccess; ++s) {
-      float t = 0.0f;
-      e = hipEventElapsedTime(&t, ev[4 * k + s], ev[4 * k + s + 1]);
-      part[s] += t;
-    }
-  }
-  float tot = 0.0f;
-  if (e == hipSuccess) e = hipEventElapsedTime(&tot, ev[0], ev.back());
-  cleanup();
-  if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("shy_time_cycles: ") + hipGetErrorString(e));
-  ms[0] = tot;
-  for (int s = 0; s < 4; ++s) ms[1 + s] = part[s];
+  c->tune = v;
+  return layout(c);
+}
+
+int shy_get_tuning(const shy_ctx* c, shy_tuning* t) {
+  if (!c || !t) return SHY_ERR_ARG;
+  *t = c->tune;
   return SHY_OK;
 }
 
 int shy_set_walk_block(shy_ctx* c, int threads) {
   if (!c) return SHY_ERR_ARG;
-  if (threads == 0) threads = 256;
-  if (threads < 64 || threads > 1024 || threads % 64)
-    return fail(c, SHY_ERR_ARG, "shy_set_walk_block: threads must be a multiple of 64 in 64..1024");
-  c->walk_block = threads;
-  drop_graph(c);
+  shy_tuning t = c->tune;
+  t.walk_block = threads;
+  return shy_set_tuning(c, &t);
+}
+
+int shy_time_cycles(shy_ctx* c, int cycles, int drops, float ms[5]) {
+  if (!c || !ms || cycles < 1) return fail(c, SHY_ERR_ARG, "shy_time_cycles: bad argument");
+  hipSetDevice(c->device);
+  if (drops >= 0) {
+    RETURN_IF(need_single(c, "shy_time_cycles with drops >= 0"));
+    RETURN_IF(shy_terrain_set_drops(c, 0, drops));
+  }
+  return time_cycles(c, cycles, ms);
+}
+
+int shy_autotune(shy_ctx* c, int cycles, shy_tuning* chosen, float* best_ms) {
+  if (!c || cycles < 1) return fail(c, SHY_ERR_ARG, "shy_autotune: bad argument");
+  hipSetDevice(c->device);
+  RETURN_IF(ensure_wave(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // Candidates are timed from a copy of the live state, which is restored
+  // after each one, so tuning never changes the trajectory.
+  const size_t n = c->n;
+  const int p0 = parity(c);
+  const uint32_t cyc0 = c->cycle;
+  std::vector<float> h0(n);
+  std::vector<float4> f0(n);
+  std::vector<uint8_t> v0(n);
+  std::vector<long long> t0(3 * n), ts0(c->ter.size());
+  DeviceState s0;
+  HIPCHK(c, hipMemcpy(h0.data(), c->h[p0], n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(f0.data(), c->fld, n * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(v0.data(), c->visit[p0], n, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(t0.data(), c->trk, 3 * n * sizeof(long long), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ts0.data(), c->tsteps, ts0.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&s0, c->state, sizeof(s0), hipMemcpyDeviceToHost));
+  auto restore = [&]() -> int {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cycle = cyc0;
+    HIPCHK(c, hipMemcpy(c->h[p0], h0.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->fld, f0.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->visit[p0], v0.data(), n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->visit[p0 ^ 1], 0, n));
+    HIPCHK(c, hipMemcpy(c->trk, t0.data(), 3 * n * sizeof(long long), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->tsteps, ts0.data(), ts0.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->state, &s0, sizeof(s0), hipMemcpyHostToDevice));
+    return SHY_OK;
+  };
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+    cus = 256;
+  std::vector<shy_tuning> cand;
+  for (int tile = 0; tile < 3; ++tile)
+    for (int bs : {64, 128, 256, 512})
+      for (int per_cu : {0, 8})  // 0: one lane per drop; 8: persistent, 8 blocks' worth per CU
+        cand.push_back(shy_tuning{bs, per_cu ? per_cu * cus * 256 / bs : 0, tile});
+  float best = 1e30f;
+  shy_tuning bt = c->tune;
+  for (const shy_tuning& t : cand) {
+    c->tune = t;
+    RETURN_IF(layout(c));
+    float ms[5];
+    RETURN_IF(time_cycles(c, 1, ms));  // warm-up
+    RETURN_IF(restore());
+    RETURN_IF(time_cycles(c, cycles, ms));
+    RETURN_IF(restore());
+    if (ms[0] < best) {
+      best = ms[0];
+      bt = t;
+    }
+  }
+  c->tune = bt;
+  RETURN_IF(layout(c));
+  if (chosen) *chosen = bt;
+  if (best_ms) *best_ms = best / (float)cycles;
   return SHY_OK;
 }
 

@@ -48,6 +48,51 @@ struct DeviceState {
   unsigned int pad;
 };
 
+// One terrain of an ensemble: its slice of the concatenated cell arrays,
+// its drops in the concatenated wave, its cascade tiles and its parameters.
+// A single-terrain context is an ensemble of one.
+struct Terrain {
+  long long cell_off;
+  int nx, ny;
+  int drop_off, drops;
+  int tile_off, tiles_y;
+  unsigned long long seed;
+  unsigned int id;  // terrain number in the spawn hash
+  int has_root;
+  shy_params P;
+};
+
+// Whole-ensemble arguments of the walk kernels (concatenated arrays).
+struct WaveArgs {
+  const Terrain* __restrict__ ter;
+  int nter;
+  int total_drops;
+  const float* __restrict__ h;
+  const float4* __restrict__ fld;
+  const float* __restrict__ root;
+  int32_t* count;
+  int32_t* ticket;
+  const int32_t* __restrict__ offset;
+  const float* __restrict__ scale;
+  int4* rec;
+  int32_t* deposit;
+  uint8_t* visit;
+  DeviceState* state;
+  long long* tsteps;  // particle-steps per terrain
+};
+
+// largest k in [0, nter) with key(k) <= v (keys ascending, key(0) <= v)
+template <class Key>
+__device__ inline int find_terrain(int nter, long long v, Key key) {
+  int lo = 0, hi = nter - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One terrain's view, as one drop walks it (pointers offset to the terrain).
 struct WalkArgs {
   const float* __restrict__ h;
   const float4* __restrict__ fld;  // {momentumx, momentumy, discharge, erf(0.4*discharge)}
@@ -68,6 +113,8 @@ struct WalkArgs {
 };
 
 struct CascArgs {
+  const Terrain* __restrict__ ter;
+  int nter;
   const float* __restrict__ h_in;
   float* __restrict__ h_out;
   const int32_t* __restrict__ dh;     // walk commits per cell, 2^-20
@@ -79,8 +126,6 @@ struct CascArgs {
   float4* fld;
   const double* __restrict__ erf_a;
   DeviceState* state;
-  int nx, ny;
-  shy_params P;
 };
 
 // ------------------------------------------------------------ reductions
@@ -235,27 +280,67 @@ __device__ void walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, 
   if (TRACE) *tr_n = nrec;
 }
 
+// Every drop of every terrain; grid-stride, so the grid may be smaller than
+// the wave (a persistent launch).
 template <int MODE>
-__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
   LaneStats S;
   long long walked = 0, skipped = 0;
-  if (tid < a.drops) {
-    const uint32_t ncell = (uint32_t)a.nx * (uint32_t)a.ny;
-    const uint64_t key = cycle_key(a.seed, a.terrain, a.state->cycle);
-    const uint32_t cid = spawn_cell(key, (uint32_t)tid, ncell);
-    if ((double)a.h[cid] < 0.1) {  // world.h:71-72
-      skipped = 1;
-    } else {
-      walked = 1;
-      walk_drop<MODE>(a, tid, (float)(cid / (uint32_t)a.ny), (float)(cid % (uint32_t)a.ny), S,
-                      nullptr, 0, nullptr);
+  const unsigned int cycle = w.state->cycle;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < w.total_drops;
+       g += gridDim.x * blockDim.x) {
+    // drop ranges are padded to whole waves, so the wave shares one terrain
+    // and its entry lives in scalar registers
+    const int g0 = __builtin_amdgcn_readfirstlane(g);
+    const int t = __builtin_amdgcn_readfirstlane(
+        find_terrain(w.nter, g0, [&](int k) { return (long long)w.ter[k].drop_off; }));
+    const Terrain T = w.ter[t];
+    long long mine = 0;  // this lane's particle-steps in this terrain
+    if (g - T.drop_off < T.drops) {  // not a padding lane
+      WalkArgs a;
+      a.h = w.h + T.cell_off;
+      a.fld = w.fld + T.cell_off;
+      a.root = T.has_root ? w.root + T.cell_off : nullptr;
+      a.count = w.count + T.cell_off;
+      a.ticket = w.ticket;
+      a.offset = w.offset + T.cell_off;
+      a.scale = w.scale ? w.scale + T.cell_off : nullptr;
+      a.rec = w.rec;
+      a.deposit = w.deposit + T.cell_off;
+      a.visit = w.visit + T.cell_off;
+      a.state = w.state;
+      a.seed = T.seed;
+      a.terrain = T.id;
+      a.nx = T.nx;
+      a.ny = T.ny;
+      a.drops = w.total_drops;  // ticket-log stride
+      a.P = T.P;
+      const uint32_t ncell = (uint32_t)T.nx * (uint32_t)T.ny;
+      const uint32_t cid =
+          spawn_cell(cycle_key(T.seed, T.id, cycle), (uint32_t)(g - T.drop_off), ncell);
+      if ((double)a.h[cid] < 0.1) {  // world.h:71-72
+        skipped += 1;
+      } else {
+        walked += 1;
+        const long long s0 = S.steps;
+        walk_drop<MODE>(a, g, (float)(cid / (uint32_t)T.ny), (float)(cid % (uint32_t)T.ny), S,
+                        nullptr, 0, nullptr);
+        mine = S.steps - s0;
+      }
+    }
+    if (MODE == kCommit && w.tsteps) {
+      // the wave is converged here (drop ranges are whole waves): one atomic
+      // per wave and terrain instead of one per drop on a shared counter
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, 64);
+      if ((threadIdx.x & 63) == 0 && mine)
+        __hip_atomic_fetch_add(&w.tsteps[t], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   if (MODE == kCommit) {
     long long v[ST_CASC_MOVES] = {S.steps, walked, skipped, S.exits, S.ends_age,
                                   S.ends_vol, S.export40, S.gain40, S.walk_dh};
-    block_add_stats<ST_CASC_MOVES>(v, a.state->stats);
+    block_add_stats<ST_CASC_MOVES>(v, w.state->stats);
   }
 }
 
@@ -330,7 +415,7 @@ __global__ void __launch_bounds__(kScanBlock) scan_top_kernel(int32_t* bsum, int
 __global__ void __launch_bounds__(kScanBlock) scan_cells_kernel(
     const int32_t* __restrict__ count, const int32_t* __restrict__ bsum, size_t n,
     int32_t* offset, float* scale, const float4* __restrict__ fld,
-    const float* __restrict__ root, shy_params P) {
+    const float* __restrict__ root, const Terrain* __restrict__ ter, int nter) {
   __shared__ int tmp[kScanBlock];
   const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
   const int k = i < n ? count[i] : 0;
@@ -338,7 +423,9 @@ __global__ void __launch_bounds__(kScanBlock) scan_cells_kernel(
   const int ex = block_exclusive_scan(k, tmp, total);
   if (i >= n) return;
   offset[i] = bsum[blockIdx.x] + ex;
-  float effD = P.depositionRate * (1.0f - (root ? root[i] : 0.0f));
+  const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
+  const shy_params& P = ter[t].P;
+  float effD = P.depositionRate * (1.0f - (ter[t].has_root ? root[i] : 0.0f));
   if (effD < 0) effD = 0;
   scale[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
 }
@@ -488,17 +575,53 @@ __device__ inline int cascade_site(int cx, int cy, int nx, int ny, const shy_par
   return moves;
 }
 
-constexpr int kCbY = 32;  // tile extent along y (contiguous)
-constexpr int kCbX = 8;   // tile extent along x
-constexpr int kCbThreads = kCbY * kCbX;
+// One terrain's view of the cascade/blend stencil.
+struct CascView {
+  const float* __restrict__ h_in;
+  float* __restrict__ h_out;
+  const int32_t* __restrict__ dh;
+  const uint8_t* __restrict__ visit;
+  const long long* __restrict__ td;
+  const long long* __restrict__ tmx;
+  const long long* __restrict__ tmy;
+  uint8_t* visit_clear;
+  float4* fld;
+  const double* __restrict__ erf_a;
+  DeviceState* state;
+  int nx, ny;
+  shy_params P;
+};
 
-__global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
+// The stencil is instantiated for the tile shapes the autotuner chooses
+// from (extent along y, the contiguous axis, by extent along x; 256 threads).
+template <int kCbY, int kCbX>
+__global__ void __launch_bounds__(kCbY * kCbX) cascade_blend_kernel(CascArgs e) {
+  constexpr int kCbThreads = kCbY * kCbX;
   __shared__ float Hs[kCbX + 4][kCbY + 4];   // post-walk heights, 2-cell halo
   __shared__ uint8_t Vs[kCbX + 6][kCbY + 6]; // cascade sites, 3-cell halo
   __shared__ uint8_t Wc[kCbX + 4][kCbY + 4]; // site windows covering a cell, 2-cell halo
   __shared__ int32_t Wd[(kCbX + 2) * (kCbY + 2)][9];
+  const int tt = find_terrain(e.nter, (long long)blockIdx.x,
+                              [&](int q) { return (long long)e.ter[q].tile_off; });
+  const Terrain& T = e.ter[tt];
+  const int lt = (int)blockIdx.x - T.tile_off;
+  CascView a;
+  a.h_in = e.h_in + T.cell_off;
+  a.h_out = e.h_out + T.cell_off;
+  a.dh = e.dh + T.cell_off;
+  a.visit = e.visit + T.cell_off;
+  a.td = e.td + T.cell_off;
+  a.tmx = e.tmx + T.cell_off;
+  a.tmy = e.tmy + T.cell_off;
+  a.visit_clear = e.visit_clear + T.cell_off;
+  a.fld = e.fld + T.cell_off;
+  a.erf_a = e.erf_a;
+  a.state = e.state;
+  a.nx = T.nx;
+  a.ny = T.ny;
+  a.P = T.P;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.y * kCbX, y0 = blockIdx.x * kCbY;
+  const int x0 = (lt / T.tiles_y) * kCbX, y0 = (lt % T.tiles_y) * kCbY;
   const int nx = a.nx, ny = a.ny;
   for (int k = tid; k < (kCbX + 4) * (kCbY + 4); k += kCbThreads) {
     const int lx = k / (kCbY + 4), ly = k % (kCbY + 4);
@@ -584,7 +707,7 @@ __global__ void __launch_bounds__(kCbThreads) cascade_blend_kernel(CascArgs a) {
   }
   long long v[1] = {moves};
   block_add_stats<1>(v, &a.state->stats[ST_CASC_MOVES]);
-  if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) a.state->cycle = a.state->cycle + 1u;
+  if (blockIdx.x == 0 && tid == 0) a.state->cycle = a.state->cycle + 1u;
 }
 
 // ---------------------------------------------------------- field kernels
