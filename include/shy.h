@@ -149,6 +149,13 @@ int shy_terrain_upload_field(shy_ctx* ctx, int terrain, int field, const float* 
 int shy_terrain_download_field(shy_ctx* ctx, int terrain, int field, float* dst);
 int shy_terrain_generate_fbm(shy_ctx* ctx, int terrain, uint64_t seed, int octaves,
                              float base_freq, float lacunarity, float gain);
+/* Device-to-device field transfer (no reference counterpart; for the
+   surrogate): export writes terrain t's (height, discharge, momentumx,
+   momentumy) as 4 floats per cell, x-major, into a caller-owned device
+   buffer; import_flow reads (discharge, momentumx, momentumy) as 3 floats per
+   cell and refreshes erf(0.4 discharge).  Both synchronise. */
+int shy_terrain_export(shy_ctx* ctx, int terrain, float* dst4_device);
+int shy_terrain_import_flow(shy_ctx* ctx, int terrain, const float* src3_device);
 /* particle-steps per terrain since the last shy_reset_stats */
 int shy_terrain_steps(shy_ctx* ctx, int64_t* steps);
 /* `cycles` World::erode calls of every terrain (each with its own drops);

@@ -1,0 +1,67 @@
+/*
+ * shy_nn.h -- BF16 MFMA kernels of the discharge/momentum surrogate.
+ *
+ * No reference counterpart: the reference converges discharge and momentum
+ * only by running the particle method (world.h:80-86).  These kernels train
+ * and run a small convolutional network that predicts the converged fields
+ * from a terrain, to warm-start the particle method.
+ *
+ * Tensors are device pointers the caller owns; activations are NHWC bf16;
+ * weights are bf16 Wt[Cout][taps*Cin] (tap-major, tap = 3*(dy+1)+(dx+1));
+ * biases and gradients FP32.  All products use v_mfma_f32_16x16x32_bf16 with
+ * FP32 accumulation.  Every call is asynchronous on `stream` (a hipStream_t;
+ * NULL = the default stream) and returns 0, or SHY_NN_ERR with
+ * shy_nn_last_error() describing the problem.
+ */
+#ifndef SHY_NN_H
+#define SHY_NN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHY_NN_ERR (-10)
+
+const char* shy_nn_last_error(void);
+
+/* y = act(conv(x, w) + bias [+ res]); taps 9 (3x3, pad 1) or 1; Cin a
+   multiple of 8; exactly one of y_bf16 / y_f32. */
+int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void* res,
+                    void* y_bf16, float* y_f32, int N, int H, int W, int Cin, int Cout,
+                    int taps, int relu, void* stream);
+/* weights for the data gradient: wf[Cin][taps*Cout] = w[Cout][(taps-1-t)*Cin] */
+int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, void* stream);
+/* dw[Cout][taps*Cin] (FP32) = sum over pixels of g[p][co] x[p+tap][ci];
+   part holds part_splits*Cout*taps*Cin floats of scratch */
+int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int part_splits,
+                      int N, int H, int W, int Cin, int Cout, int taps, void* stream);
+/* db[c] = sum over pixels of g[p][c]; part holds part_rows*C floats */
+int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P, int C,
+                     void* stream);
+int shy_nn_relu_bwd(const void* y, const void* g, void* out, size_t n, void* stream);
+/* 2x2 average pool (backward: x = dy, y = dx at full size) */
+int shy_nn_pool2(const void* x, void* y, int N, int H, int W, int C, int backward, void* stream);
+/* nearest x2 upsample of x [N][H][W][C] into channels [c_off, c_off+C) of
+   y [N][2H][2W][Cy]; backward sums the 4 children back */
+int shy_nn_up2(const void* x, void* y, int N, int H, int W, int C, int Cy, int c_off,
+               int backward, void* stream);
+/* dir 0: copy src [P][C] into channels [c_off, c_off+C) of dst [P][Cy];
+   dir 1: the reverse */
+int shy_nn_channels(const void* src, void* dst, size_t P, int C, int Cy, int c_off, int dir,
+                    void* stream);
+int shy_nn_add(const void* a, const void* b, void* y, size_t n, void* stream);
+/* masked squared error: g = 2 (y-t) m inv_count (bf16), part[block] = sum of
+   ((y-t) m)^2 over 256-element blocks (ceil(n/256) floats) */
+int shy_nn_mse(const float* y, const float* t, const float* mask, void* g_bf16, float* part,
+               size_t n, float inv_count, void* stream);
+/* Adam on FP32 master weights, optional bf16 copy for the MFMA kernels */
+int shy_nn_adam(float* w, const float* g, float* m, float* v, void* w_bf16, size_t n, float lr,
+                float b1, float b2, float eps, int step, float weight_decay, void* stream);
+int shy_nn_cast(const void* src, void* dst, size_t n, int to_bf16, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -144,6 +144,8 @@ def lib():
             "shy_set_tuning": (i32, [vp, vp]),
             "shy_get_tuning": (i32, [vp, vp]),
             "shy_autotune": (i32, [vp, i32, vp, fp]),
+            "shy_terrain_export": (i32, [vp, i32, vp]),
+            "shy_terrain_import_flow": (i32, [vp, i32, vp]),
             "shy_download_normals": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -335,6 +337,14 @@ class World:
         _check(self._ctx, lib().shy_autotune(self._ctx, cycles, C.byref(t), C.byref(ms)))
         return t.as_dict(), ms.value
 
+    def export(self, dst_ptr: int):
+        """(h, d, mx, my) into a device buffer (4 floats per cell)."""
+        _check(self._ctx, lib().shy_terrain_export(self._ctx, 0, dst_ptr))
+
+    def import_flow(self, src_ptr: int):
+        """(d, mx, my) from a device buffer (3 floats per cell)."""
+        _check(self._ctx, lib().shy_terrain_import_flow(self._ctx, 0, src_ptr))
+
     def time_cycles(self, cycles: int, drops: int) -> dict:
         """Run ``cycles`` erode calls with HIP events between kernels on the
         context's stream; returns milliseconds per kernel, summed."""
@@ -408,6 +418,14 @@ class Ensemble(World):
 
     def erode(self, cycles: int = 1, graph: bool = True):
         _check(self._ctx, lib().shy_ensemble_erode(self._ctx, cycles, 1 if graph else 0))
+
+    def export(self, terrain: int, dst_ptr: int):
+        """(h, d, mx, my) of one terrain into a device buffer (4 floats/cell)."""
+        _check(self._ctx, lib().shy_terrain_export(self._ctx, terrain, dst_ptr))
+
+    def import_flow(self, terrain: int, src_ptr: int):
+        """(d, mx, my) of one terrain from a device buffer (3 floats/cell)."""
+        _check(self._ctx, lib().shy_terrain_import_flow(self._ctx, terrain, src_ptr))
 
     def terrain_steps(self) -> np.ndarray:
         out = np.zeros(self.nter, np.int64)

@@ -716,6 +716,33 @@ int shy_terrain_download_field(shy_ctx* c, int t, int field, float* dst) {
   return SHY_OK;
 }
 
+int shy_terrain_export(shy_ctx* c, int t, float* dst4) {
+  RETURN_IF(check_terrain(c, t));
+  if (!dst4) return fail(c, SHY_ERR_ARG, "shy_terrain_export: NULL argument");
+  const Terrain& T = c->ter[t];
+  const size_t n = (size_t)T.nx * T.ny;
+  hipSetDevice(c->device);
+  export_fields_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->h[parity(c)] + T.cell_off,
+                                                               c->fld + T.cell_off, n, dst4);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SHY_OK;
+}
+
+int shy_terrain_import_flow(shy_ctx* c, int t, const float* src3) {
+  RETURN_IF(check_terrain(c, t));
+  if (!src3) return fail(c, SHY_ERR_ARG, "shy_terrain_import_flow: NULL argument");
+  const Terrain& T = c->ter[t];
+  const size_t n = (size_t)T.nx * T.ny;
+  hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  import_flow_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(src3, c->fld + T.cell_off, n,
+                                                             c->erf_a);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SHY_OK;
+}
+
 int shy_upload_field(shy_ctx* c, int field, const float* src) {
   RETURN_IF(need_single(c, "shy_upload_field"));
   return shy_terrain_upload_field(c, 0, field, src);

@@ -711,6 +711,28 @@ __global__ void __launch_bounds__(kCbY * kCbX) cascade_blend_kernel(CascArgs e) 
 }
 
 // ---------------------------------------------------------- field kernels
+// terrain fields to a caller's device buffer: 4 floats per cell (h, d, mx, my)
+__global__ void export_fields_kernel(const float* __restrict__ h, const float4* __restrict__ fld,
+                                     size_t n, float* dst) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 f = fld[i];
+  dst[4 * i] = h[i];
+  dst[4 * i + 1] = f.z;
+  dst[4 * i + 2] = f.x;
+  dst[4 * i + 3] = f.y;
+}
+
+// discharge/momentum from a caller's device buffer (3 floats per cell:
+// d, mx, my); refreshes erf(0.4 d)
+__global__ void import_flow_kernel(const float* __restrict__ src, float4* fld, size_t n,
+                                   const double* __restrict__ erf_a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = src[3 * i];
+  fld[i] = make_float4(src[3 * i + 1], src[3 * i + 2], d, erf_f32(erf_a, 0.4f * d));
+}
+
 // quad::map::normal (cellpool.h:181-204, 445-447) of every cell, xyz.
 __global__ void normals_kernel(const float* __restrict__ h, int nx, int ny, float* out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

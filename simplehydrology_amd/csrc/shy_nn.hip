@@ -1,0 +1,536 @@
+// shy_nn.hip -- BF16 MFMA kernels of the discharge/momentum surrogate (gfx950).
+//
+// Activations are NHWC bf16 (channels contiguous).  A 3x3 (pad 1) or 1x1
+// convolution is an implicit GEMM: M = pixels, N = output channels,
+// K = taps x input channels (tap-major), weights stored Wt[Cout][K] so both
+// MFMA operands load 8 consecutive k as one 16-byte vector.  Every product
+// runs on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Lane layout of
+// that instruction (checked on the device): lane l holds A[l%16][(l/16)*8+i],
+// B[(l/16)*8+i][l%16] for i < 8 and C[(l/16)*4+i][l%16] for i < 4.
+//
+//   conv_fwd_kernel    forward; epilogue adds bias, an optional residual and
+//                      an optional ReLU, stores bf16 or f32
+//   (dgrad)            the same kernel on the flipped, transposed weights
+//                      built by flip_weights_kernel
+//   conv_wgrad_kernel  dW[co][k] = sum_p g[p][co] x[p + tap][ci]: K = pixels,
+//                      staged through LDS (both operands need 8 consecutive
+//                      pixels per lane), split over pixel ranges into FP32
+//                      partials that wgrad_reduce_kernel sums in a fixed order
+//   elementwise        ReLU backward, 2x2 average pool, nearest upsample,
+//                      channel concat/split, bias gradient, MSE loss, Adam
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/shy_nn.h"
+
+namespace shy_nn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z[i] = (__bf16)0.0f;
+  return z;
+}
+
+__device__ inline bf16x8 load8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+struct ConvArgs {
+  const __bf16* __restrict__ x;    // [P][Cin], P = N*H*W
+  const __bf16* __restrict__ w;    // [Cout][K]
+  const float* __restrict__ bias;  // [Cout] or null
+  const __bf16* __restrict__ res;  // [P][Cout] or null (added before the ReLU)
+  __bf16* y_bf;                    // [P][Cout] (one of y_bf / y_f)
+  float* y_f;
+  int N, H, W, Cin, Cout, K;
+  int relu;
+};
+
+// Forward (and dgrad) convolution.  Block = 4 waves; each wave owns 32 pixels
+// x 16*NT output channels as 2 x NT MFMA accumulators.
+template <int TAPS, int NT>
+__global__ void __launch_bounds__(256) conv_fwd_kernel(ConvArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int P = a.N * a.H * a.W;
+  const int pb = (blockIdx.x * 4 + wave) * 32;
+  const int cb = blockIdx.y * 16 * NT;
+  const int K = a.K, Cin = a.Cin;
+  int pn[2], py[2], px[2];
+  bool pv[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int p = pb + r * 16 + (lane & 15);
+    pv[r] = p < P;
+    const int pp = pv[r] ? p : 0;
+    px[r] = pp % a.W;
+    py[r] = (pp / a.W) % a.H;
+    pn[r] = pp / (a.W * a.H);
+  }
+  f32x4 acc[2][NT];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < NT; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int ksteps = (K + 31) / 32;
+  for (int ks = 0; ks < ksteps; ++ks) {
+    const int k0 = ks * 32 + (lane >> 4) * 8;
+    const bool kv = k0 < K;
+    const int tap = kv ? k0 / Cin : 0;
+    const int ci0 = kv ? k0 - tap * Cin : 0;
+    const int dy = TAPS == 9 ? tap / 3 - 1 : 0;
+    const int dx = TAPS == 9 ? tap % 3 - 1 : 0;
+    bf16x8 A[2], B[NT];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int yy = py[r] + dy, xx = px[r] + dx;
+      const bool ok = kv && pv[r] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+      A[r] = ok ? load8(a.x + ((size_t)(pn[r] * a.H + yy) * a.W + xx) * Cin + ci0) : zero8();
+    }
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+      const int co = cb + c * 16 + (lane & 15);
+      B[c] = (kv && co < a.Cout) ? load8(a.w + (size_t)co * K + k0) : zero8();
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < NT; ++c) acc[r][c] = mfma(A[r], B[c], acc[r][c]);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+      const int co = cb + c * 16 + (lane & 15);
+      if (co >= a.Cout) continue;
+      const float bv = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = pb + r * 16 + (lane >> 4) * 4 + i;
+        if (p >= P) continue;
+        float v = acc[r][c][i] + bv;
+        const size_t o = (size_t)p * a.Cout + co;
+        if (a.res) v += (float)a.res[o];
+        if (a.relu) v = v > 0.0f ? v : 0.0f;
+        if (a.y_f) a.y_f[o] = v;
+        else a.y_bf[o] = (__bf16)v;
+      }
+    }
+}
+
+// dgrad weights: Wf[ci][tap'*Cout + co] = W[co][(TAPS-1-tap')*Cin + ci]
+__global__ void flip_weights_kernel(const __bf16* __restrict__ w, __bf16* wf, int Cin, int Cout,
+                                    int taps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = Cin * Cout * taps;
+  if (i >= n) return;
+  const int ci = i / (taps * Cout);
+  const int rem = i % (taps * Cout);
+  const int tp = rem / Cout, co = rem % Cout;
+  wf[i] = w[(size_t)co * taps * Cin + (taps - 1 - tp) * Cin + ci];
+}
+
+struct WgradArgs {
+  const __bf16* __restrict__ x;  // [P][Cin] input of the convolution
+  const __bf16* __restrict__ g;  // [P][Cout] gradient of its output
+  float* part;                   // [splits][Cout][K]
+  int N, H, W, Cin, Cout, K, taps;
+  int chunk;                     // pixels per split (multiple of 32)
+};
+
+// One wave per block: a 32 (co) x 32 (k) tile of dW over one pixel range.
+__global__ void __launch_bounds__(64) conv_wgrad_kernel(WgradArgs a) {
+  __shared__ __bf16 Gs[32][32 + 8];  // [pixel][co]
+  __shared__ __bf16 Xs[32][32 + 8];  // [pixel][k]
+  const int lane = threadIdx.x;
+  const int P = a.N * a.H * a.W;
+  const int cb = blockIdx.x * 32, kb = blockIdx.y * 32;
+  const int p_begin = blockIdx.z * a.chunk;
+  const int p_end = min(P, p_begin + a.chunk);
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int p0 = p_begin; p0 < p_end; p0 += 32) {
+    // stage: 32 pixels x 4 groups of 8 channels, for G and for the tap-shifted X
+    for (int t = lane; t < 128; t += 64) {
+      const int pi = t >> 2, grp = t & 3;
+      const int p = p0 + pi;
+      const int co = cb + grp * 8;
+      bf16x8 gv = zero8();
+      if (p < p_end && co < a.Cout) gv = load8(a.g + (size_t)p * a.Cout + co);
+      const int k0 = kb + grp * 8;
+      bf16x8 xv = zero8();
+      if (p < p_end && k0 < a.K) {
+        const int tap = k0 / a.Cin, ci0 = k0 - tap * a.Cin;
+        const int dy = a.taps == 9 ? tap / 3 - 1 : 0, dx = a.taps == 9 ? tap % 3 - 1 : 0;
+        const int xx = p % a.W, yy = (p / a.W) % a.H, nn = p / (a.W * a.H);
+        const int y2 = yy + dy, x2 = xx + dx;
+        if (y2 >= 0 && y2 < a.H && x2 >= 0 && x2 < a.W)
+          xv = load8(a.x + ((size_t)(nn * a.H + y2) * a.W + x2) * a.Cin + ci0);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        Gs[pi][grp * 8 + i] = gv[i];
+        Xs[pi][grp * 8 + i] = xv[i];
+      }
+    }
+    __syncthreads();
+    // A[m = co][kk = pixel], B[kk = pixel][n = k]
+    bf16x8 A[2], B[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) A[r][i] = Gs[(lane >> 4) * 8 + i][r * 16 + (lane & 15)];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) B[c][i] = Xs[(lane >> 4) * 8 + i][c * 16 + (lane & 15)];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[r][c] = mfma(A[r], B[c], acc[r][c]);
+    __syncthreads();
+  }
+  float* out = a.part + (size_t)blockIdx.z * a.Cout * a.K;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = cb + r * 16 + (lane >> 4) * 4 + i;
+        const int k = kb + c * 16 + (lane & 15);
+        if (co < a.Cout && k < a.K) out[(size_t)co * a.K + k] = acc[r][c][i];
+      }
+}
+
+// dW = sum over splits in a fixed order (deterministic)
+__global__ void wgrad_reduce_kernel(const float* __restrict__ part, float* dw, int n, int splits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += part[(size_t)k * n + i];
+  dw[i] = s;
+}
+
+// db[co] = sum_p g[p][co]: per-block partial column sums, fixed order
+__global__ void bias_grad_kernel(const __bf16* __restrict__ g, float* part, int P, int C,
+                                 int rows_per_block) {
+  const int co = threadIdx.x;  // blockDim.x >= C
+  if (co >= C) return;
+  const int p0 = blockIdx.x * rows_per_block;
+  const int p1 = min(P, p0 + rows_per_block);
+  float s = 0.0f;
+  for (int p = p0; p < p1; ++p) s += (float)g[(size_t)p * C + co];
+  part[(size_t)blockIdx.x * C + co] = s;
+}
+
+__global__ void relu_bwd_kernel(const __bf16* __restrict__ y, const __bf16* __restrict__ g,
+                                __bf16* out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (float)y[i] > 0.0f ? g[i] : (__bf16)0.0f;
+}
+
+// 2x2 average pool, NHWC, H and W even
+__global__ void pool2_fwd_kernel(const __bf16* __restrict__ x, __bf16* y, int N, int H, int W,
+                                 int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ho = H / 2, Wo = W / 2;
+  const size_t n = (size_t)N * Ho * Wo * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  const size_t q = i / C;
+  const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), nn = (int)(q / ((size_t)Wo * Ho));
+  const size_t b = ((size_t)(nn * H + 2 * yo) * W + 2 * xo) * C + c;
+  const float s = (float)x[b] + (float)x[b + C] + (float)x[b + (size_t)W * C] +
+                  (float)x[b + (size_t)W * C + C];
+  y[i] = (__bf16)(0.25f * s);
+}
+
+__global__ void pool2_bwd_kernel(const __bf16* __restrict__ gy, __bf16* gx, int N, int H, int W,
+                                 int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)N * H * W * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  const size_t q = i / C;
+  const int x = (int)(q % W), y = (int)((q / W) % H), nn = (int)(q / ((size_t)W * H));
+  const int Ho = H / 2, Wo = W / 2;
+  gx[i] = (__bf16)(0.25f * (float)gy[((size_t)(nn * Ho + y / 2) * Wo + x / 2) * C + c]);
+}
+
+// nearest x2 upsample of x [N][H][W][C] into channels [c_off, c_off+C) of
+// y [N][2H][2W][Cy] (the first half of a concat)
+__global__ void up2_fwd_kernel(const __bf16* __restrict__ x, __bf16* y, int N, int H, int W, int C,
+                               int Cy, int c_off) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ho = 2 * H, Wo = 2 * W;
+  const size_t n = (size_t)N * Ho * Wo * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  const size_t q = i / C;
+  const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), nn = (int)(q / ((size_t)Wo * Ho));
+  y[q * Cy + c_off + c] = x[((size_t)(nn * H + yo / 2) * W + xo / 2) * C + c];
+}
+
+__global__ void up2_bwd_kernel(const __bf16* __restrict__ gy, __bf16* gx, int N, int H, int W,
+                               int C, int Cy, int c_off) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)N * H * W * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  const size_t q = i / C;
+  const int x = (int)(q % W), y = (int)((q / W) % H), nn = (int)(q / ((size_t)W * H));
+  const int Wo = 2 * W;
+  const size_t b = ((size_t)(nn * 2 * H + 2 * y) * Wo + 2 * x) * Cy + c_off + c;
+  const float s = (float)gy[b] + (float)gy[b + Cy] + (float)gy[b + (size_t)Wo * Cy] +
+                  (float)gy[b + (size_t)Wo * Cy + Cy];
+  gx[i] = (__bf16)s;
+}
+
+// copy x [P][C] into channels [c_off, c_off+C) of y [P][Cy] (dir 0), or back (dir 1)
+__global__ void channels_copy_kernel(const __bf16* __restrict__ src, __bf16* dst, size_t P, int C,
+                                     int Cy, int c_off, int dir) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * C) return;
+  const size_t p = i / C;
+  const int c = (int)(i % C);
+  if (dir == 0) dst[p * Cy + c_off + c] = src[i];
+  else dst[i] = src[p * Cy + c_off + c];
+}
+
+__global__ void add_kernel(const __bf16* __restrict__ a, const __bf16* __restrict__ b, __bf16* y,
+                           size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = (__bf16)((float)a[i] + (float)b[i]);
+}
+
+// masked MSE: loss partials and d(loss)/dy = 2 (y - t) m / count
+__global__ void mse_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                           const float* __restrict__ m, __bf16* g, float* part, size_t n,
+                           float inv_count) {
+  __shared__ float red[256];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < n) {
+    const float mk = m ? m[i] : 1.0f;
+    const float d = (y[i] - t[i]) * mk;
+    l = d * d;
+    g[i] = (__bf16)(2.0f * d * inv_count);
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; optional bf16 copy
+__global__ void adam_kernel(float* w, const float* __restrict__ g, float* m, float* v,
+                            __bf16* w_bf, size_t n, float lr, float b1, float b2, float eps,
+                            float c1, float c2, float wd) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] + wd * w[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (w_bf) w_bf[i] = (__bf16)wi;
+}
+
+__global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (__bf16)x[i];
+}
+
+__global__ void bf16_to_f32_kernel(const __bf16* __restrict__ x, float* y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+}  // namespace shy_nn
+
+using namespace shy_nn;
+
+namespace {
+thread_local std::string g_nn_err;
+int nn_fail(const std::string& m) {
+  g_nn_err = m;
+  return SHY_NN_ERR;
+}
+int nn_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return nn_fail(std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+inline unsigned blocks(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+}  // namespace
+
+extern "C" {
+
+const char* shy_nn_last_error(void) { return g_nn_err.c_str(); }
+
+int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void* res, void* y_bf16,
+                    float* y_f32, int N, int H, int W, int Cin, int Cout, int taps, int relu,
+                    void* stream) {
+  if (!x || !w || (!y_bf16 && !y_f32) || (y_bf16 && y_f32)) return nn_fail("conv_fwd: bad pointers");
+  if (N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (taps != 9 && taps != 1))
+    return nn_fail("conv_fwd: need Cin a multiple of 8 and taps 9 or 1");
+  ConvArgs a;
+  a.x = (const __bf16*)x;
+  a.w = (const __bf16*)w;
+  a.bias = bias;
+  a.res = (const __bf16*)res;
+  a.y_bf = (__bf16*)y_bf16;
+  a.y_f = y_f32;
+  a.N = N; a.H = H; a.W = W; a.Cin = Cin; a.Cout = Cout; a.K = taps * Cin;
+  a.relu = relu;
+  const long long P = (long long)N * H * W;
+  if (P > INT32_MAX / 2) return nn_fail("conv_fwd: too many pixels");
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gx = (unsigned)((P + 127) / 128);
+  if (Cout > 16) {
+    dim3 g(gx, (unsigned)((Cout + 31) / 32));
+    if (taps == 9) conv_fwd_kernel<9, 2><<<g, 256, 0, s>>>(a);
+    else conv_fwd_kernel<1, 2><<<g, 256, 0, s>>>(a);
+  } else {
+    dim3 g(gx, 1);
+    if (taps == 9) conv_fwd_kernel<9, 1><<<g, 256, 0, s>>>(a);
+    else conv_fwd_kernel<1, 1><<<g, 256, 0, s>>>(a);
+  }
+  return nn_check("conv_fwd");
+}
+
+int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, void* stream) {
+  if (!w || !wf || (taps != 9 && taps != 1)) return nn_fail("flip_weights: bad argument");
+  const int n = Cin * Cout * taps;
+  flip_weights_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)w, (__bf16*)wf,
+                                                                  Cin, Cout, taps);
+  return nn_check("flip_weights");
+}
+
+int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int part_splits,
+                      int N, int H, int W, int Cin, int Cout, int taps, void* stream) {
+  if (!x || !g || !dw || !part || part_splits < 1) return nn_fail("conv_wgrad: bad pointers");
+  if (Cin % 8 || Cout % 8 || (taps != 9 && taps != 1)) return nn_fail("conv_wgrad: channels must be multiples of 8");
+  const long long P = (long long)N * H * W;
+  int chunk = (int)((P + part_splits - 1) / part_splits);
+  chunk = (chunk + 31) / 32 * 32;
+  const int splits = (int)((P + chunk - 1) / chunk);
+  WgradArgs a;
+  a.x = (const __bf16*)x;
+  a.g = (const __bf16*)g;
+  a.part = part;
+  a.N = N; a.H = H; a.W = W; a.Cin = Cin; a.Cout = Cout; a.K = taps * Cin; a.taps = taps;
+  a.chunk = chunk;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)((Cout + 31) / 32), (unsigned)((a.K + 31) / 32), (unsigned)splits);
+  conv_wgrad_kernel<<<grid, 64, 0, s>>>(a);
+  if (int r = nn_check("conv_wgrad")) return r;
+  const int n = Cout * a.K;
+  wgrad_reduce_kernel<<<blocks(n), 256, 0, s>>>(part, dw, n, splits);
+  return nn_check("wgrad_reduce");
+}
+
+int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P, int C,
+                     void* stream) {
+  if (!g || !db || !part || C > 1024 || part_rows < 1) return nn_fail("bias_grad: bad argument");
+  const int rows = (P + part_rows - 1) / part_rows;
+  const int nb = (P + rows - 1) / rows;
+  hipStream_t s = (hipStream_t)stream;
+  const int bs = (C + 63) / 64 * 64;
+  bias_grad_kernel<<<nb, bs, 0, s>>>((const __bf16*)g, part, P, C, rows);
+  if (int r = nn_check("bias_grad")) return r;
+  wgrad_reduce_kernel<<<blocks(C), 256, 0, s>>>(part, db, C, nb);
+  return nn_check("bias_grad_reduce");
+}
+
+int shy_nn_relu_bwd(const void* y, const void* g, void* out, size_t n, void* stream) {
+  relu_bwd_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)y, (const __bf16*)g,
+                                                               (__bf16*)out, n);
+  return nn_check("relu_bwd");
+}
+
+int shy_nn_pool2(const void* x, void* y, int N, int H, int W, int C, int backward, void* stream) {
+  if (H % 2 || W % 2) return nn_fail("pool2: H and W must be even");
+  hipStream_t s = (hipStream_t)stream;
+  if (!backward)
+    pool2_fwd_kernel<<<blocks((size_t)N * H * W * C / 4), 256, 0, s>>>((const __bf16*)x, (__bf16*)y,
+                                                                       N, H, W, C);
+  else
+    pool2_bwd_kernel<<<blocks((size_t)N * H * W * C), 256, 0, s>>>((const __bf16*)x, (__bf16*)y, N,
+                                                                   H, W, C);
+  return nn_check("pool2");
+}
+
+int shy_nn_up2(const void* x, void* y, int N, int H, int W, int C, int Cy, int c_off, int backward,
+               void* stream) {
+  if (c_off < 0 || c_off + C > Cy) return nn_fail("up2: channel range outside the target");
+  hipStream_t s = (hipStream_t)stream;
+  if (!backward)
+    up2_fwd_kernel<<<blocks((size_t)N * 4 * H * W * C), 256, 0, s>>>(
+        (const __bf16*)x, (__bf16*)y, N, H, W, C, Cy, c_off);
+  else
+    up2_bwd_kernel<<<blocks((size_t)N * H * W * C), 256, 0, s>>>((const __bf16*)x, (__bf16*)y, N, H,
+                                                                 W, C, Cy, c_off);
+  return nn_check("up2");
+}
+
+int shy_nn_channels(const void* src, void* dst, size_t P, int C, int Cy, int c_off, int dir,
+                    void* stream) {
+  if (c_off < 0 || c_off + C > Cy) return nn_fail("channels: range outside the target");
+  channels_copy_kernel<<<blocks(P * C), 256, 0, (hipStream_t)stream>>>(
+      (const __bf16*)src, (__bf16*)dst, P, C, Cy, c_off, dir);
+  return nn_check("channels");
+}
+
+int shy_nn_add(const void* a, const void* b, void* y, size_t n, void* stream) {
+  add_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)a, (const __bf16*)b,
+                                                          (__bf16*)y, n);
+  return nn_check("add");
+}
+
+int shy_nn_mse(const float* y, const float* t, const float* mask, void* g_bf16, float* part,
+               size_t n, float inv_count, void* stream) {
+  mse_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>(y, t, mask, (__bf16*)g_bf16, part, n,
+                                                          inv_count);
+  return nn_check("mse");
+}
+
+int shy_nn_adam(float* w, const float* g, float* m, float* v, void* w_bf16, size_t n, float lr,
+                float b1, float b2, float eps, int step, float weight_decay, void* stream) {
+  if (step < 1) return nn_fail("adam: step must be >= 1");
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  adam_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>(w, g, m, v, (__bf16*)w_bf16, n, lr, b1,
+                                                           b2, eps, c1, c2, weight_decay);
+  return nn_check("adam");
+}
+
+int shy_nn_cast(const void* src, void* dst, size_t n, int to_bf16, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (to_bf16)
+    f32_to_bf16_kernel<<<blocks(n), 256, 0, s>>>((const float*)src, (__bf16*)dst, n);
+  else
+    bf16_to_f32_kernel<<<blocks(n), 256, 0, s>>>((const __bf16*)src, (float*)dst, n);
+  return nn_check("cast");
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+"""Convolutional surrogate that warm-starts discharge and momentum.
+
+No reference counterpart: the reference converges the exponentially averaged
+discharge/momentum fields (source/world.h:80-86) only by running the particle
+method.  This module predicts the converged fields from a terrain so the
+particle method can start near its fixed point.  The particle method stays
+the fallback and the judge (:func:`verify_warm_start`).
+
+The network is a 4-level residual U-Net, 3x3 convolutions,
+32-64-96-128 channels, BF16 weights and activations with FP32 accumulation.
+Every operation -- forward, backward (data and weight gradients), pooling,
+upsampling, loss and Adam -- is a HIP kernel of ``libshy.so``
+(``csrc/shy_nn.hip``, BF16 MFMA for all convolutions).  torch supplies device
+memory and the stream only.
+
+Input channels (NHWC, 8 with zero padding): height, then the transformed
+discharge and momentum of :func:`encode_flow`.  Output: 3 channels in the
+same transformed space (the head has 8 outputs; the last 5 are unused).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import lib as _shy_lib
+
+_nn = None
+
+
+def nn():
+    global _nn
+    if _nn is None:
+        L = _shy_lib()
+        vp, i32, f32, sz = C.c_void_p, C.c_int, C.c_float, C.c_size_t
+        sig = {
+            "shy_nn_last_error": (C.c_char_p, []),
+            "shy_nn_conv_fwd": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32,
+                                      vp]),
+            "shy_nn_flip_weights": (i32, [vp, vp, i32, i32, i32, vp]),
+            "shy_nn_conv_wgrad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
+            "shy_nn_bias_grad": (i32, [vp, vp, vp, i32, i32, i32, vp]),
+            "shy_nn_relu_bwd": (i32, [vp, vp, vp, sz, vp]),
+            "shy_nn_pool2": (i32, [vp, vp, i32, i32, i32, i32, i32, vp]),
+            "shy_nn_up2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
+            "shy_nn_channels": (i32, [vp, vp, sz, i32, i32, i32, i32, vp]),
+            "shy_nn_add": (i32, [vp, vp, vp, sz, vp]),
+            "shy_nn_mse": (i32, [vp, vp, vp, vp, vp, sz, f32, vp]),
+            "shy_nn_adam": (i32, [vp, vp, vp, vp, vp, sz, f32, f32, f32, f32, i32, f32, vp]),
+            "shy_nn_cast": (i32, [vp, vp, sz, i32, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _nn = L
+    return _nn
+
+
+def _ck(rc):
+    if rc != 0:
+        raise RuntimeError("shy_nn: " + nn().shy_nn_last_error().decode())
+
+
+def _p(t):
+    return None if t is None else t.data_ptr()
+
+
+def _stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+# ------------------------------------------------------------------ encoding
+D_SCALE = 5.0
+
+
+def encode_flow(d, mx, my):
+    """(discharge, momentum) -> network space: log1p(d)/5, signed log1p(|m|)/5."""
+    import torch
+    return torch.stack([torch.log1p(d.clamp_min(0)) / D_SCALE,
+                        torch.sign(mx) * torch.log1p(mx.abs()) / D_SCALE,
+                        torch.sign(my) * torch.log1p(my.abs()) / D_SCALE], -1)
+
+
+def decode_flow(y):
+    import torch
+    d = torch.expm1((y[..., 0] * D_SCALE).clamp(max=20.0)).clamp_min(0)
+    mx = torch.sign(y[..., 1]) * torch.expm1((y[..., 1].abs() * D_SCALE).clamp(max=20.0))
+    my = torch.sign(y[..., 2]) * torch.expm1((y[..., 2].abs() * D_SCALE).clamp(max=20.0))
+    return d, mx, my
+
+
+def make_input(h, d, mx, my):
+    """NHWC bf16 input with 8 channels: h, encoded flow, zeros."""
+    import torch
+    x = torch.zeros(*h.shape, 8, dtype=torch.float32, device=h.device)
+    x[..., 0] = h
+    x[..., 1:4] = encode_flow(d, mx, my)
+    return x.to(torch.bfloat16).contiguous()
+
+
+# -------------------------------------------------------------------- layers
+class Conv:
+    """taps x taps convolution with FP32 master weights and a bf16 copy."""
+
+    def __init__(self, cin, cout, taps, gen, device):
+        import torch
+        self.cin, self.cout, self.taps = cin, cout, taps
+        fan_in = cin * taps
+        std = math.sqrt(2.0 / fan_in)
+        w = torch.from_numpy(gen.standard_normal((cout, taps * cin)).astype(np.float32) * std)
+        self.w = w.to(device)
+        self.b = torch.zeros(cout, dtype=torch.float32, device=device)
+        self.w_bf = self.w.to(torch.bfloat16)
+        self.wf_bf = torch.empty(cin * taps * cout, dtype=torch.bfloat16, device=device)
+        self.dw = torch.zeros_like(self.w)
+        self.db = torch.zeros_like(self.b)
+        self.m = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
+        self.v = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
+
+    def forward(self, x, shape, res=None, relu=True, out_f32=False):
+        import torch
+        N, H, W = shape
+        y = torch.empty(N, H, W, self.cout, device=x.device,
+                        dtype=torch.float32 if out_f32 else torch.bfloat16)
+        _ck(nn().shy_nn_conv_fwd(_p(x), _p(self.w_bf), _p(self.b), _p(res),
+                                 None if out_f32 else _p(y), _p(y) if out_f32 else None,
+                                 N, H, W, self.cin, self.cout, self.taps, 1 if relu else 0,
+                                 _stream()))
+        return y
+
+    def backward(self, x, g, shape, scratch, need_dx=True):
+        """g: gradient of the pre-activation output (bf16). Returns dx (bf16)."""
+        import torch
+        N, H, W = shape
+        P = N * H * W
+        part, splits = scratch.part_for(self, P)
+        _ck(nn().shy_nn_conv_wgrad(_p(x), _p(g), _p(self.dw), _p(part), splits, N, H, W,
+                                   self.cin, self.cout, self.taps, _stream()))
+        _ck(nn().shy_nn_bias_grad(_p(g), _p(self.db), _p(scratch.bias_part), scratch.bias_rows,
+                                  P, self.cout, _stream()))
+        if not need_dx:
+            return None
+        _ck(nn().shy_nn_flip_weights(_p(self.w_bf), _p(self.wf_bf), self.cin, self.cout,
+                                     self.taps, _stream()))
+        dx = torch.empty(N, H, W, self.cin, dtype=torch.bfloat16, device=g.device)
+        _ck(nn().shy_nn_conv_fwd(_p(g), _p(self.wf_bf), None, None, _p(dx), None, N, H, W,
+                                 self.cout, self.cin, self.taps, 0, _stream()))
+        return dx
+
+    def step(self, lr, t, wd=0.0):
+        s = _stream()
+        _ck(nn().shy_nn_adam(_p(self.w), _p(self.dw), _p(self.m[0]), _p(self.v[0]),
+                             _p(self.w_bf), self.w.numel(), lr, 0.9, 0.999, 1e-8, t, wd, s))
+        _ck(nn().shy_nn_adam(_p(self.b), _p(self.db), _p(self.m[1]), _p(self.v[1]), None,
+                             self.b.numel(), lr, 0.9, 0.999, 1e-8, t, 0.0, s))
+
+    def params(self):
+        return [self.w, self.b]
+
+
+class Scratch:
+    """FP32 partial-sum buffers for the weight and bias gradients."""
+
+    def __init__(self, device, max_floats=1 << 24):
+        import torch
+        self.part = torch.empty(max_floats, dtype=torch.float32, device=device)
+        self.bias_rows = 1024
+        self.bias_part = torch.empty(self.bias_rows * 1024, dtype=torch.float32, device=device)
+
+    def part_for(self, conv, P):
+        tiles = ((conv.cout + 31) // 32) * ((conv.taps * conv.cin + 31) // 32)
+        splits = max(1, min(P // 512, -(-4096 // tiles)))
+        splits = min(splits, self.part.numel() // (conv.cout * conv.taps * conv.cin))
+        return self.part, max(1, splits)
+
+
+def _relu_bwd(y, g):
+    import torch
+    out = torch.empty_like(g)
+    _ck(nn().shy_nn_relu_bwd(_p(y), _p(g), _p(out), g.numel(), _stream()))
+    return out
+
+
+def _add(a, b):
+    import torch
+    out = torch.empty_like(a)
+    _ck(nn().shy_nn_add(_p(a), _p(b), _p(out), a.numel(), _stream()))
+    return out
+
+
+class ResStage:
+    """conv_a (cin->c) ReLU, conv_b ReLU, conv_c + skip from a, ReLU."""
+
+    def __init__(self, cin, c, gen, device):
+        self.a = Conv(cin, c, 9, gen, device)
+        self.b = Conv(c, c, 9, gen, device)
+        self.c = Conv(c, c, 9, gen, device)
+
+    def forward(self, x, shape):
+        ya = self.a.forward(x, shape)
+        yb = self.b.forward(ya, shape)
+        yc = self.c.forward(yb, shape, res=ya)
+        self.saved = (x, ya, yb, yc)
+        return yc
+
+    def backward(self, g, shape, scratch, need_dx=True):
+        x, ya, yb, yc = self.saved
+        gc = _relu_bwd(yc, g)
+        gb = self.c.backward(yb, gc, shape, scratch)
+        gb = _relu_bwd(yb, gb)
+        ga = _add(gc, self.b.backward(ya, gb, shape, scratch))
+        ga = _relu_bwd(ya, ga)
+        return self.a.backward(x, ga, shape, scratch, need_dx=need_dx)
+
+    def convs(self):
+        return [self.a, self.b, self.c]
+
+
+class UNet:
+    CH = (32, 64, 96, 128)
+
+    def __init__(self, seed=0, device="cuda:0"):
+        gen = np.random.default_rng(seed)
+        c0, c1, c2, c3 = self.CH
+        self.e = [ResStage(8, c0, gen, device), ResStage(c0, c1, gen, device),
+                  ResStage(c1, c2, gen, device), ResStage(c2, c3, gen, device)]
+        self.d = [ResStage(c1 + c0, c0, gen, device), ResStage(c2 + c1, c1, gen, device),
+                  ResStage(c3 + c2, c2, gen, device)]
+        self.head = Conv(c0, 8, 1, gen, device)
+        self.device = device
+        self.scratch = Scratch(device)
+        self.t = 0
+
+    def convs(self):
+        out = []
+        for s in self.e + self.d:
+            out += s.convs()
+        return out + [self.head]
+
+    def n_params(self):
+        return sum(c.w.numel() + c.b.numel() for c in self.convs())
+
+    def flops(self, N, H, W):
+        """multiply-adds x 2 of one forward pass"""
+        f = 0
+        c0, c1, c2, c3 = self.CH
+        sizes = [(H >> k) * (W >> k) for k in range(4)]
+        for k, s in enumerate(self.e):
+            for cv in s.convs():
+                f += 2 * cv.taps * cv.cin * cv.cout * sizes[k]
+        for k, s in enumerate(self.d):
+            for cv in s.convs():
+                f += 2 * cv.taps * cv.cin * cv.cout * sizes[k]
+        f += 2 * self.head.cin * self.head.cout * sizes[0]
+        return f * N
+
+    def forward(self, x):
+        """x: [N, H, W, 8] bf16 (H, W multiples of 8). Returns [N, H, W, 8] f32."""
+        import torch
+        N, H, W, _ = x.shape
+        self.shapes = [(N, H >> k, W >> k) for k in range(4)]
+        skips = []
+        h = x
+        for k in range(4):
+            h = self.e[k].forward(h, self.shapes[k])
+            if k < 3:
+                skips.append(h)
+                n, hh, ww = self.shapes[k]
+                p = torch.empty(n, hh // 2, ww // 2, h.shape[-1], dtype=torch.bfloat16,
+                                device=x.device)
+                _ck(nn().shy_nn_pool2(_p(h), _p(p), n, hh, ww, h.shape[-1], 0, _stream()))
+                h = p
+        self.cat = []
+        for k in (2, 1, 0):
+            n, hh, ww = self.shapes[k]
+            cu, cs = h.shape[-1], skips[k].shape[-1]
+            u = torch.empty(n, hh, ww, cu + cs, dtype=torch.bfloat16, device=x.device)
+            _ck(nn().shy_nn_up2(_p(h), _p(u), n, hh // 2, ww // 2, cu, cu + cs, 0, 0, _stream()))
+            _ck(nn().shy_nn_channels(_p(skips[k]), _p(u), n * hh * ww, cs, cu + cs, cu, 0,
+                                     _stream()))
+            self.cat.append((cu, cs))
+            h = self.d[k].forward(u, self.shapes[k])
+        self.h0 = h
+        return self.head.forward(h, self.shapes[0], relu=False, out_f32=True)
+
+    def backward(self, gy):
+        """gy: [N, H, W, 8] bf16 gradient of the output."""
+        import torch
+        g = self.head.backward(self.h0, gy, self.shapes[0], self.scratch)
+        gskip = [None, None, None]
+        for k, (cu, cs) in zip((0, 1, 2), reversed(self.cat)):
+            n, hh, ww = self.shapes[k]
+            gu = self.d[k].backward(g, self.shapes[k], self.scratch)
+            gs = torch.empty(n, hh, ww, cs, dtype=torch.bfloat16, device=gy.device)
+            _ck(nn().shy_nn_channels(_p(gu), _p(gs), n * hh * ww, cs, cu + cs, cu, 1, _stream()))
+            gskip[k] = gs
+            g = torch.empty(n, hh // 2, ww // 2, cu, dtype=torch.bfloat16, device=gy.device)
+            _ck(nn().shy_nn_up2(_p(gu), _p(g), n, hh // 2, ww // 2, cu, cu + cs, 0, 1, _stream()))
+        for k in (3, 2, 1, 0):
+            gx = self.e[k].backward(g, self.shapes[k], self.scratch, need_dx=k > 0)
+            if k == 0:
+                break
+            n, hh, ww = self.shapes[k - 1]
+            c = gx.shape[-1]
+            gp = torch.empty(n, hh, ww, c, dtype=torch.bfloat16, device=gy.device)
+            _ck(nn().shy_nn_pool2(_p(gx), _p(gp), n, hh, ww, c, 1, _stream()))
+            g = _add(gp, gskip[k - 1])
+
+    def step(self, lr, wd=0.0):
+        self.t += 1
+        for cv in self.convs():
+            cv.step(lr, self.t, wd)
+
+    def state_dict(self):
+        return {f"w{i}": c.w.detach().cpu().numpy() for i, c in enumerate(self.convs())} | \
+               {f"b{i}": c.b.detach().cpu().numpy() for i, c in enumerate(self.convs())}
+
+    def load_state_dict(self, sd):
+        import torch
+        for i, c in enumerate(self.convs()):
+            c.w.copy_(torch.from_numpy(sd[f"w{i}"]))
+            c.b.copy_(torch.from_numpy(sd[f"b{i}"]))
+            c.w_bf.copy_(c.w.to(torch.bfloat16))
+
+
+def mse_loss_and_grad(y, target, mask=None):
+    """y [N,H,W,8] f32 head output; target [N,H,W,3] f32. Returns (loss, g bf16)."""
+    import torch
+    t8 = torch.zeros_like(y)
+    t8[..., :3] = target
+    m8 = torch.zeros_like(y)
+    m8[..., :3] = 1.0 if mask is None else mask[..., None]
+    count = float(m8.sum().item())
+    g = torch.empty(y.shape, dtype=torch.bfloat16, device=y.device)
+    nblk = -(-y.numel() // 256)
+    part = torch.empty(nblk, dtype=torch.float32, device=y.device)
+    _ck(nn().shy_nn_mse(_p(y), _p(t8), _p(m8), _p(g), _p(part), y.numel(), 1.0 / count,
+                        _stream()))
+    return part.sum() / count, g
+
+
+def predict_tiled(net, h, d, mx, my, tile=256, halo=32):
+    """Apply the network tile-wise with a halo to an arbitrarily large map
+    (inputs [H, W] f32 on the device); returns decoded (d, mx, my)."""
+    import torch
+    H, W = h.shape
+    core = tile - 2 * halo
+    out = torch.zeros(H, W, 3, dtype=torch.float32, device=h.device)
+    x_full = make_input(h, d, mx, my)
+    pad = torch.zeros(H + 2 * halo + tile, W + 2 * halo + tile, 8, dtype=torch.bfloat16,
+                      device=h.device)
+    pad[halo:halo + H, halo:halo + W] = x_full
+    for y0 in range(0, H, core):
+        for x0 in range(0, W, core):
+            xt = pad[y0:y0 + tile, x0:x0 + tile].unsqueeze(0).contiguous()
+            yt = net.forward(xt)[0, halo:halo + core, halo:halo + core, :3]
+            hh, ww = min(core, H - y0), min(core, W - x0)
+            out[y0:y0 + hh, x0:x0 + ww] = yt[:hh, :ww]
+    return decode_flow(out)

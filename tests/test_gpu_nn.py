@@ -1,0 +1,184 @@
+"""GPU: the surrogate's BF16 MFMA kernels against plain FP32 references.
+
+Each HIP kernel is compared with the same operation in float32 torch on the
+CPU, on the same bf16-rounded inputs.  Tolerances are written per test: the
+kernels accumulate in FP32 and round activations to bf16 (8 significant
+bits), so single layers agree to ~1e-2 relative and a whole U-Net backward
+pass to a few percent.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def rel(a, b):
+    a = a.double()
+    b = b.double()
+    return float((a - b).norm() / (b.norm() + 1e-30))
+
+
+@pytest.fixture(scope="module")
+def sg(shy):
+    from simplehydrology_amd import surrogate
+    surrogate.nn()
+    return surrogate
+
+
+def ref_conv(x_nhwc, w_rows, b, taps, cin, cout):
+    """x [N,H,W,Cin] f32, w [Cout][taps*Cin] tap-major -> y [N,H,W,Cout]."""
+    k = 3 if taps == 9 else 1
+    w = w_rows.view(cout, k, k, cin).permute(0, 3, 1, 2)  # OIHW
+    y = torch.nn.functional.conv2d(x_nhwc.permute(0, 3, 1, 2), w, b, padding=k // 2)
+    return y.permute(0, 2, 3, 1)
+
+
+@pytest.mark.parametrize("N,H,W,cin,cout,taps", [(2, 16, 24, 8, 32, 9), (1, 32, 32, 64, 96, 9),
+                                                 (3, 8, 40, 224, 96, 9), (2, 16, 16, 32, 8, 1),
+                                                 (1, 13, 7, 16, 8, 9)])
+def test_conv_forward_dgrad_wgrad(sg, N, H, W, cin, cout, taps):
+    g = torch.Generator().manual_seed(cin * 7 + cout)
+    x = bf(torch.randn(N, H, W, cin, generator=g))
+    w = bf(torch.randn(cout, taps * cin, generator=g) * 0.1)
+    b = torch.randn(cout, generator=g) * 0.1
+    res = bf(torch.randn(N, H, W, cout, generator=g))
+    dev = "cuda:0"
+    conv = sg.Conv(cin, cout, taps, np.random.default_rng(0), dev)
+    conv.w.copy_(w.to(dev))
+    conv.w_bf.copy_(w.to(dev).to(torch.bfloat16))
+    conv.b.copy_(b.to(dev))
+    xd = x.to(dev).to(torch.bfloat16).contiguous()
+    resd = res.to(dev).to(torch.bfloat16).contiguous()
+    print("conv", N, H, W, cin, cout, taps, flush=True)
+    y = conv.forward(xd, (N, H, W), res=resd, relu=True, out_f32=True)
+    yr = torch.relu(ref_conv(x, w, b, taps, cin, cout) + res)
+    assert rel(y.cpu(), yr) < 2e-3
+    # gradients of a linear (no relu) output against autograd
+    gy = bf(torch.randn(N, H, W, cout, generator=g))
+    xr = x.clone().requires_grad_(True)
+    wr = w.clone().requires_grad_(True)
+    br = b.clone().requires_grad_(True)
+    out = ref_conv(xr, wr, br, taps, cin, cout)
+    out.backward(gy)
+    scratch = sg.Scratch(dev)
+    dx = conv.backward(xd, gy.to(dev).to(torch.bfloat16).contiguous(), (N, H, W), scratch)
+    assert rel(conv.dw.cpu(), wr.grad) < 1e-4  # FP32 accumulation of bf16 products
+    assert rel(conv.db.cpu(), br.grad) < 1e-4
+    assert rel(dx.float().cpu(), xr.grad) < 1e-2  # dx is rounded to bf16
+
+
+def test_pool_upsample_channels(sg):
+    dev = "cuda:0"
+    g = torch.Generator().manual_seed(3)
+    N, H, W, C = 2, 8, 12, 16
+    x = bf(torch.randn(N, H, W, C, generator=g))
+    xd = x.to(dev).to(torch.bfloat16)
+    p = torch.empty(N, H // 2, W // 2, C, dtype=torch.bfloat16, device=dev)
+    sg._ck(sg.nn().shy_nn_pool2(xd.data_ptr(), p.data_ptr(), N, H, W, C, 0, None))
+    pr = x.view(N, H // 2, 2, W // 2, 2, C).mean((2, 4))
+    assert rel(p.float().cpu(), pr) < 4e-3
+    gx = torch.empty_like(xd)
+    sg._ck(sg.nn().shy_nn_pool2(p.data_ptr(), gx.data_ptr(), N, H, W, C, 1, None))
+    expect = (0.25 * p.float().cpu()).repeat_interleave(2, 1).repeat_interleave(2, 2)
+    assert rel(gx.float().cpu(), expect) < 4e-3
+    u = torch.zeros(N, H * 2, W * 2, C + 8, dtype=torch.bfloat16, device=dev)
+    sg._ck(sg.nn().shy_nn_up2(xd.data_ptr(), u.data_ptr(), N, H, W, C, C + 8, 0, 0, None))
+    ur = x.repeat_interleave(2, 1).repeat_interleave(2, 2)
+    assert torch.equal(u[..., :C].float().cpu(), ur)
+    back = torch.empty_like(xd)
+    sg._ck(sg.nn().shy_nn_up2(u.data_ptr(), back.data_ptr(), N, H, W, C, C + 8, 0, 1, None))
+    assert rel(back.float().cpu(), 4 * x) < 4e-3
+    s = torch.randn(N, H * 2, W * 2, 8, generator=g).to(torch.bfloat16).to(dev)
+    sg._ck(sg.nn().shy_nn_channels(s.data_ptr(), u.data_ptr(), N * H * W * 4, 8, C + 8, C, 0, None))
+    assert torch.equal(u[..., C:].cpu(), s.cpu())
+
+
+def test_adam_closed_form(sg):
+    dev = "cuda:0"
+    w = torch.tensor([1.0, -2.0, 0.5], device=dev)
+    gr = torch.tensor([0.1, -0.3, 0.0], device=dev)
+    m = torch.zeros(3, device=dev)
+    v = torch.zeros(3, device=dev)
+    sg._ck(sg.nn().shy_nn_adam(w.data_ptr(), gr.data_ptr(), m.data_ptr(), v.data_ptr(), None, 3,
+                               0.01, 0.9, 0.999, 1e-8, 1, 0.0, None))
+    # first Adam step moves each weight by lr * sign(g) (bias-corrected m/sqrt(v))
+    exp = torch.tensor([1.0 - 0.01, -2.0 + 0.01, 0.5])
+    assert torch.allclose(w.cpu(), exp, atol=1e-6)
+
+
+def torch_unet(net, x):
+    """The same U-Net in float32 torch on the CPU, sharing net's weights."""
+    params = []
+
+    def conv(cv, h, res=None, relu=True):
+        w = cv.w.detach().cpu().clone().requires_grad_(True)
+        b = cv.b.detach().cpu().clone().requires_grad_(True)
+        params.append((cv, w, b))
+        y = ref_conv(h, w, b, cv.taps, cv.cin, cv.cout)
+        if res is not None:
+            y = y + res
+        return torch.relu(y) if relu else y
+
+    def stage(s, h):
+        a = conv(s.a, h)
+        b = conv(s.b, a)
+        return conv(s.c, b, res=a)
+
+    skips = []
+    h = x
+    for k in range(4):
+        h = stage(net.e[k], h)
+        if k < 3:
+            skips.append(h)
+            n, hh, ww, c = h.shape
+            h = h.view(n, hh // 2, 2, ww // 2, 2, c).mean((2, 4))
+    for k in (2, 1, 0):
+        u = h.repeat_interleave(2, 1).repeat_interleave(2, 2)
+        h = stage(net.d[k], torch.cat([u, skips[k]], -1))
+    return conv(net.head, h, relu=False), params
+
+
+def test_unet_gradients_match_fp32_reference(sg):
+    dev = "cuda:0"
+    net = sg.UNet(seed=4, device=dev)
+    g = torch.Generator().manual_seed(9)
+    N, H, W = 2, 32, 32
+    x = bf(torch.randn(N, H, W, 8, generator=g) * 0.5)
+    t = torch.randn(N, H, W, 3, generator=g) * 0.3
+    print("unet forward", flush=True)
+    y = net.forward(x.to(dev).to(torch.bfloat16).contiguous())
+    loss, gy = sg.mse_loss_and_grad(y, t.to(dev))
+    net.backward(gy)
+    yr, params = torch_unet(net, x)
+    lr = ((yr[..., :3] - t) ** 2).mean()
+    lr.backward()
+    assert rel(y.cpu()[..., :3], yr.detach()[..., :3]) < 3e-2
+    assert abs(loss.item() - lr.item()) < 3e-2 * lr.item()
+    worst = 0.0
+    for cv, w, b in params:
+        worst = max(worst, rel(cv.dw.cpu(), w.grad), rel(cv.db.cpu(), b.grad))
+    print("worst relative gradient error", worst, flush=True)
+    assert worst < 0.08
+
+
+def test_unet_training_reduces_loss(sg):
+    """A few Adam steps on one fixed batch must cut the loss (end-to-end wiring)."""
+    dev = "cuda:0"
+    net = sg.UNet(seed=1, device=dev)
+    g = torch.Generator().manual_seed(2)
+    x = (torch.randn(4, 32, 32, 8, generator=g) * 0.5).to(dev).to(torch.bfloat16)
+    t = torch.tanh(x[..., :3].float() * 2.0)
+    losses = []
+    for _ in range(30):
+        y = net.forward(x)
+        loss, gy = sg.mse_loss_and_grad(y, t)
+        net.backward(gy)
+        net.step(3e-3)
+        losses.append(loss.item())
+    assert losses[-1] < 0.3 * losses[0], losses
