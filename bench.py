@@ -7,14 +7,20 @@ parameters, 65536 drops per cycle.  A "step" of this script is one cycle =
 one World::erode(65536) call (walk + cascade/blend).  The reported value is
 particle-steps/s over exactly --steps timed cycles, inputs resident in HBM.
 
-Prints one JSON line.  Roofline: the commit pass of the drop walk (the
-dominant kernel), HBM-bound, algorithmic bytes per particle-step
-B_COMMIT = 69 (reads: 5 heights, the float4 {mx, my, d, erf(0.4 d)}, the
-burst scale = 40 B; commits: int32 height, 3 x int64 tracks, 1 visit byte =
-29 B).  The count pass reads the same 36 B of map and commits one int32
-count: B_COUNT = 40.  CPU baseline: the oracle's
-sequential restatement of World::erode on the same terrain and spawn stream,
-one core, on a bounded sample of cycles.
+Prints one JSON line.  Roofline: the kernel with the largest measured time
+per cycle, against the HBM peak, with its algorithmic bytes per
+particle-step:
+  count walk (walk_kernel)  B_COUNT = 61: reads 5 heights and the float4
+      {mx, my, d, erf(0.4 d)} of the step's cell = 36 B; ticket atomic 4 B;
+      ticket log 4 B; step log 16 B; cascade-site byte 1 B
+  commit replay (replay_kernel)  B_REPLAY = 44: reads step log 16 B, ticket
+      4 B, bucket offset 4 B, burst scale 4 B; writes the 16-B bucket record
+``traffic`` is the HBM bytes per launch of that kernel measured by rocprofv3
+PMC runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy-kernel calibration) on the
+same command, read from profiles/hbm_traffic.json when it holds an entry for
+this workload, else null.  CPU baseline: the oracle's sequential restatement
+of World::erode on the same terrain and spawn stream, one core, on a bounded
+sample of cycles.
 """
 import argparse
 import json
@@ -26,9 +32,32 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-B_COMMIT = 69         # algorithmic bytes per particle-step, commit walk
-B_COUNT = 40          # algorithmic bytes per particle-step, count walk
+B_COUNT = 61          # algorithmic bytes per particle-step, count walk
+B_REPLAY = 44         # algorithmic bytes per particle-step, commit replay
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+KERNELS = (("walk_kernel", "ms_count_walk", B_COUNT), ("replay_kernel", "ms_replay", B_REPLAY))
+
+
+def roofline(tm: dict, cycles: int, steps: int, workload: str) -> dict:
+    """Roofline of the dominant kernel (largest summed time) and the
+    achieved bandwidth of each walk kernel."""
+    per = {}
+    for name, key, b in KERNELS:
+        avg_s = tm[key] / 1e3 / cycles
+        per[name] = {"ms_avg": avg_s * 1e3, "bytes_per_step": b,
+                     "achieved": steps / cycles * b / avg_s / 1e9}
+    dom = max(per, key=lambda k: per[k]["ms_avg"])
+    d = per[dom]
+    traffic = None
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            entry = json.load(f).get(workload, {}).get(dom)
+        if entry:
+            traffic = entry["bytes_per_launch"]
+    return {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": d["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+            "bytes_per_step": d["bytes_per_step"], "kernels": per}
 
 
 def cpu_model() -> str:
@@ -100,8 +129,6 @@ def run_ensemble(shy, args) -> dict:
     sec = tm["ms_total"] / 1e3
     cells = sum(s["nx"] * s["ny"] for s in specs)
     drops = sum(s["drops"] for s in specs)
-    steps_per_launch = steps / args.steps
-    achieved = steps_per_launch * B_COMMIT / (tm["ms_commit_walk"] / 1e3 / args.steps) / 1e9
     e.close()
     return {
         "metric": "particle-steps/s, ensemble of uneven terrains",
@@ -115,9 +142,7 @@ def run_ensemble(shy, args) -> dict:
                    "tuned_ms_per_cycle": tuned_ms},
         "cycles_per_s": args.steps / sec, "particle_steps": steps,
         "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
-        "roofline": {"bound": "hbm", "kernel": "walk_kernel<kCommit>", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_step": B_COMMIT},
+        "roofline": roofline(tm, args.steps, steps, "ensemble64"),
         "cpu_baseline": None,
     }
 
@@ -181,10 +206,6 @@ def main():
     steps = s1["steps"] - s0["steps"]
     sec = tm["ms_total"] / 1e3
     value = steps / sec
-    walk_avg_s = tm["ms_commit_walk"] / 1e3 / args.steps
-    steps_per_launch = steps / args.steps
-    achieved = steps_per_launch * B_COMMIT / walk_avg_s / 1e9
-    count_gbs = steps_per_launch * B_COUNT / (tm["ms_count_walk"] / 1e3 / args.steps) / 1e9
     out = {
         "metric": f"particle-steps/s at {n}x{n}",
         "value": value,
@@ -206,10 +227,7 @@ def main():
         "particle_steps": steps,
         "host_wall_s": wall,
         "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
-        "roofline": {"bound": "hbm", "kernel": "walk_kernel<kCommit>", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_step": B_COMMIT,
-                     "count_walk_achieved": count_gbs},
+        "roofline": roofline(tm, args.steps, steps, f"single{n}_drops{drops}"),
     }
     w.close()
     if not args.no_cpu:

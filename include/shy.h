@@ -149,6 +149,7 @@ int shy_terrain_upload_field(shy_ctx* ctx, int terrain, int field, const float* 
 int shy_terrain_download_field(shy_ctx* ctx, int terrain, int field, float* dst);
 int shy_terrain_generate_fbm(shy_ctx* ctx, int terrain, uint64_t seed, int octaves,
                              float base_freq, float lacunarity, float gain);
+int shy_terrain_generate_reference(shy_ctx* ctx, int terrain, int seed, int tileres);
 /* Device-to-device field transfer (no reference counterpart; for the
    surrogate): export writes terrain t's (height, discharge, momentumx,
    momentumy) as 4 floats per cell, x-major, into a caller-owned device
@@ -185,6 +186,13 @@ int shy_download_normals(shy_ctx* ctx, float* xyz);
    (source/cellpool.h:349-409).  base_freq <= 0 means 1/max(nx, ny). */
 int shy_generate_fbm(shy_ctx* ctx, uint64_t seed, int octaves, float base_freq,
                      float lacunarity, float gain);
+
+/* quad::map::init's own terrain (source/cellpool.h:349-409): eight layers of
+   FastNoiseLite OpenSimplex2 3D FBm noise (frequencies 1..128, weights
+   0.6^(k+1)) sampled at (x/tileres, y/tileres, seed % 10000), normalised as
+   (h - min)/(max - min) with min <= 0 <= max.  The reference's world is
+   512x512 with tileres 512 (cellpool.h:167-176).  Writes heights only. */
+int shy_generate_reference(shy_ctx* ctx, int seed, int tileres);
 
 /* World::erode(cycles) (source/world.h:54-88): one cycle of `drops` spawns.
    Wave-synchronous: every drop reads the map as it stood at the start of the
@@ -236,7 +244,7 @@ int shy_autotune(shy_ctx* ctx, int cycles, shy_tuning* chosen, float* best_ms);
    HIP events on the context's stream between kernels.  drops >= 0 sets the
    drops of a single-terrain context first; drops < 0 keeps every terrain's
    own.  ms[0] = wall time of the whole sequence; ms[1..4] = summed durations
-   of the count walk, the scan, the commit walk and the reduce + cascade/blend
+   of the count walk, the scan, the commit replay and the reduce + cascade/blend
    stencil, in milliseconds. */
 int shy_time_cycles(shy_ctx* ctx, int cycles, int drops, float ms[5]);
 

@@ -6,7 +6,7 @@
  * header and keeps its loop (SimpleHydrology.cpp:313-325):
  *
  *   shy::World::SEED = 42;
- *   shy::World::map.init(512, 512, shy::World::SEED);   // fBm terrain on the GPU
+ *   shy::World::map.init(512, 512, shy::World::SEED);   // the reference's terrain, on the GPU
  *   for (;;) { shy::World::erode(512); shy::World::map.sync(); ... }
  *
  * The static parameters keep the reference's names and defaults; each call
@@ -64,7 +64,11 @@ struct map {
   std::vector<cell> cells;
   std::vector<float> erfd, normals;
 
-  void init(int sx, int sy, unsigned int seed);  // quad::map::init (cellpool.h:323-411)
+  // quad::map::init (cellpool.h:323-411): the OpenSimplex2 terrain at
+  // tileres 512 (cellpool.h:167-169), seed passed as the reference's int SEED
+  void init(int sx, int sy, unsigned int seed);
+  // the seeded Perlin fBm of the benchmark workloads instead
+  void init_fbm(int sx, int sy, unsigned int seed);
   void release() {
     if (ctx) shy_destroy(ctx);
     ctx = nullptr;
@@ -158,6 +162,16 @@ inline void World::push_params() {
 }
 
 inline void map::init(int sx, int sy, unsigned int seed) {
+  release();
+  nx = sx;
+  ny = sy;
+  const shy_params p = World::params();
+  check(nullptr, shy_create(&p, nx, ny, seed, &ctx));
+  check(ctx, shy_generate_reference(ctx, (int)seed, 512));
+  sync();
+}
+
+inline void map::init_fbm(int sx, int sy, unsigned int seed) {
   release();
   nx = sx;
   ny = sy;

@@ -100,6 +100,13 @@ def lib():
         L.orc_fbm.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                               C.c_float, fp]
         L.orc_fbm_perm.argtypes = [C.c_uint64, C.c_int, u8p]
+        L.orc_os2_single3.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float]
+        L.orc_os2_single3.restype = C.c_float
+        L.orc_fnl_noise3.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float]
+        L.orc_fnl_noise3.restype = C.c_float
+        L.orc_fnl_world.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, fp]
+        L.orc_os2_edge.argtypes = [C.c_int]
+        L.orc_os2_edge.restype = C.c_int
         L.orc_normal.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
         L.orc_seq_erode.argtypes = [fp, C.c_int, C.c_int, P, C.c_uint64, C.c_uint32,
                                     C.c_uint32, C.c_int, C.POINTER(SeqLedger)]
@@ -154,6 +161,37 @@ def fbm(seed: int, nx: int, ny: int, octaves: int = 8, base_freq: float | None =
     out = np.empty(nx * ny, np.float32)
     lib().orc_fbm(seed, nx, ny, octaves, float(base_freq), lacunarity, gain, out)
     return out.reshape(nx, ny)
+
+
+def fnl_noise3(freq: float, x: float, y: float, z: float) -> float:
+    """GetNoise(x, y, z) of the OpenSimplex2 FBm that quad::map::init uses."""
+    return lib().orc_fnl_noise3(freq, x, y, z)
+
+
+def fnl_world(nx: int, ny: int, seed: int, tileres: int = 512) -> np.ndarray:
+    """quad::map::init's normalised terrain (cellpool.h:349-409)."""
+    out = np.empty(nx * ny, np.float32)
+    lib().orc_fnl_world(nx, ny, seed, tileres, out)
+    return out.reshape(nx, ny)
+
+
+# --------------------------------------------- reference build (oracle/_ref)
+_REF_FNL = os.path.join(_HERE, "_ref", "libfnl_ref.so")
+_ref_fnl = None
+
+
+def ref_fnl():
+    """The reference's own FastNoiseLite compiled by ``make -C oracle ref``, or
+    None when it has not been built (it needs the reference sources)."""
+    global _ref_fnl
+    if _ref_fnl is None and os.path.exists(_REF_FNL):
+        L = C.CDLL(_REF_FNL)
+        L.fnl_ref_noise3.argtypes = [C.c_float] * 4
+        L.fnl_ref_noise3.restype = C.c_float
+        L.fnl_ref_world.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
+                                    np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
+        _ref_fnl = L
+    return _ref_fnl
 
 
 def fbm_perm(seed: int, octave: int) -> np.ndarray:

@@ -169,6 +169,140 @@ void orc_fbm(uint64_t seed, int nx, int ny, int octaves, float base_freq,
     out[i] = range > 0.0f ? (out[i] - mn) / range : 0.0f;
 }
 
+/* --------------------------------------------- the reference's terrain */
+/* quad::map::init (cellpool.h:349-409) drives FastNoiseLite, a third-party
+   header vendored in the reference (source/include/FastNoiseLite.h, MIT, no
+   version number in the file).  Restated here: GetNoise(x, y, z)
+   (FastNoiseLite.h:321-339) of a FastNoiseLite(1337) with NoiseType
+   OpenSimplex2 and FractalType FBm at its defaults (3 octaves, lacunarity 2,
+   gain 0.5, weighted strength 0, bounding 1/1.75; :114-129), which is the
+   DefaultOpenSimplex2 rotation (:687-727), the FBm sum (:866-885) and
+   SingleOpenSimplex2 (:1054-1153).  Signed overflow in the hash wraps mod
+   2^32 (what the reference's compiled code does). */
+#define OS_PX 501125321
+#define OS_PY 1136930381
+#define OS_PZ 1720413743
+
+static int32_t os_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+static int32_t os_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+/* FastNoiseLite.h:2529-2547: entries 0..59 run through the 12 cube edges
+   five times; 60..63 repeat edges 8, 1, 9, 3 */
+static const float OS_EDGE[12][3] = {
+    {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {1, 0, 1}, {-1, 0, 1},
+    {1, 0, -1}, {-1, 0, -1}, {1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}};
+
+int orc_os2_edge(int entry) {
+  static const int tail[4] = {8, 1, 9, 3};
+  return entry < 60 ? entry % 12 : tail[entry - 60];
+}
+
+static float os_grad(int32_t seed, int32_t i, int32_t j, int32_t k, float x, float y, float z) {
+  int32_t h = os_mul(seed ^ i ^ j ^ k, 0x27d4eb2d); /* :500-506 */
+  h ^= h >> 15;                                     /* :545 */
+  const float* g = OS_EDGE[orc_os2_edge((h & 252) >> 2)];
+  return x * g[0] + y * g[1] + z * g[2];
+}
+
+static int os_round(float f) { return f >= 0 ? (int)(f + 0.5f) : (int)(f - 0.5f); } /* :453 */
+
+float orc_os2_single3(int32_t seed, float x, float y, float z) {
+  int32_t i = os_round(x), j = os_round(y), k = os_round(z);
+  float x0 = x - (float)i, y0 = y - (float)j, z0 = z - (float)k;
+  int xs = (int)(-1.0f - x0) | 1, ys = (int)(-1.0f - y0) | 1, zs = (int)(-1.0f - z0) | 1;
+  float ax0 = (float)xs * -x0, ay0 = (float)ys * -y0, az0 = (float)zs * -z0;
+  i = os_mul(i, OS_PX);
+  j = os_mul(j, OS_PY);
+  k = os_mul(k, OS_PZ);
+  float value = 0.0f;
+  float a = (0.6f - x0 * x0) - (y0 * y0 + z0 * z0);
+  for (int l = 0; l < 2; ++l) {
+    if (a > 0) value += (a * a) * (a * a) * os_grad(seed, i, j, k, x0, y0, z0);
+    float b = a + 1.0f;
+    int32_t i1 = i, j1 = j, k1 = k;
+    float x1 = x0, y1 = y0, z1 = z0;
+    if (ax0 >= ay0 && ax0 >= az0) {
+      x1 += (float)xs;
+      b -= (float)(xs * 2) * x1;
+      i1 = os_add(i1, -os_mul(xs, OS_PX));
+    } else if (ay0 > ax0 && ay0 >= az0) {
+      y1 += (float)ys;
+      b -= (float)(ys * 2) * y1;
+      j1 = os_add(j1, -os_mul(ys, OS_PY));
+    } else {
+      z1 += (float)zs;
+      b -= (float)(zs * 2) * z1;
+      k1 = os_add(k1, -os_mul(zs, OS_PZ));
+    }
+    if (b > 0) value += (b * b) * (b * b) * os_grad(seed, i1, j1, k1, x1, y1, z1);
+    if (l == 1) break;
+    ax0 = 0.5f - ax0;
+    ay0 = 0.5f - ay0;
+    az0 = 0.5f - az0;
+    x0 = (float)xs * ax0;
+    y0 = (float)ys * ay0;
+    z0 = (float)zs * az0;
+    a += (0.75f - ax0) - (ay0 + az0);
+    i = os_add(i, (xs >> 1) & OS_PX);
+    j = os_add(j, (ys >> 1) & OS_PY);
+    k = os_add(k, (zs >> 1) & OS_PZ);
+    xs = -xs;
+    ys = -ys;
+    zs = -zs;
+    seed = ~seed;
+  }
+  return value * 32.69428253173828125f;
+}
+
+float orc_fnl_noise3(float freq, float x, float y, float z) {
+  x *= freq;
+  y *= freq;
+  z *= freq;
+  const float r3 = (float)(2.0 / 3.0);
+  const float r = (x + y + z) * r3;
+  x = r - x;
+  y = r - y;
+  z = r - z;
+  int32_t seed = 1337;
+  float sum = 0.0f, amp = 1 / 1.75f;
+  for (int o = 0; o < 3; ++o) {
+    const float noise = orc_os2_single3(seed++, x, y, z);
+    sum += noise * amp;
+    const float t = 0.0f, lo = 1.0f, hi = (noise + 1) * 0.5f; /* Lerp(1, hi, 0) */
+    amp *= lo + t * (hi - lo);
+    x *= 2.0f;
+    y *= 2.0f;
+    z *= 2.0f;
+    amp *= 0.5f;
+  }
+  return sum;
+}
+
+void orc_fnl_world(int nx, int ny, int seed, int tileres, float* out) {
+  /* cellpool.h:360-381: eight layers, frequency x2 and weight x0.6 (in
+     double, stored to float) per layer */
+  const float pz = (float)(seed % 10000);
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      const float px = (float)x / (float)tileres, py = (float)y / (float)tileres;
+      float h = 0.0f, freq = 1.0f, scale = 0.6f;
+      for (int o = 0; o < 8; ++o) {
+        h += scale * orc_fnl_noise3(freq, px, py, pz);
+        freq *= 2;
+        scale = (float)(scale * 0.6);
+      }
+      out[(size_t)x * ny + y] = h;
+    }
+  /* cellpool.h:385-404: min and max start at 0 */
+  float mn = 0.0f, mx = 0.0f;
+  const size_t n = (size_t)nx * ny;
+  for (size_t i = 0; i < n; ++i) {
+    mn = mn < out[i] ? mn : out[i];
+    mx = mx > out[i] ? mx : out[i];
+  }
+  for (size_t i = 0; i < n; ++i) out[i] = (out[i] - mn) / (mx - mn);
+}
+
 /* ------------------------------------------------------------ accessors */
 static inline int oob(int nx, int ny, int x, int y) {
   /* quad::map::oob, cellpool.h:413-419 */

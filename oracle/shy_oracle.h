@@ -87,6 +87,14 @@ void orc_fbm(uint64_t seed, int nx, int ny, int octaves, float base_freq,
              float lacunarity, float gain, float* out);
 void orc_fbm_perm(uint64_t seed, int octave, uint8_t* perm256);
 
+/* quad::map::init's terrain (cellpool.h:349-409) over FastNoiseLite's
+   OpenSimplex2: one 3D noise sample, one GetNoise of the 3-octave FBm, and
+   the whole normalised world; the gradient-set entry -> cube edge map */
+float orc_os2_single3(int32_t seed, float x, float y, float z);
+float orc_fnl_noise3(float freq, float x, float y, float z);
+void orc_fnl_world(int nx, int ny, int seed, int tileres, float* out);
+int orc_os2_edge(int entry);
+
 /* normal of reference _normal (cellpool.h:181-204); h x-major floats */
 void orc_normal(const float* h, int nx, int ny, int x, int y, float* n3);
 

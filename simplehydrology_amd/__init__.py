@@ -117,6 +117,8 @@ def lib():
             "shy_upload_field": (i32, [vp, i32, vp]),
             "shy_download_field": (i32, [vp, i32, vp]),
             "shy_generate_fbm": (i32, [vp, u64, i32, C.c_float, C.c_float, C.c_float]),
+            "shy_generate_reference": (i32, [vp, i32, i32]),
+            "shy_terrain_generate_reference": (i32, [vp, i32, i32, i32]),
             "shy_erode": (i32, [vp, i32]),
             "shy_erode_cycles": (i32, [vp, i32, i32, i32]),
             "shy_trace_drop": (i32, [vp, C.c_float, C.c_float, i32, vp, C.POINTER(i32)]),
@@ -158,6 +160,15 @@ def lib():
 
 def device_count() -> int:
     return lib().shy_device_count()
+
+
+def _torch_sync():
+    """Device buffers handed to export/import may be produced or consumed by
+    work queued on torch's stream; the library orders only its own stream."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.synchronize()
 
 
 def _check(ctx, rc: int):
@@ -285,6 +296,10 @@ class World:
         _check(self._ctx, lib().shy_generate_fbm(self._ctx, seed, octaves, base_freq,
                                                   lacunarity, gain))
 
+    def generate_reference(self, seed: int, tileres: int = 512):
+        """``quad::map::init``'s OpenSimplex2 terrain (cellpool.h:349-409)."""
+        _check(self._ctx, lib().shy_generate_reference(self._ctx, int(seed), int(tileres)))
+
     # -- the hot path ----------------------------------------------------------
     def erode(self, cycles: int):
         """``World::erode(cycles)``: one cycle of ``cycles`` drop spawns."""
@@ -339,10 +354,12 @@ class World:
 
     def export(self, dst_ptr: int):
         """(h, d, mx, my) into a device buffer (4 floats per cell)."""
+        _torch_sync()
         _check(self._ctx, lib().shy_terrain_export(self._ctx, 0, dst_ptr))
 
     def import_flow(self, src_ptr: int):
         """(d, mx, my) from a device buffer (3 floats per cell)."""
+        _torch_sync()
         _check(self._ctx, lib().shy_terrain_import_flow(self._ctx, 0, src_ptr))
 
     def time_cycles(self, cycles: int, drops: int) -> dict:
@@ -351,7 +368,7 @@ class World:
         ms = (C.c_float * 5)()
         _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, drops, ms))
         return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scan": ms[2],
-                "ms_commit_walk": ms[3], "ms_reduce_cascade": ms[4]}
+                "ms_replay": ms[3], "ms_reduce_cascade": ms[4]}
 
 
 class Drop:
@@ -404,6 +421,10 @@ class Ensemble(World):
         _check(self._ctx, lib().shy_terrain_generate_fbm(self._ctx, terrain, seed, octaves,
                                                           base_freq, lacunarity, gain))
 
+    def generate_reference(self, terrain: int, seed: int, tileres: int = 512):
+        _check(self._ctx, lib().shy_terrain_generate_reference(self._ctx, terrain, int(seed),
+                                                                int(tileres)))
+
     def field(self, terrain: int, field: str) -> np.ndarray:
         nx, ny = self.sizes[terrain]
         out = np.empty((nx, ny), np.float32)
@@ -421,10 +442,12 @@ class Ensemble(World):
 
     def export(self, terrain: int, dst_ptr: int):
         """(h, d, mx, my) of one terrain into a device buffer (4 floats/cell)."""
+        _torch_sync()
         _check(self._ctx, lib().shy_terrain_export(self._ctx, terrain, dst_ptr))
 
     def import_flow(self, terrain: int, src_ptr: int):
         """(d, mx, my) of one terrain from a device buffer (3 floats/cell)."""
+        _torch_sync()
         _check(self._ctx, lib().shy_terrain_import_flow(self._ctx, terrain, src_ptr))
 
     def terrain_steps(self) -> np.ndarray:
@@ -436,4 +459,4 @@ class Ensemble(World):
         ms = (C.c_float * 5)()
         _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, -1, ms))
         return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scan": ms[2],
-                "ms_commit_walk": ms[3], "ms_reduce_cascade": ms[4]}
+                "ms_replay": ms[3], "ms_reduce_cascade": ms[4]}

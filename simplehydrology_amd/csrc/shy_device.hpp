@@ -98,7 +98,6 @@ struct CellView {
   float h, xm, xp, ym, yp;  // h(x,y), h(x-1,y), h(x+1,y), h(x,y-1), h(x,y+1)
   float mx, my, d, erfd;    // momentum, discharge, erf(0.4 d)
   float root;               // rootdensity
-  float scale;              // burst scale of the cell (commit pass)
 };
 
 // quad::_normal (cellpool.h:181-204) from a loaded stencil.  Plane k is

@@ -57,7 +57,10 @@ struct shy_ctx {
   int32_t* bsum = nullptr;
   int32_t* ticket = nullptr;
   int4* rec = nullptr;
-  size_t wave_cap = 0;  // steps the ticket log and buckets can hold
+  int4* slog = nullptr;  // [step][drop] step log of the count pass
+  size_t wave_cap = 0;   // steps the ticket log, step log and buckets can hold
+  int4* dlog = nullptr;  // [drop] how each walk ended
+  size_t dlog_cap = 0;
   long long* tsteps = nullptr;
   DeviceState* state = nullptr;
   double* erf_a = nullptr;
@@ -80,6 +83,7 @@ constexpr float kMaxAgeLimit = 1.0e6f;
 constexpr int kTileY[3] = {32, 64, 16};
 constexpr int kTileX[3] = {8, 4, 16};
 constexpr int kWaveDrops = 64;  // drop ranges are padded to whole wave64s
+constexpr int kReplayBlock = 256;
 
 std::string g_noctx_err;
 
@@ -176,22 +180,35 @@ size_t wave_steps(const shy_ctx* c) {
   return (size_t)c->total_drops * ((size_t)age + 1);
 }
 
-// Size the ticket log and buckets for the current drops (never inside a capture).
+// Size the logs and buckets for the current drops (never inside a capture).
 int ensure_wave(shy_ctx* c) {
   const size_t need = wave_steps(c);
+  const size_t need_d = (size_t)std::max(c->total_drops, 1);
   if (need > (size_t)INT32_MAX)
     return fail(c, SHY_ERR_ARG, "drops x (maxAge+1) exceeds 2^31 steps per wave");
-  if (need <= c->wave_cap) return SHY_OK;
+  if (need <= c->wave_cap && need_d <= c->dlog_cap) return SHY_OK;
   drop_graph(c);  // a captured graph holds the old buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (c->ticket) hipFree(c->ticket);
-  if (c->rec) hipFree(c->rec);
-  c->ticket = nullptr;
-  c->rec = nullptr;
-  c->wave_cap = 0;
-  HIPCHK(c, hipMalloc(&c->ticket, need * sizeof(int32_t)));
-  HIPCHK(c, hipMalloc(&c->rec, need * sizeof(int4)));
-  c->wave_cap = need;
+  if (need > c->wave_cap) {
+    if (c->ticket) hipFree(c->ticket);
+    if (c->rec) hipFree(c->rec);
+    if (c->slog) hipFree(c->slog);
+    c->ticket = nullptr;
+    c->rec = nullptr;
+    c->slog = nullptr;
+    c->wave_cap = 0;
+    HIPCHK(c, hipMalloc(&c->ticket, need * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc(&c->rec, need * sizeof(int4)));
+    HIPCHK(c, hipMalloc(&c->slog, need * sizeof(int4)));
+    c->wave_cap = need;
+  }
+  if (need_d > c->dlog_cap) {
+    if (c->dlog) hipFree(c->dlog);
+    c->dlog = nullptr;
+    c->dlog_cap = 0;
+    HIPCHK(c, hipMalloc(&c->dlog, need_d * sizeof(int4)));
+    c->dlog_cap = need_d;
+  }
   return SHY_OK;
 }
 
@@ -208,6 +225,8 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.offset = c->offset;
   w.scale = nullptr;
   w.rec = c->rec;
+  w.slog = c->slog;
+  w.dlog = c->dlog;
   w.deposit = c->deposit;
   w.visit = c->visit[p];
   w.state = c->state;
@@ -238,7 +257,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   const int nter = (int)c->ter.size();
   const bool walk = c->total_drops > 0;
   if (walk) {
-    walk_kernel<kCount><<<grid, bs, 0, c->stream>>>(w);
+    walk_kernel<<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
@@ -252,7 +271,8 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   if (walk) {
     w.scale = c->scale;
-    walk_kernel<kCommit><<<grid, bs, 0, c->stream>>>(w);
+    replay_kernel<<<(c->total_drops + kReplayBlock - 1) / kReplayBlock, kReplayBlock, 0,
+                    c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
@@ -425,7 +445,7 @@ int shy_destroy(shy_ctx* c) {
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
                   c->offset, c->scale, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
-                  c->tsteps, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
+                  c->slog, c->dlog, c->tsteps, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -814,12 +834,38 @@ int shy_terrain_generate_fbm(shy_ctx* c, int t, uint64_t seed, int octaves, floa
   HIPCHK(c, hipGetLastError());
   minmax_partial_kernel<<<kPartBlocks, 256, 0, c->stream>>>(out, n, c->part);
   HIPCHK(c, hipGetLastError());
-  minmax_fold_kernel<<<1, 256, 0, c->stream>>>(c->part, kPartBlocks);
+  minmax_fold_kernel<<<1, 256, 0, c->stream>>>(c->part, kPartBlocks, 0);
   HIPCHK(c, hipGetLastError());
   normalize_kernel<<<grid, bs, 0, c->stream>>>(out, n, c->part);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SHY_OK;
+}
+
+int shy_terrain_generate_reference(shy_ctx* c, int t, int seed, int tileres) {
+  RETURN_IF(check_terrain(c, t));
+  if (tileres < 1) return fail(c, SHY_ERR_ARG, "shy_generate_reference: tileres must be >= 1");
+  const Terrain& T = c->ter[t];
+  hipSetDevice(c->device);
+  const size_t n = (size_t)T.nx * T.ny;
+  float* out = c->h[parity(c)] + T.cell_off;
+  const int bs = 256;
+  const size_t grid = (n + bs - 1) / bs;
+  fnl_world_kernel<<<grid, bs, 0, c->stream>>>(out, T.nx, T.ny, seed, (float)tileres);
+  HIPCHK(c, hipGetLastError());
+  minmax_partial_kernel<<<kPartBlocks, 256, 0, c->stream>>>(out, n, c->part);
+  HIPCHK(c, hipGetLastError());
+  minmax_fold_kernel<<<1, 256, 0, c->stream>>>(c->part, kPartBlocks, 1);
+  HIPCHK(c, hipGetLastError());
+  normalize_kernel<<<grid, bs, 0, c->stream>>>(out, n, c->part);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SHY_OK;
+}
+
+int shy_generate_reference(shy_ctx* c, int seed, int tileres) {
+  RETURN_IF(need_single(c, "shy_generate_reference"));
+  return shy_terrain_generate_reference(c, 0, seed, tileres);
 }
 
 int shy_generate_fbm(shy_ctx* c, uint64_t seed, int octaves, float base_freq, float lacunarity,

@@ -5,7 +5,11 @@
 //
 //  walk_kernel<kCount>   one drop per lane walks its trajectory; for every
 //                        step it takes a ticket (relaxed int32 atomic with
-//                        return) on the step's start cell and logs it.
+//                        return) on the step's start cell and logs, coalesced
+//                        as [step][drop], the ticket and the step's
+//                        snapshot-only quantities {start cell, c_eq, tracks};
+//                        it marks the cascade sites (water.h:151) and logs
+//                        how each drop ended.
 //  scan_*_kernel         exclusive prefix sum of the per-cell step counts
 //                        (bucket offsets), fused with the per-cell burst scale
 //                        s = (1-(1-gamma)^k)/(gamma k), gamma = min(1, effD (1 +
@@ -14,13 +18,16 @@
 //                        sequence would (water.h:127-132 as a linear feedback),
 //                        which keeps the wave stable where crowded cells would
 //                        overshoot.
-//  walk_kernel<kCommit>  the same trajectories again; each step's commit --
-//                        erosion/deposition effD*cdiff*s quantised once to
-//                        int32 (2^-20, moved between map and drop as that
-//                        integer) and the discharge/momentum tracks (int32,
-//                        2^-24) -- goes with one plain 16-byte store into the
-//                        start cell's bucket at its ticket: no read-modify-write
-//                        of the map per step.
+//  replay_kernel         replays each drop's logged steps: the only chain left
+//                        is the sediment recurrence (arithmetic); the log loads
+//                        stream and the per-cell gathers (bucket offset, burst
+//                        scale) do not depend on the previous step, so they are
+//                        issued ahead.  Each step's commit -- erosion/deposition
+//                        effD*cdiff*s quantised once to int32 (2^-20, moved
+//                        between map and drop as that integer) and the
+//                        discharge/momentum tracks (int32, 2^-24) -- goes with
+//                        one plain 16-byte store into the start cell's bucket
+//                        at its ticket: no read-modify-write of the map per step.
 //  reduce_kernel         per cell, sums its bucket in integers (so the result
 //                        does not depend on the order drops ran) plus the
 //                        terminal deposits.
@@ -75,11 +82,16 @@ struct WaveArgs {
   const int32_t* __restrict__ offset;
   const float* __restrict__ scale;
   int4* rec;
+  int4* slog;         // [step][drop] {start cell, c_eq bits, qt(vol sx), qt(vol sy)}
+  int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* deposit;
   uint8_t* visit;
   DeviceState* state;
   long long* tsteps;  // particle-steps per terrain
 };
+
+// how a drop's walk ended (dlog.y)
+enum DropEnd { kEndAge = 0, kEndVol = 1, kEndExit = 2, kEndSkipped = 3, kEndPad = 4 };
 
 // largest k in [0, nter) with key(k) <= v (keys ascending, key(0) <= v)
 template <class Key>
@@ -99,10 +111,7 @@ struct WalkArgs {
   const float* __restrict__ root;  // nullptr when the map has no root density
   int32_t* count;                  // steps starting at each cell (tickets)
   int32_t* ticket;                 // [step][drop] ticket log
-  const int32_t* __restrict__ offset;  // bucket start per cell
-  const float* __restrict__ scale;     // burst scale per cell (commit pass)
-  int4* rec;                       // buckets: {-dh, volume, vol*sx, vol*sy} per step
-  int32_t* deposit;                // terminal deposits, 2^-20
+  int4* slog;                      // [step][drop] step log (count pass)
   uint8_t* visit;                  // cascade sites
   DeviceState* state;
   uint64_t seed;
@@ -167,7 +176,6 @@ __device__ inline CellView load_cell(const WalkArgs& a, int x, int y, int i) {
   c.d = f.z;
   c.erfd = f.w;
   c.root = a.root ? a.root[i] : 0.0f;
-  c.scale = a.scale ? a.scale[i] : 1.0f;
   return c;
 }
 
@@ -183,24 +191,24 @@ __device__ inline void atomic_add_i64(long long* p, long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-enum WalkMode { kTrace = 0, kCount = 1, kCommit = 2 };
+enum WalkMode { kTrace = 0, kCount = 1 };
 
 // One drop from spawn to termination: Drop(pos) then while(drop.descend())
-// (source/world.h:74-76, source/water.h:58-156).  kTrace records per-step
-// states of one drop on the frozen map (unscaled, as Drop::descend);
-// kCount only counts steps per start cell; kCommit commits.
+// (source/world.h:74-76, source/water.h:58-156) on the cycle's snapshot.
+// kTrace records per-step states of one drop (unscaled, as Drop::descend);
+// kCount takes the tickets and writes the step log the replay commits from.
+// Returns the steps taken; *end and *end_cell say how and where it ended.
 template <int MODE>
-__device__ void walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, LaneStats& S,
-                          shy_drop_step* tr, int max_tr, int* tr_n) {
+__device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, int* end,
+                         int* end_cell, shy_drop_step* tr, int max_tr, int* tr_n) {
   constexpr bool TRACE = MODE == kTrace;
-  constexpr bool COMMIT = MODE == kCommit;
+  constexpr bool COUNT = MODE == kCount;
   const shy_params& P = a.P;
   const int nx = a.nx, ny = a.ny;
   int ix = (int)x0, iy = (int)y0;
   int i0 = ix * ny + iy;
   float px = x0, py = y0, sx = 0.0f, sy = 0.0f, vol = 1.0f, sed = 0.0f;
-  int age = 0;
-  double gain = 0.0;
+  int age = 0, steps = 0;
   int nrec = 0;
   CellView c = load_cell(a, ix, iy, i0);
   for (;;) {
@@ -208,23 +216,17 @@ __device__ void walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, 
     normal_from(c, ix, iy, nx, ny, n0, n1, n2);  // water.h:70
     (void)n1;
     if ((float)age > P.maxAge || vol < P.minVol) {  // water.h:74-82
-      if (COMMIT) {
-        const int32_t q = quant_h(sed);
-        atomic_add_i32(&a.deposit[i0], q);
-        S.walk_dh += q;
-        if ((float)age > P.maxAge) S.ends_age++; else S.ends_vol++;
-      }
+      *end = (float)age > P.maxAge ? kEndAge : kEndVol;
+      *end_cell = i0;
       break;
     }
     // step k of this drop (k = age) holds ticket t on its start cell
     const size_t klog = (size_t)age * a.drops + lane_drop;
-    if (MODE == kCount) {
+    if (COUNT) {
       a.ticket[klog] =
           __hip_atomic_fetch_add(&a.count[i0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    int slot = 0;
-    if (COMMIT) slot = a.offset[i0] + a.ticket[klog];
-    S.steps++;
+    ++steps;
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
     step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);  // water.h:95-111
@@ -236,56 +238,48 @@ __device__ void walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, 
     const float h2 = ex ? (float)((double)c.h - 0.002) : cn.h;  // water.h:120-124
     float c_eq = (1.0f + P.entrainment * c.erfd) * (c.h - h2);  // water.h:127
     if (c_eq < 0) c_eq = 0;
-    const float cdiff = (c_eq - sed);
-    float ero;  // water.h:131-132
-    if (COMMIT) {
-      // one quantised amount leaves the map and enters the drop; the step's
-      // tracks (water.h:115-117) ride in the same bucket record
-      const int32_t q = quant_h((effD * cdiff) * c.scale);
-      ero = (float)q * kHQInv;
-      a.rec[slot] = make_int4(-q, quant_t(vol), quant_t(vol * sx), quant_t(vol * sy));
-      S.walk_dh -= q;
-    } else {
-      ero = effD * cdiff;
+    if (COUNT) {
+      // everything of the step that the snapshot fixes; the tracks
+      // (water.h:115-117) use the volume before this step's evaporation
+      a.slog[klog] = make_int4(i0, __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
     }
-    sed += ero;
-    const float s1 = evap_sed(sed, P.evapRate);  // water.h:135-136
-    gain += (double)s1 - (double)sed;
-    sed = s1;
-    vol = evap_vol(vol, P.evapRate);
     if (TRACE) {
+      const float ero = effD * (c_eq - sed);  // water.h:128-132
+      sed += ero;
+      sed = evap_sed(sed, P.evapRate);  // water.h:135
+      const float v1 = evap_vol(vol, P.evapRate);
       if (nrec < max_tr) {
         shy_drop_step r;
         r.age = age; r.status = ex ? 1 : 0;
         r.pos[0] = px; r.pos[1] = py; r.speed[0] = sx; r.speed[1] = sy;
-        r.volume = vol; r.sediment = sed; r.erosion = ero; r.h2 = h2;
+        r.volume = v1; r.sediment = sed; r.erosion = ero; r.h2 = h2;
         tr[nrec] = r;
       }
       ++nrec;
-      if (ex || nrec >= max_tr) break;
-    }
-    if (ex) {  // water.h:139-142
-      if (COMMIT) {
-        S.exits++;
-        S.export40 += quant_40((double)sed);
+      if (ex || nrec >= max_tr) {
+        *end = ex ? kEndExit : kEndAge;
+        *end_cell = i0;
+        break;
       }
+    }
+    vol = evap_vol(vol, P.evapRate);  // water.h:136
+    if (ex) {  // water.h:139-142
+      *end = kEndExit;
+      *end_cell = i0;
       break;
     }
-    if (COMMIT) a.visit[j] = 1;  // water.h:151 -> cascade site
+    if (COUNT) a.visit[j] = 1;  // water.h:151 -> cascade site
     ++age;
     c = cn;
     ix = jx; iy = jy; i0 = j;
   }
-  if (COMMIT) S.gain40 += quant_40(gain);
   if (TRACE) *tr_n = nrec;
+  return steps;
 }
 
-// Every drop of every terrain; grid-stride, so the grid may be smaller than
-// the wave (a persistent launch).
-template <int MODE>
+// Count pass over every drop of every terrain; grid-stride, so the grid may
+// be smaller than the wave (a persistent launch).
 __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
-  LaneStats S;
-  long long walked = 0, skipped = 0;
   const unsigned int cycle = w.state->cycle;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < w.total_drops;
        g += gridDim.x * blockDim.x) {
@@ -295,53 +289,141 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
     const int t = __builtin_amdgcn_readfirstlane(
         find_terrain(w.nter, g0, [&](int k) { return (long long)w.ter[k].drop_off; }));
     const Terrain T = w.ter[t];
-    long long mine = 0;  // this lane's particle-steps in this terrain
+    int4 dl = make_int4(0, kEndPad, 0, 0);
     if (g - T.drop_off < T.drops) {  // not a padding lane
-      WalkArgs a;
+      WalkArgs a{};
       a.h = w.h + T.cell_off;
       a.fld = w.fld + T.cell_off;
       a.root = T.has_root ? w.root + T.cell_off : nullptr;
       a.count = w.count + T.cell_off;
       a.ticket = w.ticket;
-      a.offset = w.offset + T.cell_off;
-      a.scale = w.scale ? w.scale + T.cell_off : nullptr;
-      a.rec = w.rec;
-      a.deposit = w.deposit + T.cell_off;
+      a.slog = w.slog;
       a.visit = w.visit + T.cell_off;
-      a.state = w.state;
-      a.seed = T.seed;
-      a.terrain = T.id;
       a.nx = T.nx;
       a.ny = T.ny;
-      a.drops = w.total_drops;  // ticket-log stride
+      a.drops = w.total_drops;  // log stride
       a.P = T.P;
       const uint32_t ncell = (uint32_t)T.nx * (uint32_t)T.ny;
       const uint32_t cid =
           spawn_cell(cycle_key(T.seed, T.id, cycle), (uint32_t)(g - T.drop_off), ncell);
-      if ((double)a.h[cid] < 0.1) {  // world.h:71-72
-        skipped += 1;
-      } else {
-        walked += 1;
-        const long long s0 = S.steps;
-        walk_drop<MODE>(a, g, (float)(cid / (uint32_t)T.ny), (float)(cid % (uint32_t)T.ny), S,
-                        nullptr, 0, nullptr);
-        mine = S.steps - s0;
+      dl.y = kEndSkipped;
+      if ((double)a.h[cid] >= 0.1) {  // world.h:71-72
+        int end = 0, end_cell = 0;
+        dl.x = walk_drop<kCount>(a, g, (float)(cid / (uint32_t)T.ny),
+                                 (float)(cid % (uint32_t)T.ny), &end, &end_cell, nullptr, 0,
+                                 nullptr);
+        dl.y = end;
+        dl.z = end_cell;
       }
     }
-    if (MODE == kCommit && w.tsteps) {
-      // the wave is converged here (drop ranges are whole waves): one atomic
+    w.dlog[g] = dl;
+  }
+}
+
+// The commits of every drop from its step log (see the file comment).  The
+// arithmetic is the walk's, step for step: cdiff = c_eq - sediment, one
+// quantised amount q = quant_h(effD cdiff s) moves from the map into the
+// drop, then evaporation (water.h:128-136); the terminal deposit or the
+// export ends the drop (water.h:74-82, 139-142).
+struct ReplayStep {
+  int4 L;
+  int tk;
+  int off;
+  float s;
+  float root;
+};
+
+__global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
+  LaneStats S;
+  long long walked = 0, skipped = 0;
+  const size_t stride = (size_t)w.total_drops;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < w.total_drops;
+       g += gridDim.x * blockDim.x) {
+    const int g0 = __builtin_amdgcn_readfirstlane(g);
+    const int t = __builtin_amdgcn_readfirstlane(
+        find_terrain(w.nter, g0, [&](int k) { return (long long)w.ter[k].drop_off; }));
+    const Terrain T = w.ter[t];
+    const int4 dl = w.dlog[g];
+    const int n = dl.y <= kEndExit ? dl.x : 0;
+    if (dl.y == kEndSkipped) skipped += 1;
+    if (dl.y <= kEndExit) {
+      walked += 1;
+      const int32_t* __restrict__ offset = w.offset + T.cell_off;
+      const float* __restrict__ scale = w.scale + T.cell_off;
+      const float* __restrict__ root = T.has_root ? w.root + T.cell_off : nullptr;
+      const float dep = T.P.depositionRate, e = T.P.evapRate;
+      float sed = 0.0f, vol = 1.0f;
+      double gain = 0.0;
+      long long wdh = 0;
+      // two-deep prefetch: the log of step k+2 and the gathers of step k+1
+      // are in flight while step k computes
+      // (rows k >= n of the log are not this drop's: never read them)
+      ReplayStep cur{}, nxt{};
+      int4 L2 = make_int4(0, 0, 0, 0);
+      int tk2 = 0;
+      if (n > 0) {
+        cur.L = w.slog[g];
+        cur.tk = w.ticket[g];
+        cur.off = offset[cur.L.x];
+        cur.s = scale[cur.L.x];
+        cur.root = root ? root[cur.L.x] : 0.0f;
+      }
+      if (n > 1) {
+        nxt.L = w.slog[stride + g];
+        nxt.tk = w.ticket[stride + g];
+      }
+      for (int k = 0; k < n; ++k) {
+        if (k + 2 < n) {
+          const size_t kl = (size_t)(k + 2) * stride + g;
+          L2 = w.slog[kl];
+          tk2 = w.ticket[kl];
+        }
+        if (k + 1 < n) {
+          nxt.off = offset[nxt.L.x];
+          nxt.s = scale[nxt.L.x];
+          nxt.root = root ? root[nxt.L.x] : 0.0f;
+        }
+        float effD = dep * (1.0f - cur.root);  // water.h:86-87
+        if (effD < 0) effD = 0;
+        const float cdiff = __int_as_float(cur.L.y) - sed;
+        const int32_t q = quant_h((effD * cdiff) * cur.s);
+        w.rec[cur.off + cur.tk] = make_int4(-q, quant_t(vol), cur.L.z, cur.L.w);
+        wdh -= q;
+        sed += (float)q * kHQInv;
+        const float s1 = evap_sed(sed, e);
+        gain += (double)s1 - (double)sed;
+        sed = s1;
+        vol = evap_vol(vol, e);
+        cur = nxt;
+        nxt.L = L2;
+        nxt.tk = tk2;
+      }
+      S.steps += n;
+      if (dl.y == kEndExit) {
+        S.exits++;
+        S.export40 += quant_40((double)sed);
+      } else {
+        const int32_t q = quant_h(sed);
+        atomic_add_i32(&w.deposit[T.cell_off + dl.z], q);
+        wdh += q;
+        if (dl.y == kEndAge) S.ends_age++; else S.ends_vol++;
+      }
+      S.walk_dh += wdh;
+      S.gain40 += quant_40(gain);
+    }
+    if (w.tsteps) {
+      // the wave shares one terrain (drop ranges are whole waves): one atomic
       // per wave and terrain instead of one per drop on a shared counter
+      long long mine = n;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, 64);
       if ((threadIdx.x & 63) == 0 && mine)
         __hip_atomic_fetch_add(&w.tsteps[t], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  if (MODE == kCommit) {
-    long long v[ST_CASC_MOVES] = {S.steps, walked, skipped, S.exits, S.ends_age,
-                                  S.ends_vol, S.export40, S.gain40, S.walk_dh};
-    block_add_stats<ST_CASC_MOVES>(v, w.state->stats);
-  }
+  long long v[ST_CASC_MOVES] = {S.steps, walked, skipped, S.exits, S.ends_age,
+                                S.ends_vol, S.export40, S.gain40, S.walk_dh};
+  block_add_stats<ST_CASC_MOVES>(v, w.state->stats);
 }
 
 // r^k by binary exponentiation, least significant bit first.
@@ -507,8 +589,8 @@ __global__ void __launch_bounds__(kReduceCells) reduce_kernel(
 __global__ void trace_kernel(WalkArgs a, float x0, float y0, shy_drop_step* out, int max_steps,
                              int* n_out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  LaneStats S;
-  walk_drop<kTrace>(a, 0, x0, y0, S, out, max_steps, n_out);
+  int end = 0, end_cell = 0;
+  walk_drop<kTrace>(a, 0, x0, y0, &end, &end_cell, out, max_steps, n_out);
 }
 
 // --------------------------------------------------------------- cascade
@@ -811,6 +893,136 @@ __global__ void fbm_kernel(float* out, int nx, int ny, const uint8_t* __restrict
   out[i] = s;
 }
 
+// ------------------------------------------- the reference's own terrain
+// quad::map::init (cellpool.h:349-409): eight layers of FastNoiseLite noise
+// (OpenSimplex2, 3D, FBm of 3 octaves, seed 1337, lacunarity 2, gain 0.5,
+// bounding 1/1.75) at frequencies 1, 2, 4, ... and weights 0.6, 0.36, ...,
+// sampled at (x/tileres, y/tileres, SEED % 10000); then (h - min)/(max - min)
+// with min <= 0 <= max.  The noise follows FastNoiseLite.h:321-339 (GetNoise),
+// 687-727 (the DefaultOpenSimplex2 rotation), 866-885 (FBm) and 1054-1153
+// (SingleOpenSimplex2), every float operation in the same order.
+
+constexpr int32_t kOsPrimeX = 501125321;
+constexpr int32_t kOsPrimeY = 1136930381;
+constexpr int32_t kOsPrimeZ = 1720413743;
+
+// integer products wrap mod 2^32 (the reference's signed overflow)
+__device__ inline int32_t wrap_mul(int32_t a, int32_t b) {
+  return (int32_t)((uint32_t)a * (uint32_t)b);
+}
+__device__ inline int32_t wrap_add(int32_t a, int32_t b) {
+  return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+
+// The 3D gradient set of FastNoiseLite.h:2529-2547: 64 entries = the 12
+// cube-edge directions five times over, then edges 8, 1, 9, 3.  Edge e has
+// its zero in component e/4; the two non-zero components are + unless bit 0
+// (first) or bit 1 (second) of e%4 is set.
+__device__ inline float os2_grad(int32_t seed, int32_t i, int32_t j, int32_t k, float x, float y,
+                                 float z) {
+  int32_t h = wrap_mul(seed ^ i ^ j ^ k, 0x27d4eb2d);  // FastNoiseLite.h:500-506
+  h ^= h >> 15;                                       // arithmetic shift
+  const int entry = (h & (63 << 2)) >> 2;             // :542-554
+  const int e = entry < 60 ? entry % 12 : (0x3918 >> (4 * (entry - 60))) & 15;
+  const float a = (e & 1) ? -1.0f : 1.0f, b = (e & 2) ? -1.0f : 1.0f;
+  const int zero = e >> 2;
+  const float gx = zero == 0 ? 0.0f : a;
+  const float gy = zero == 0 ? a : (zero == 1 ? 0.0f : b);
+  const float gz = zero == 2 ? 0.0f : b;
+  return x * gx + y * gy + z * gz;
+}
+
+__device__ inline int os2_round(float f) { return f >= 0 ? (int)(f + 0.5f) : (int)(f - 0.5f); }
+
+// SingleOpenSimplex2 (FastNoiseLite.h:1054-1153): two offset rotated cube grids
+__device__ inline float os2_single3(int32_t seed, float x, float y, float z) {
+  int32_t i = os2_round(x), j = os2_round(y), k = os2_round(z);
+  float x0 = (float)(x - (float)i), y0 = (float)(y - (float)j), z0 = (float)(z - (float)k);
+  int xs = (int)(-1.0f - x0) | 1, ys = (int)(-1.0f - y0) | 1, zs = (int)(-1.0f - z0) | 1;
+  float ax0 = (float)xs * -x0, ay0 = (float)ys * -y0, az0 = (float)zs * -z0;
+  i = wrap_mul(i, kOsPrimeX);
+  j = wrap_mul(j, kOsPrimeY);
+  k = wrap_mul(k, kOsPrimeZ);
+  float value = 0.0f;
+  float a = (0.6f - x0 * x0) - (y0 * y0 + z0 * z0);
+  for (int l = 0;; ++l) {
+    if (a > 0) value += (a * a) * (a * a) * os2_grad(seed, i, j, k, x0, y0, z0);
+    float b = a + 1.0f;
+    int32_t i1 = i, j1 = j, k1 = k;
+    float x1 = x0, y1 = y0, z1 = z0;
+    if (ax0 >= ay0 && ax0 >= az0) {
+      x1 += (float)xs;
+      b -= (float)(xs * 2) * x1;
+      i1 = wrap_add(i1, -wrap_mul(xs, kOsPrimeX));
+    } else if (ay0 > ax0 && ay0 >= az0) {
+      y1 += (float)ys;
+      b -= (float)(ys * 2) * y1;
+      j1 = wrap_add(j1, -wrap_mul(ys, kOsPrimeY));
+    } else {
+      z1 += (float)zs;
+      b -= (float)(zs * 2) * z1;
+      k1 = wrap_add(k1, -wrap_mul(zs, kOsPrimeZ));
+    }
+    if (b > 0) value += (b * b) * (b * b) * os2_grad(seed, i1, j1, k1, x1, y1, z1);
+    if (l == 1) break;
+    ax0 = 0.5f - ax0;
+    ay0 = 0.5f - ay0;
+    az0 = 0.5f - az0;
+    x0 = (float)xs * ax0;
+    y0 = (float)ys * ay0;
+    z0 = (float)zs * az0;
+    a += (0.75f - ax0) - (ay0 + az0);
+    i = wrap_add(i, (xs >> 1) & kOsPrimeX);
+    j = wrap_add(j, (ys >> 1) & kOsPrimeY);
+    k = wrap_add(k, (zs >> 1) & kOsPrimeZ);
+    xs = -xs;
+    ys = -ys;
+    zs = -zs;
+    seed = ~seed;
+  }
+  return value * 32.69428253173828125f;
+}
+
+// GetNoise(x, y, z) of a FastNoiseLite(1337) set to OpenSimplex2 + FBm with
+// its default 3 octaves, at frequency f
+__device__ inline float fnl_noise3(float f, float x, float y, float z) {
+  x *= f;
+  y *= f;
+  z *= f;
+  const float r3 = (float)(2.0 / 3.0);
+  const float r = (x + y + z) * r3;  // rotation, not skew
+  x = r - x;
+  y = r - y;
+  z = r - z;
+  int32_t seed = 1337;
+  float sum = 0.0f, amp = 1.0f / 1.75f;
+  for (int o = 0; o < 3; ++o) {
+    const float noise = os2_single3(seed++, x, y, z);
+    sum += noise * amp;
+    amp *= 1.0f + 0.0f * ((noise + 1.0f) * 0.5f - 1.0f);  // Lerp(1, .., weightedStrength 0)
+    x *= 2.0f;
+    y *= 2.0f;
+    z *= 2.0f;
+    amp *= 0.5f;
+  }
+  return sum;
+}
+
+__global__ void fnl_world_kernel(float* out, int nx, int ny, int seed, float tileres) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int x = (int)(i / (size_t)ny), y = (int)(i % (size_t)ny);
+  const float px = (float)x / tileres, py = (float)y / tileres;
+  const float pz = (float)(seed % 10000);
+  float h = 0.0f, freq = 1.0f, scale = 0.6f;
+  for (int o = 0; o < 8; ++o) {
+    h += scale * fnl_noise3(freq, px, py, pz);
+    freq *= 2.0f;
+    scale = (float)((double)scale * 0.6);
+  }
+  out[i] = h;
+}
+
 // min/max of a float array: per-block partials then one block folds them.
 __global__ void minmax_partial_kernel(const float* __restrict__ v, size_t n, float2* part) {
   __shared__ float smn[256], smx[256];
@@ -834,10 +1046,11 @@ __global__ void minmax_partial_kernel(const float* __restrict__ v, size_t n, flo
   if (threadIdx.x == 0) part[blockIdx.x] = make_float2(smn[0], smx[0]);
 }
 
-// fold the per-block partials into part[0] (one block)
-__global__ void minmax_fold_kernel(float2* part, int nparts) {
+// fold the per-block partials into part[0] (one block); with_zero starts
+// the fold at min = max = 0 (quad::map::init, cellpool.h:385-392)
+__global__ void minmax_fold_kernel(float2* part, int nparts, int with_zero) {
   __shared__ float smn[256], smx[256];
-  float mn = __builtin_inff(), mx = -__builtin_inff();
+  float mn = with_zero ? 0.0f : __builtin_inff(), mx = with_zero ? 0.0f : -__builtin_inff();
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
     const float2 p = part[k];
     mn = p.x < mn ? p.x : mn;

@@ -8,6 +8,7 @@
 // Drop::descend on the live map afterwards.
 #include <cstdint>
 #include <cstdio>
+#include <string>
 #include <cstdlib>
 #include <cstring>
 
@@ -28,9 +29,11 @@ int main(int argc, char** argv) {
   const int cycles = argc > 2 ? std::atoi(argv[2]) : 10;
   const int drops = argc > 3 ? std::atoi(argv[3]) : 4096;
   const unsigned seed = argc > 4 ? (unsigned)std::atoi(argv[4]) : 1u;
+  const bool fbm = argc > 5 && std::string(argv[5]) == "fbm";
   try {
     shy::World::SEED = seed;
-    shy::World::map.init(n, n, shy::World::SEED);
+    if (fbm) shy::World::map.init_fbm(n, n, shy::World::SEED);
+    else shy::World::map.init(n, n, shy::World::SEED);
     for (int i = 0; i < cycles; ++i) shy::World::erode(drops);
     shy::World::map.sync();
     uint64_t h = 1469598103934665603ULL;

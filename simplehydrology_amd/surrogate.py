@@ -4,7 +4,7 @@ No reference counterpart: the reference converges the exponentially averaged
 discharge/momentum fields (source/world.h:80-86) only by running the particle
 method.  This module predicts the converged fields from a terrain so the
 particle method can start near its fixed point.  The particle method stays
-the fallback and the judge (:func:`verify_warm_start`).
+the fallback and the judge (``train_surrogate.warm_start``).
 
 The network is a 4-level residual U-Net, 3x3 convolutions,
 32-64-96-128 channels, BF16 weights and activations with FP32 accumulation.
@@ -13,9 +13,12 @@ upsampling, loss and Adam -- is a HIP kernel of ``libshy.so``
 (``csrc/shy_nn.hip``, BF16 MFMA for all convolutions).  torch supplies device
 memory and the stream only.
 
-Input channels (NHWC, 8 with zero padding): height, then the transformed
-discharge and momentum of :func:`encode_flow`.  Output: 3 channels in the
-same transformed space (the head has 8 outputs; the last 5 are unused).
+Input channels (NHWC, 8): height centred on the window mean (x4), the
+central-difference slopes along x and y (height per cell x 512), the
+transformed discharge and momentum of :func:`encode_flow` of the current
+state, two zero channels.  Output: converged (discharge, momentumx,
+momentumy) in the transformed space (the head has 8 outputs; the last 5 are
+unused).
 """
 from __future__ import annotations
 
@@ -92,13 +95,38 @@ def decode_flow(y):
     return d, mx, my
 
 
-def make_input(h, d, mx, my):
-    """NHWC bf16 input with 8 channels: h, encoded flow, zeros."""
+SLOPE_SCALE = 512.0
+H_SCALE = 4.0
+
+
+def features(h, d, mx, my):
+    """Per-cell input features of a whole map ([X, Y] f32 device tensors) ->
+    [X, Y, 8] f32.  Channel 0 holds the raw height; :func:`window_batch`
+    centres it per window."""
     import torch
-    x = torch.zeros(*h.shape, 8, dtype=torch.float32, device=h.device)
-    x[..., 0] = h
-    x[..., 1:4] = encode_flow(d, mx, my)
-    return x.to(torch.bfloat16).contiguous()
+    X, Y = h.shape
+    f = torch.zeros(X, Y, 8, dtype=torch.float32, device=h.device)
+    f[..., 0] = h
+    hp = torch.nn.functional.pad(h[None, None], (1, 1, 1, 1), mode="replicate")[0, 0]
+    f[..., 1] = (hp[2:, 1:-1] - hp[:-2, 1:-1]) * SLOPE_SCALE
+    f[..., 2] = (hp[1:-1, 2:] - hp[1:-1, :-2]) * SLOPE_SCALE
+    f[..., 3:6] = encode_flow(d, mx, my)
+    return f
+
+
+def window_batch(feats, origins, tile):
+    """Stack windows feats[x0:x0+tile, y0:y0+tile] into a bf16 NHWC batch,
+    with the height channel centred and scaled per window."""
+    import torch
+    xs = torch.stack([feats[x0:x0 + tile, y0:y0 + tile] for x0, y0 in origins])
+    h = xs[..., 0]
+    xs[..., 0] = (h - h.mean(dim=(1, 2), keepdim=True)) * H_SCALE
+    return xs.to(torch.bfloat16).contiguous()
+
+
+def make_input(h, d, mx, my):
+    """One window covering the whole map: [1, X, Y, 8] bf16."""
+    return window_batch(features(h, d, mx, my), [(0, 0)], h.shape[0])[0]
 
 
 # -------------------------------------------------------------------- layers
@@ -342,21 +370,36 @@ def mse_loss_and_grad(y, target, mask=None):
     return part.sum() / count, g
 
 
-def predict_tiled(net, h, d, mx, my, tile=256, halo=32):
-    """Apply the network tile-wise with a halo to an arbitrarily large map
-    (inputs [H, W] f32 on the device); returns decoded (d, mx, my)."""
-    import torch
-    H, W = h.shape
+def _tile_starts(n, tile, halo):
+    """(window start, core start, core length) covering [0, n): cores of
+    tile-2*halo cells, windows shifted inward at the map edges."""
     core = tile - 2 * halo
-    out = torch.zeros(H, W, 3, dtype=torch.float32, device=h.device)
-    x_full = make_input(h, d, mx, my)
-    pad = torch.zeros(H + 2 * halo + tile, W + 2 * halo + tile, 8, dtype=torch.bfloat16,
-                      device=h.device)
-    pad[halo:halo + H, halo:halo + W] = x_full
-    for y0 in range(0, H, core):
-        for x0 in range(0, W, core):
-            xt = pad[y0:y0 + tile, x0:x0 + tile].unsqueeze(0).contiguous()
-            yt = net.forward(xt)[0, halo:halo + core, halo:halo + core, :3]
-            hh, ww = min(core, H - y0), min(core, W - x0)
-            out[y0:y0 + hh, x0:x0 + ww] = yt[:hh, :ww]
+    out = []
+    c0 = 0
+    while c0 < n:
+        clen = min(core, n - c0)
+        w0 = min(max(c0 - halo, 0), n - tile)
+        out.append((w0, c0, clen))
+        c0 += clen
+    return out
+
+
+def predict_tiled(net, h, d, mx, my, tile=256, halo=32, batch=16):
+    """Apply the network window-wise to a map of any size >= tile (inputs
+    [X, Y] f32 device tensors): each window of tile x tile cells contributes
+    its core, at least ``halo`` cells from every window edge that is not a map
+    edge.  Returns decoded (d, mx, my)."""
+    import torch
+    X, Y = h.shape
+    if X < tile or Y < tile:
+        raise ValueError(f"map {X}x{Y} is smaller than one {tile}x{tile} window")
+    feats = features(h, d, mx, my)
+    out = torch.empty(X, Y, 3, dtype=torch.float32, device=h.device)
+    jobs = [(wx, cx, lx, wy, cy, ly) for wx, cx, lx in _tile_starts(X, tile, halo)
+            for wy, cy, ly in _tile_starts(Y, tile, halo)]
+    for b0 in range(0, len(jobs), batch):
+        js = jobs[b0:b0 + batch]
+        y = net.forward(window_batch(feats, [(j[0], j[3]) for j in js], tile))
+        for k, (wx, cx, lx, wy, cy, ly) in enumerate(js):
+            out[cx:cx + lx, cy:cy + ly] = y[k, cx - wx:cx - wx + lx, cy - wy:cy - wy + ly, :3]
     return decode_flow(out)

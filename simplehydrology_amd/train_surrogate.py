@@ -1,233 +1,413 @@
-"""Surrogate pipeline: data from the GPU simulator, training, warm start.
+"""Surrogate pipeline: simulator-made data, training, warm start and its gate.
 
-1. ``generate``: batches of seeded 256x256 fBm terrains run as one ensemble
-   (one launch per kernel for the whole batch) from rest for ``cycles``
-   cycles; the sample is (initial height, zero flow) -> the mean of
-   (discharge, momentumx, momentumy) over the last ``avg`` cycles.
-2. ``train``: the U-Net of :mod:`surrogate` with Adam, every operation a
-   HIP kernel (BF16 MFMA convolutions, FP32 accumulation).
-3. ``warm_start``: tile-wise inference with a halo, written into a terrain
-   as its initial discharge/momentum, then the plain particle method runs K
-   verification cycles and the warm start is rejected if the discharge field
-   moved by more than the stated L1 bound (the particle method is the judge
-   and the fallback).
+The discharge/momentum maps of the particle method are exponential averages
+of per-cycle tracks (source/world.h:80-86, rate lrate = 0.1), so from a cold
+start (all zero) they take tens of cycles to fill and then keep
+self-organising for hundreds of cycles while drops follow the momentum
+field.  The surrogate predicts the long-run mean fields so that the particle
+method can start near them.
 
-Run ``python -m simplehydrology_amd.train_surrogate --help``.
+Data (:func:`simulate_flow`): the GPU engine runs a seeded fBm terrain with
+the deposition rate set to zero (the same kernels; the terrain then stays
+fixed and the long-run mean flow is a well-defined target).  A sample pairs
+the state after ``c0`` cold cycles (height, discharge, momentum) with the mean
+(discharge, momentum) over cycles ``[t0, t1)``.  Training draws 256x256
+windows from such maps (:func:`train`), with the dihedral flips that map the
+method onto itself.
+
+Warm start (:func:`warm_start`): ``c0`` plain cycles, window-wise inference
+with a 32-cell halo, import of the predicted fields, then ``K`` plain
+verification cycles.  The gate compares the mean of erf(0.4 d) over the last
+``W`` verification cycles with the prediction and rejects the warm start
+(keeping the plain run's state) when their relative L1 distance exceeds
+``gate_l1``.  The particle method stays the judge and the fallback.
+
+Evaluation (:func:`evaluate`): relative L1 error of trailing ``W``-cycle
+mean fields against the mean over ``[ref0, ref1)`` of a long cold run, for
+the cold run and the warm-started run; time-to-steady at matched error is
+the wall time the cold run needs to reach the warm run's error after its
+verification cycles.
+
+``python -m simplehydrology_amd.train_surrogate --help``
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import sys
 import time
 
 import numpy as np
 
-from . import Ensemble, Params, World
+from . import Params, World
 from . import surrogate as sg
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "weights", "surrogate_unet.npz")
 
 
 def l1_rel(a, b):
-    """sum |a-b| / sum |b| (torch tensors)"""
+    """sum |a - b| / sum |b| over all cells."""
     return float((a - b).abs().sum() / b.abs().sum().clamp_min(1e-30))
 
 
-def flow_tensor(world_or_ens, terrain, n, device):
+def erf_discharge(d):
+    """node::discharge (source/cellpool.h:230): erf(0.4 d)."""
     import torch
-    buf = torch.empty(n, n, 4, dtype=torch.float32, device=device)
-    if terrain is None:
-        world_or_ens.export(buf.data_ptr())
-    else:
-        world_or_ens.export(terrain, buf.data_ptr())
-    return buf
+    return torch.erf(0.4 * d)
 
 
-def generate(n_terrains, size=256, cycles=80, avg=20, batch=256, seed0=0, device="cuda:0",
-             log=print):
-    """Simulator-made training pairs.  Returns (x bf16 [N,S,S,8], y f32 [N,S,S,3])."""
+def flow_world(n, seed, deposition=0.0, octaves=8, base_freq=0.0):
+    p = Params()
+    p.depositionRate = deposition
+    w = World(n, n, seed=seed, params=p)
+    w.generate_fbm(seed, octaves, base_freq)
+    return w
+
+
+class Recorder:
+    """Exports (h, d, mx, my) after a cycle into one reused device buffer."""
+
+    def __init__(self, world, device="cuda:0"):
+        import torch
+        self.w = world
+        self.buf = torch.empty(world.nx, world.ny, 4, dtype=torch.float32, device=device)
+
+    def grab(self):
+        self.w.export(self.buf.data_ptr())
+        return self.buf
+
+
+def simulate_flow(n, seed, c0=10, t0=100, t1=200, device="cuda:0"):
+    """One training map: state after c0 cycles -> mean flow over [t0, t1).
+
+    Returns (state [n,n,4] f32: h, d, mx, my), (target [n,n,3] f32: mean d,
+    mean mx, mean my) and the particle steps taken."""
     import torch
-    drops = size * size // 16
-    xs = torch.empty(n_terrains, size, size, 8, dtype=torch.bfloat16, device=device)
-    ys = torch.empty(n_terrains, size, size, 3, dtype=torch.float32, device=device)
+    w = flow_world(n, seed)
+    drops = n * n // 16
+    rec = Recorder(w, device)
+    w.erode_cycles(c0, drops, graph=True)
+    state = rec.grab().clone()
+    w.erode_cycles(t0 - c0, drops, graph=True)
+    acc = torch.zeros(n, n, 3, dtype=torch.float32, device=device)
+    for _ in range(t1 - t0):
+        w.erode_cycles(1, drops, graph=False)
+        acc += rec.grab()[..., 1:4]
+    acc /= (t1 - t0)
+    steps = int(w.stats()["steps"])
+    w.close()
+    return state, acc, steps
+
+
+def generate(n_maps, n=2048, c0=10, t0=100, t1=200, seed0=1000, device="cuda:0", log=print):
+    import torch
+    states = torch.empty(n_maps, n, n, 4, dtype=torch.float32, device=device)
+    targets = torch.empty(n_maps, n, n, 3, dtype=torch.float32, device=device)
+    t_start = time.perf_counter()
     steps = 0
-    t0 = time.perf_counter()
-    for b0 in range(0, n_terrains, batch):
-        nb = min(batch, n_terrains - b0)
-        specs = [dict(nx=size, ny=size, drops=drops, seed=seed0 + b0 + i) for i in range(nb)]
-        e = Ensemble(specs)
-        for i, s in enumerate(specs):
-            e.generate_fbm(i, s["seed"], 8)
-        buf = torch.empty(size, size, 4, dtype=torch.float32, device=device)
-        for i in range(nb):
-            e.export(i, buf.data_ptr())
-            z = torch.zeros_like(buf[..., 0])
-            xs[b0 + i] = sg.make_input(buf[..., 0], z, z, z)
-        e.erode(cycles - avg, graph=True)
-        acc = torch.zeros(nb, size, size, 3, dtype=torch.float32, device=device)
-        for _ in range(avg):
-            e.erode(1, graph=False)
-            for i in range(nb):
-                e.export(i, buf.data_ptr())
-                acc[i] += buf[..., 1:4]
-        acc /= avg
-        ys[b0:b0 + nb] = sg.encode_flow(acc[..., 0], acc[..., 1], acc[..., 2])
-        steps += int(e.stats()["steps"])
-        e.close()
-        log(f"generated {b0 + nb}/{n_terrains} terrains, {time.perf_counter() - t0:.1f} s")
-    return xs, ys, {"seconds": time.perf_counter() - t0, "particle_steps": steps,
-                    "cycles": cycles, "avg": avg, "size": size}
+    for i in range(n_maps):
+        s, t, k = simulate_flow(n, seed0 + i, c0, t0, t1, device)
+        states[i] = s
+        targets[i] = t
+        steps += k
+        if (i + 1) % 4 == 0 or i + 1 == n_maps:
+            torch.cuda.synchronize()
+            log(json.dumps({"generated": i + 1, "of": n_maps,
+                            "seconds": round(time.perf_counter() - t_start, 2)}))
+    torch.cuda.synchronize()
+    info = {"maps": n_maps, "size": n, "c0": c0, "mean_over": [t0, t1],
+            "particle_steps": steps, "seconds": time.perf_counter() - t_start}
+    return states, targets, info
 
 
-def train(net, xs, ys, epochs=4, batch=8, lr=2e-3, log=print, val=None):
+def map_features(states):
+    """[M, n, n, 8] f32 features of every training map (height raw in ch 0)."""
     import torch
-    n = xs.shape[0]
-    steps_per_epoch = n // batch
-    total = epochs * steps_per_epoch
-    t0 = time.perf_counter()
+    return torch.stack([sg.features(s[..., 0], s[..., 1], s[..., 2], s[..., 3]) for s in states])
+
+
+def _augment(x, y, k):
+    """Dihedral symmetry k in [0, 8): bit 0 flips x, bit 1 flips y, bit 2
+    transposes.  x [B,T,T,8] (h, gx, gy, d, mx, my, 0, 0), y [B,T,T,3]."""
+    import torch
+    if k & 1:
+        x = torch.flip(x, (1,))
+        y = torch.flip(y, (1,))
+        x[..., 1] = -x[..., 1]
+        x[..., 4] = -x[..., 4]
+        y[..., 1] = -y[..., 1]
+    if k & 2:
+        x = torch.flip(x, (2,))
+        y = torch.flip(y, (2,))
+        x[..., 2] = -x[..., 2]
+        x[..., 5] = -x[..., 5]
+        y[..., 2] = -y[..., 2]
+    if k & 4:
+        x = x.transpose(1, 2)
+        y = y.transpose(1, 2)
+        x = x[..., [0, 2, 1, 3, 5, 4, 6, 7]]
+        y = y[..., [0, 2, 1]]
+    return x.contiguous(), y.contiguous()
+
+
+def sample_batch(feats, targets_enc, batch, tile, rng):
+    """Random windows (and symmetries) from the training maps."""
+    import torch
+    M, n = feats.shape[0], feats.shape[1]
+    ms = rng.integers(0, M, batch)
+    xs = rng.integers(0, n - tile + 1, batch)
+    ys = rng.integers(0, n - tile + 1, batch)
+    x = torch.stack([feats[m, a:a + tile, b:b + tile] for m, a, b in zip(ms, xs, ys)])
+    y = torch.stack([targets_enc[m, a:a + tile, b:b + tile] for m, a, b in zip(ms, xs, ys)])
+    x, y = _augment(x, y, int(rng.integers(0, 8)))
+    h = x[..., 0]
+    x[..., 0] = (h - h.mean(dim=(1, 2), keepdim=True)) * sg.H_SCALE
+    return x.to(torch.bfloat16).contiguous(), y
+
+
+def train(net, feats, targets_enc, steps=2000, batch=16, tile=256, lr=2e-3, seed=0, log=print,
+          log_every=200):
+    """Adam with cosine decay on random windows; every op a libshy kernel."""
+    import torch
+    rng = np.random.default_rng(seed)
     hist = []
-    g = torch.Generator(device="cpu").manual_seed(0)
-    k = 0
-    for ep in range(epochs):
-        perm = torch.randperm(n, generator=g)
-        run = 0.0
-        for s in range(steps_per_epoch):
-            idx = perm[s * batch:(s + 1) * batch].to(xs.device)
-            x = xs.index_select(0, idx).contiguous()
-            y = ys.index_select(0, idx)
-            out = net.forward(x)
-            loss, gy = sg.mse_loss_and_grad(out, y)
-            net.backward(gy)
-            # cosine decay
-            cur = lr * 0.5 * (1.0 + np.cos(np.pi * k / max(1, total)))
-            net.step(cur)
-            k += 1
-            run += float(loss.item()) if s % 20 == 0 else 0.0
-        torch.cuda.synchronize()
-        entry = {"epoch": ep + 1, "seconds": time.perf_counter() - t0}
-        if val is not None:
-            entry["val_loss"] = evaluate(net, *val)
-        hist.append(entry)
-        log(json.dumps(entry))
-    return hist
-
-
-def evaluate(net, xs, ys, batch=8):
-    import torch
-    tot, cnt = 0.0, 0
-    for b0 in range(0, xs.shape[0], batch):
-        out = net.forward(xs[b0:b0 + batch].contiguous())
-        tot += float(((out[..., :3] - ys[b0:b0 + batch]) ** 2).sum())
-        cnt += out[..., :3].numel()
-    torch.cuda.synchronize()
-    return tot / cnt
-
-
-def discharge_error(net, xs, ys, batch=8):
-    """Mean relative L1 error of the decoded discharge field."""
-    errs = []
-    for b0 in range(0, xs.shape[0], batch):
-        out = net.forward(xs[b0:b0 + batch].contiguous())
-        d, _, _ = sg.decode_flow(out[..., :3])
-        dt, _, _ = sg.decode_flow(ys[b0:b0 + batch])
-        for i in range(d.shape[0]):
-            errs.append(l1_rel(d[i], dt[i]))
-    return float(np.mean(errs))
-
-
-def conv_timing(net, batch=8, size=256, reps=5):
-    """FLOP/s of one training step (forward + backward) from HIP events."""
-    import torch
-    x = torch.zeros(batch, size, size, 8, dtype=torch.bfloat16, device=net.device)
-    y = torch.zeros(batch, size, size, 3, dtype=torch.float32, device=net.device)
-    out = net.forward(x)
-    loss, gy = sg.mse_loss_and_grad(out, y)
-    net.backward(gy)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        out = net.forward(x)
-    e1.record()
-    torch.cuda.synchronize()
-    fwd_ms = e0.elapsed_time(e1) / reps
-    e0.record()
-    for _ in range(reps):
+    t0 = time.perf_counter()
+    for k in range(steps):
+        x, y = sample_batch(feats, targets_enc, batch, tile, rng)
         out = net.forward(x)
         loss, gy = sg.mse_loss_and_grad(out, y)
         net.backward(gy)
-    e1.record()
+        net.step(lr * 0.5 * (1.0 + np.cos(np.pi * k / steps)))
+        if (k + 1) % log_every == 0 or k + 1 == steps:
+            v = float(loss.item())
+            entry = {"step": k + 1, "loss": v, "seconds": round(time.perf_counter() - t0, 2)}
+            hist.append(entry)
+            log(json.dumps(entry))
     torch.cuda.synchronize()
-    step_ms = e0.elapsed_time(e1) / reps
-    f = net.flops(batch, size, size)
-    return {"batch": batch, "forward_ms": fwd_ms, "train_step_ms": step_ms,
-            "forward_tflops": f / fwd_ms / 1e9, "train_tflops": 3 * f / step_ms / 1e9,
-            "gflop_per_sample_forward": f / batch / 1e9}
+    return hist, time.perf_counter() - t0
 
 
-def steady_probe(size=256, terrains=16, cycles=200, device="cuda:0", log=print):
-    """Per-cycle relative L1 change of discharge and of erf(0.4 d), averaged
-    over a batch of terrains run cold from rest."""
+def train_timing(net, batch=16, tile=256, reps=5):
+    """Forward and forward+backward+Adam time of one batch (HIP events) and
+    the algorithmic FLOP rate (forward FLOPs x 3 for a training step)."""
     import torch
-    drops = size * size // 16
-    e = Ensemble([dict(nx=size, ny=size, drops=drops, seed=500 + i) for i in range(terrains)])
-    for i in range(terrains):
-        e.generate_fbm(i, 500 + i, 8)
-    prev = [None] * terrains
-    out = []
-    buf = torch.empty(size, size, 4, dtype=torch.float32, device=device)
-    for c in range(cycles):
-        e.erode(1, graph=False)
-        dd, de = [], []
-        for i in range(terrains):
-            e.export(i, buf.data_ptr())
-            d = buf[..., 1].clone()
-            if prev[i] is not None:
-                dd.append(l1_rel(d, prev[i]))
-                de.append(l1_rel(torch.erf(0.4 * d), torch.erf(0.4 * prev[i])))
-            prev[i] = d
-        if dd:
-            out.append((c, float(np.mean(dd)), float(np.mean(de))))
-    e.close()
-    return out
+    x = torch.zeros(batch, tile, tile, 8, dtype=torch.bfloat16, device=net.device)
+    y = torch.zeros(batch, tile, tile, 3, dtype=torch.float32, device=net.device)
+
+    def fwd():
+        return net.forward(x)
+
+    def step():
+        out = net.forward(x)
+        loss, gy = sg.mse_loss_and_grad(out, y)
+        net.backward(gy)
+        net.step(0.0)
+
+    step()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    for _ in range(reps):
+        fwd()
+    e[1].record()
+    for _ in range(reps):
+        step()
+    e[2].record()
+    torch.cuda.synchronize()
+    f = net.flops(batch, tile, tile)
+    fwd_ms = e[0].elapsed_time(e[1]) / reps
+    step_ms = e[1].elapsed_time(e[2]) / reps
+    return {"batch": batch, "tile": tile, "forward_ms": fwd_ms, "train_step_ms": step_ms,
+            "forward_gflop": f / 1e9, "forward_tflops": f / fwd_ms / 1e9,
+            "train_tflops": 3 * f / step_ms / 1e9}
+
+
+def validate(net, states, targets, tile=256, halo=32):
+    """Relative L1 of erf(0.4 d) predicted window-wise vs the target mean."""
+    errs = []
+    for s, t in zip(states, targets):
+        d, _, _ = sg.predict_tiled(net, s[..., 0], s[..., 1], s[..., 2], s[..., 3], tile, halo)
+        errs.append(l1_rel(erf_discharge(d), erf_discharge(t[..., 0])))
+    return errs
+
+
+class Timer:
+    """Wall time between two full device synchronisations (the engine runs
+    on its own stream, torch's events would not see it)."""
+
+    def __init__(self):
+        import torch
+        torch.cuda.synchronize()
+        self.t0 = time.perf_counter()
+
+    def ms(self):
+        import torch
+        torch.cuda.synchronize()
+        return (time.perf_counter() - self.t0) * 1e3
+
+
+def warm_start(net, world, drops, seed, K=20, W=10, gate_l1=0.35, tile=256, halo=32):
+    """Predict, import, verify.  ``world`` holds the plain run's current
+    state (spawn seed ``seed``).  Returns a dict with the gate statistic and
+    the accepted flag; on rejection the state before the import is restored
+    (the plain run continues from where it was)."""
+    import torch
+    rec = Recorder(world)
+    st = rec.grab().clone()
+    before, cycle_before = world.cells(), world.cycle
+    t = Timer()
+    d, mx, my = sg.predict_tiled(net, st[..., 0], st[..., 1], st[..., 2], st[..., 3], tile,
+                                 halo)
+    flow = torch.stack([d, mx, my], -1).contiguous()
+    world.import_flow(flow.data_ptr())
+    infer_ms = t.ms()
+    pred = erf_discharge(d)
+    acc = torch.zeros_like(pred)
+    t = Timer()
+    for c in range(K):
+        world.erode_cycles(1, drops, graph=False)
+        if c >= K - W:
+            acc += erf_discharge(rec.grab()[..., 1])
+    verify_ms = t.ms()
+    acc /= W
+    gate = l1_rel(acc, pred)
+    accepted = gate <= gate_l1
+    if not accepted:
+        world.upload_cells(before)
+        world.set_seed(seed, cycle_before)
+    return {"accepted": accepted, "gate_l1": gate, "gate_threshold": gate_l1,
+            "infer_ms": infer_ms, "verify_ms": verify_ms, "K": K, "W": W,
+            "window_mean": acc}
+
+
+def evaluate(net, n=2048, seed=7, c0=10, K=20, W=10, ref=(200, 400), warm_cycles=100,
+             gate_l1=0.35, log=print):
+    """Cold vs warm-started plain runs on a fixed fBm terrain (deposition 0):
+    errors of trailing W-cycle means vs the long cold run's mean over ref."""
+    import torch
+    drops = n * n // 16
+    ref0, ref1 = ref
+    w = flow_world(n, seed)
+    rec = Recorder(w)
+    Ds = torch.empty(ref1, n, n, dtype=torch.float16, device="cuda:0")
+    cyc_ms = []
+    state_c0 = None
+    for c in range(ref1):
+        t = Timer()
+        w.erode_cycles(1, drops, graph=False)
+        cyc_ms.append(t.ms())
+        Ds[c] = erf_discharge(rec.grab()[..., 1]).half()
+        if c + 1 == c0:
+            state_c0 = w.cells()
+    # plain cycle time with the graph path (what a production run uses)
+    t = Timer()
+    w.erode_cycles(20, drops, graph=True)
+    cycle_ms = t.ms() / 20
+    w.close()
+    Q = Ds[ref0:ref1].float().mean(0)
+
+    def win(c):  # trailing mean ending at cycle c (1-based)
+        return Ds[c - W:c].float().mean(0)
+    e_cold = {c: l1_rel(win(c), Q) for c in range(W, ref0 + 1)}
+    # warm start from the cold run's state after c0 cycles
+    w2 = flow_world(n, seed)
+    w2.upload_cells(state_c0)
+    w2.set_seed(seed, c0)
+    ws = warm_start(net, w2, drops, seed, K=K, W=W, gate_l1=gate_l1)
+    e_warm = {c0 + K: l1_rel(ws.pop("window_mean"), Q)}
+    rec2 = Recorder(w2)
+    hist = []
+    for c in range(c0 + K + 1, c0 + K + warm_cycles + 1):
+        w2.erode_cycles(1, drops, graph=False)
+        hist.append(erf_discharge(rec2.grab()[..., 1]))
+        if len(hist) > W:
+            hist.pop(0)
+        if c % 10 == 0 and len(hist) == W:
+            e_warm[c] = l1_rel(torch.stack(hist).mean(0), Q)
+    w2.close()
+    target = e_warm[c0 + K]
+    matched = next((c for c in sorted(e_cold) if e_cold[c] <= target), None)
+    warm_ms = c0 * cycle_ms + ws["infer_ms"] + K * cycle_ms
+    res = {"n": n, "seed": seed, "drops_per_cycle": drops, "c0": c0, "K": K, "W": W,
+           "ref_window": list(ref), "cycle_ms": cycle_ms,
+           "gate": {k: v for k, v in ws.items()},
+           "e_warm": e_warm,
+           "e_cold": {c: e_cold[c] for c in sorted(e_cold) if c % 10 == 0 or c == c0 + K},
+           "cold_cycles_to_match": matched,
+           "warm_cycles": c0 + K, "warm_ms": warm_ms,
+           "cold_ms_to_match": None if matched is None else matched * cycle_ms,
+           "speedup": None if matched is None else matched * cycle_ms / warm_ms}
+    log(json.dumps(res))
+    return res
+
+
+def save_weights(net, path=WEIGHTS):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez_compressed(path, **net.state_dict())
+
+
+def load_weights(net, path=WEIGHTS):
+    with np.load(path, allow_pickle=False) as z:
+        net.load_state_dict({k: z[k] for k in z.files})
+    return net
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description="surrogate data, training and warm start")
-    ap.add_argument("--probe-steady", action="store_true")
-    ap.add_argument("--terrains", type=int, default=512)
-    ap.add_argument("--val", type=int, default=64)
-    ap.add_argument("--cycles", type=int, default=80)
-    ap.add_argument("--avg", type=int, default=20)
-    ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--maps", type=int, default=32, help="training maps")
+    ap.add_argument("--val-maps", type=int, default=2)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--c0", type=int, default=10)
+    ap.add_argument("--mean-over", type=int, nargs=2, default=(100, 200))
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--out", default="")
+    ap.add_argument("--eval-size", type=int, default=2048)
+    ap.add_argument("--eval-seed", type=int, default=7)
+    ap.add_argument("--ref", type=int, nargs=2, default=(200, 400))
+    ap.add_argument("--out", default="", help="write the trained weights here")
+    ap.add_argument("--load", default="", help="skip training, load weights")
+    ap.add_argument("--timing-only", action="store_true")
     args = ap.parse_args(argv)
     import torch
 
     def log(s):
         print(s, flush=True)
 
-    if args.probe_steady:
-        for c, a, b in steady_probe(log=log):
-            if c % 5 == 0 or c < 10:
-                log(f"cycle {c:4d}  dL1(d) {a:.5f}  dL1(erf) {b:.5f}")
-        return
-    xs, ys, ginfo = generate(args.terrains + args.val, cycles=args.cycles, avg=args.avg, log=log)
-    xv, yv = xs[args.terrains:], ys[args.terrains:]
-    xs, ys = xs[:args.terrains], ys[:args.terrains]
     net = sg.UNet(seed=0)
-    log(json.dumps({"params": net.n_params(), "gen": ginfo}))
-    log(json.dumps(conv_timing(net, batch=args.batch)))
-    hist = train(net, xs, ys, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log,
-                 val=(xv, yv))
-    res = {"gen": ginfo, "history": hist, "val_discharge_l1": discharge_error(net, xv, yv)}
-    log(json.dumps(res))
-    if args.out:
-        np.savez(args.out, **net.state_dict())
+    log(json.dumps({"params": net.n_params()}))
+    log(json.dumps(train_timing(net, batch=args.batch)))
+    if args.timing_only:
+        return
+    res = {}
+    if args.load:
+        load_weights(net, args.load)
+    else:
+        t0, t1 = args.mean_over
+        states, targets, ginfo = generate(args.maps + args.val_maps, args.size, args.c0, t0, t1,
+                                          log=log)
+        res["data"] = ginfo
+        feats = map_features(states[:args.maps])
+        tenc = sg.encode_flow(targets[..., 0], targets[..., 1], targets[..., 2])
+        net = sg.UNet(seed=0)
+        hist, secs = train(net, feats, tenc[:args.maps], steps=args.steps, batch=args.batch,
+                           lr=args.lr, log=log)
+        res["train"] = {"steps": args.steps, "batch": args.batch, "seconds": secs,
+                        "final_loss": hist[-1]["loss"]}
+        del feats, tenc
+        vs, vt = states[args.maps:], targets[args.maps:]
+        res["val_erf_discharge_l1"] = validate(net, vs, vt)
+        # what the plain state at c0 alone gives (no network)
+        res["val_state_c0_l1"] = [l1_rel(erf_discharge(s[..., 1]), erf_discharge(t[..., 0]))
+                                  for s, t in zip(vs, vt)]
+        log(json.dumps(res))
+        del states, targets
+        if args.out:
+            save_weights(net, args.out)
+    torch.cuda.empty_cache()
+    res["warm_start"] = evaluate(net, n=args.eval_size, seed=args.eval_seed, c0=args.c0,
+                                 ref=tuple(args.ref), log=log)
     return res
 
 

@@ -121,19 +121,24 @@ def test_cpp_facade_demo_matches_python_path(shy):
     import subprocess
     demo = os.path.join(os.path.dirname(shy.LIB_PATH), "world_demo")
     n, cycles, drops, seed = 64, 6, 512, 3
-    r = subprocess.run([demo, str(n), str(cycles), str(drops), str(seed)], capture_output=True,
-                       text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    out = json.loads(r.stdout)
-    w = shy.World(n, n, seed=seed)
-    w.generate_fbm(seed, 8)
-    for _ in range(cycles):
-        w.erode(drops)
-    assert out["hash"] == fnv_hash(w.field("height"), w.field("discharge"))
-    assert out["particle_steps"] == w.stats()["steps"]
-    drop = w.drop((n // 2 + 0.5, n // 3 + 0.5))
-    steps = 0
-    while drop.descend():
-        steps += 1
-    assert out["drop_steps"] == steps
-    w.close()
+    for terrain in ("reference", "fbm"):
+        print("demo", terrain, flush=True)
+        r = subprocess.run([demo, str(n), str(cycles), str(drops), str(seed), terrain],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out = json.loads(r.stdout)
+        w = shy.World(n, n, seed=seed)
+        if terrain == "fbm":
+            w.generate_fbm(seed, 8)
+        else:
+            w.generate_reference(seed, 512)
+        for _ in range(cycles):
+            w.erode(drops)
+        assert out["hash"] == fnv_hash(w.field("height"), w.field("discharge"))
+        assert out["particle_steps"] == w.stats()["steps"]
+        drop = w.drop((n // 2 + 0.5, n // 3 + 0.5))
+        steps = 0
+        while drop.descend():
+            steps += 1
+        assert out["drop_steps"] == steps
+        w.close()

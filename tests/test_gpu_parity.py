@@ -42,6 +42,19 @@ def test_fbm_bit_exact(shy, orc):
         w.close()
 
 
+def test_reference_terrain_bit_exact(shy, orc):
+    """quad::map::init's OpenSimplex2 terrain on the device against the oracle
+    (itself pinned against the reference's FastNoiseLite build)."""
+    for nx, ny, seed, tr in ((512, 512, 1, 512), (300, 200, 4242, 512), (64, 96, -3, 64)):
+        say("reference terrain", nx, ny, seed, tr)
+        w = shy.World(nx, ny, seed=1)
+        w.generate_reference(seed, tr)
+        g = w.field("height")
+        o = orc.fnl_world(nx, ny, seed, tr)
+        assert np.array_equal(bits(g), bits(o)), (nx, ny, seed, np.abs(g - o).max())
+        w.close()
+
+
 def test_erf_field_bit_exact(shy, orc):
     rng = np.random.default_rng(5)
     w = shy.World(128, 96)

@@ -331,6 +331,49 @@ def test_fbm_generator_properties():
         assert sorted(O.fbm_perm(1, o).tolist()) == list(range(256))
 
 
+def test_fnl_gradient_set_structure():
+    """FastNoiseLite's 3D gradient set (FastNoiseLite.h:2529-2547): the 12
+    cube edges five times over, then edges 8, 1, 9, 3; every edge has one
+    zero and two unit components and the 12 are distinct."""
+    edges = [O.lib().orc_os2_edge(e) for e in range(64)]
+    assert edges[:60] == list(range(12)) * 5 and edges[60:] == [8, 1, 9, 3]
+
+
+@pytest.mark.skipif(O.ref_fnl() is None, reason="oracle/_ref not built (needs the reference "
+                                                 "sources: make -C oracle ref)")
+def test_fnl_restatement_matches_reference_build():
+    """The C restatement of quad::map::init's noise against the reference's own
+    FastNoiseLite.h compiled in place: bit-exact noise samples over four
+    coordinate scales and three frequencies, and whole normalised worlds
+    (including a negative seed and a non-square map)."""
+    R = O.ref_fnl()
+    rng = np.random.default_rng(11)
+    for scale in (1.0, 37.0, 1e4):
+        pts = (rng.standard_normal((1500, 3)) * scale).astype(np.float32)
+        for f in (1.0, 0.01, 64.0):
+            for x, y, z in pts:
+                a = np.float32(O.fnl_noise3(f, float(x), float(y), float(z)))
+                b = np.float32(R.fnl_ref_noise3(f, float(x), float(y), float(z)))
+                assert a.view(np.int32) == b.view(np.int32), (f, x, y, z, a, b)
+    for nx, ny, seed, tr in ((96, 64, 1, 512), (128, 128, 123456, 512), (80, 120, -77, 64)):
+        a = O.fnl_world(nx, ny, seed, tr)
+        b = np.empty(nx * ny, np.float32)
+        R.fnl_ref_world(nx, ny, seed, tr, b)
+        assert np.array_equal(a.view(np.int32), b.reshape(nx, ny).view(np.int32)), (nx, ny, seed)
+
+
+def test_fnl_world_properties():
+    """Normalisation with min <= 0 <= max: the result spans [0, 1] exactly,
+    the seed enters only through seed % 10000, and the noise is bounded."""
+    a = O.fnl_world(64, 64, 5)
+    assert a.min() >= 0.0 and a.max() <= 1.0 and (a.min() == 0.0 or a.max() == 1.0)
+    assert np.array_equal(a, O.fnl_world(64, 64, 10005))
+    assert not np.array_equal(a, O.fnl_world(64, 64, 6))
+    rng = np.random.default_rng(2)
+    v = [O.fnl_noise3(1.0, *map(float, rng.uniform(-50, 50, 3))) for _ in range(3000)]
+    assert max(abs(x) for x in v) <= 1.0
+
+
 def _hyps(h):
     return np.quantile(h, np.linspace(0, 1, 21))
 
