@@ -21,6 +21,12 @@ same command, read from profiles/hbm_traffic.json when it holds an entry for
 this workload, else null.  CPU baseline: the oracle's sequential restatement
 of World::erode on the same terrain and spawn stream, one core, on a bounded
 sample of cycles.
+
+The second half of the metric, time-to-steady discharge with the surrogate
+warm start vs the plain GPU method, is measured on a 2048x2048 fixed fBm
+terrain (262144 drops/cycle) with the committed trained network and reported
+in the same line under "time_to_steady", with the network's MFMA rate under
+"surrogate_roofline" (--no-surrogate skips both).
 """
 import argparse
 import json
@@ -147,6 +153,36 @@ def run_ensemble(shy, args) -> dict:
     }
 
 
+def surrogate_leg(seed: int, size: int) -> dict:
+    """Time-to-steady discharge, warm-started (trained network, committed
+    weights) vs cold plain GPU run on a fixed fBm terrain of side ``size``;
+    plus the MFMA rate of the network's convolutions (forward and one
+    training step on a batch of 16 windows of 256x256)."""
+    from simplehydrology_amd import surrogate as sg
+    from simplehydrology_amd import train_surrogate as ts
+    net = ts.load_weights(sg.UNet(seed=0), ts.WEIGHTS)
+    tt = ts.train_timing(sg.UNet(seed=0), batch=16)
+    r = ts.evaluate(net, n=size, seed=seed, c0=10, K=20, W=10, ref=(200, 400), warm_cycles=0,
+                    log=lambda *_: None)
+    fwd = tt["forward_tflops"]
+    return {
+        "time_to_steady": {
+            "definition": "wall time until the trailing 10-cycle mean of erf(0.4 d) reaches "
+                          "the warm run's relative L1 error (vs the cold run's mean over "
+                          "cycles 200-400) after its 20 verification cycles; warm = 10 plain "
+                          "cycles + window-wise inference + 20 verification cycles",
+            "terrain": f"{size}x{size} fBm seed {seed}, deposition 0, {size * size // 16} "
+                       "drops/cycle",
+            "matched_l1": r["e_warm"][30], "cold_cycles": r["cold_cycles_to_match"],
+            "cold_ms": r["cold_ms_to_match"], "warm_ms": r["warm_ms"],
+            "speedup": r["speedup"], "gate": r["gate"], "cycle_ms": r["cycle_ms"]},
+        "surrogate_roofline": {
+            "bound": "mfma", "kernel": "conv_fwd_kernel (U-Net forward, batch 16 x 256^2)",
+            "achieved": fwd, "peak": 2500.0, "unit": "TFLOP/s", "frac": fwd / 2500.0,
+            "train_step_tflops": tt["train_tflops"], "forward_gflop": tt["forward_gflop"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +199,9 @@ def main():
                          "launch per kernel")
     ap.add_argument("--terrains", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the time-to-steady (surrogate vs plain) leg")
+    ap.add_argument("--surrogate-size", type=int, default=2048)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -230,6 +269,8 @@ def main():
         "roofline": roofline(tm, args.steps, steps, f"single{n}_drops{drops}"),
     }
     w.close()
+    if not args.no_surrogate:
+        out.update(surrogate_leg(7, args.surrogate_size))
     if not args.no_cpu:
         cb = cpu_baseline(n, drops, args.seed, args.cpu_cycles)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -98,6 +98,14 @@ def lib():
     """Load ``libshy.so``; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch's ROCm build ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7); the first one loaded serves the whole process,
+        # and torch does not run on a newer one loaded before it.  Load
+        # torch's first when it is installed, so both share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
                                "or python -m simplehydrology_amd.build")

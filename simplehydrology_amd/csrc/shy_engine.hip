@@ -8,8 +8,9 @@
 //   root        float   root density
 //   visit[2]    uint8   cascade sites, ping-pong
 //   count       int32   walk steps per start cell (tickets); cleared by reduce
-//   offset      int32   bucket start per cell (prefix sum of count)
-//   scale       float   burst scale per cell
+//   first       int32   drops whose first step starts at the cell; cleared by
+//                       the scan
+//   osc         int2    {bucket start (prefix sum of count), burst scale} per cell
 //   dh          int32   summed walk commits per cell, 2^-20 (written by reduce)
 //   trk[3]      int64   summed tracks per cell, 2^-24 (written by reduce; stay
 //                       readable until the next cycle, as the reference's
@@ -17,7 +18,9 @@
 //   deposit     int32   terminal deposits, 2^-20; cleared by reduce
 // and per wave (sized by total drops x (max maxAge + 1) steps):
 //   ticket      int32   [step][drop] ticket log of the count pass
+//   slog        int4    [step][drop] step log of the count pass
 //   rec         int4    per-step commit records, bucketed by start cell
+//   dlog        int4    [drop] how each walk ended
 // A device table of Terrain entries maps drops, cells and cascade tiles to
 // their terrain, so every kernel covers the whole ensemble in one launch.
 // Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle
@@ -49,8 +52,8 @@ struct shy_ctx {
   float* root = nullptr;
   uint8_t* visit[2] = {nullptr, nullptr};
   int32_t* count = nullptr;
-  int32_t* offset = nullptr;
-  float* scale = nullptr;
+  int32_t* first = nullptr;
+  int2* osc = nullptr;
   int32_t* dh = nullptr;
   long long* trk = nullptr;  // 3*n: d, mx, my
   int32_t* deposit = nullptr;
@@ -221,9 +224,9 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.fld = c->fld;
   w.root = c->root;
   w.count = c->count;
+  w.first = c->first;
   w.ticket = c->ticket;
-  w.offset = c->offset;
-  w.scale = nullptr;
+  w.osc = c->osc;
   w.rec = c->rec;
   w.slog = c->slog;
   w.dlog = c->dlog;
@@ -264,20 +267,20 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (walk) {
     scan_sums_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, n, c->bsum);
     scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->bsum, nb);
-    scan_cells_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, c->bsum, n, c->offset, c->scale,
-                                                       c->fld, c->root, c->d_ter, nter);
+    scan_cells_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, c->bsum, n, c->osc, c->first,
+                                                       c->visit[p], c->fld, c->root, c->d_ter,
+                                                       nter);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   if (walk) {
-    w.scale = c->scale;
     replay_kernel<<<(c->total_drops + kReplayBlock - 1) / kReplayBlock, kReplayBlock, 0,
                     c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   reduce_kernel<<<(n + kReduceCells - 1) / kReduceCells, kReduceCells, 0, c->stream>>>(
-      c->count, c->offset, c->rec, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
+      c->count, c->osc, c->rec, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   HIPCHK(c, hipGetLastError());
   CascArgs a;
   const int q = p ^ 1;
@@ -444,7 +447,7 @@ int shy_destroy(shy_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
-                  c->offset, c->scale, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
+                  c->first, c->osc, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
                   c->slog, c->dlog, c->tsteps, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)
@@ -507,8 +510,8 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->count, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc count");
-  if ((e = hipMalloc(&c->offset, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc offset");
-  if ((e = hipMalloc(&c->scale, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc scale");
+  if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
+  if ((e = hipMalloc(&c->osc, n * sizeof(int2))) != hipSuccess) return bail(e, "hipMalloc osc");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");
@@ -527,7 +530,8 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   hipMemsetAsync(c->fld, 0, n * sizeof(float4), c->stream);
   hipMemsetAsync(c->root, 0, n * sizeof(float), c->stream);
   hipMemsetAsync(c->count, 0, n * sizeof(int32_t), c->stream);
-  hipMemsetAsync(c->offset, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->first, 0, n * sizeof(int32_t), c->stream);
+  hipMemsetAsync(c->osc, 0, n * sizeof(int2), c->stream);
   hipMemsetAsync(c->dh, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
   hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);

@@ -3,15 +3,19 @@
 // One cycle (= one World::erode call, source/world.h:54-88) is one wave of
 // drops that all read the map as it stood when the cycle began:
 //
-//  walk_kernel<kCount>   one drop per lane walks its trajectory; for every
+//  walk_kernel           one drop per lane walks its trajectory; for every
 //                        step it takes a ticket (relaxed int32 atomic with
 //                        return) on the step's start cell and logs, coalesced
 //                        as [step][drop], the ticket and the step's
 //                        snapshot-only quantities {start cell, c_eq, tracks};
-//                        it marks the cascade sites (water.h:151) and logs
-//                        how each drop ended.
+//                        per drop it logs how the walk ended, counts its
+//                        first step's cell and marks its end cell.  The
+//                        cascade sites (arrival cells, water.h:151) follow
+//                        per cell: a cell is a site iff more steps start there
+//                        than first steps, or a walk ended there.
 //  scan_*_kernel         exclusive prefix sum of the per-cell step counts
-//                        (bucket offsets), fused with the per-cell burst scale
+//                        (bucket offsets), the cascade-site marks, fused with
+//                        the per-cell burst scale
 //                        s = (1-(1-gamma)^k)/(gamma k), gamma = min(1, effD (1 +
 //                        entrainment erf(0.4 d))): the k drops through one cell
 //                        on the same snapshot commit together what k drops in
@@ -78,9 +82,9 @@ struct WaveArgs {
   const float4* __restrict__ fld;
   const float* __restrict__ root;
   int32_t* count;
+  int32_t* first;     // drops whose first step starts at the cell
   int32_t* ticket;
-  const int32_t* __restrict__ offset;
-  const float* __restrict__ scale;
+  const int2* __restrict__ osc;  // {bucket offset, burst-scale bits} per cell
   int4* rec;
   int4* slog;         // [step][drop] {start cell, c_eq bits, qt(vol sx), qt(vol sy)}
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
@@ -268,7 +272,6 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       *end_cell = i0;
       break;
     }
-    if (COUNT) a.visit[j] = 1;  // water.h:151 -> cascade site
     ++age;
     c = cn;
     ix = jx; iy = jy; i0 = j;
@@ -298,7 +301,6 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.count = w.count + T.cell_off;
       a.ticket = w.ticket;
       a.slog = w.slog;
-      a.visit = w.visit + T.cell_off;
       a.nx = T.nx;
       a.ny = T.ny;
       a.drops = w.total_drops;  // log stride
@@ -314,6 +316,13 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
                                  nullptr);
         dl.y = end;
         dl.z = end_cell;
+        if (dl.x > 0) {
+          // arrivals = steps past the first; a walk that ended after a step
+          // ended on its last arrival (water.h:151)
+          __hip_atomic_fetch_add(&w.first[T.cell_off + cid], 1, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+          if (end != kEndExit) w.visit[T.cell_off + end_cell] = 1;
+        }
       }
     }
     w.dlog[g] = dl;
@@ -325,15 +334,13 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
 // quantised amount q = quant_h(effD cdiff s) moves from the map into the
 // drop, then evaporation (water.h:128-136); the terminal deposit or the
 // export ends the drop (water.h:74-82, 139-142).
-struct ReplayStep {
-  int4 L;
-  int tk;
-  int off;
-  float s;
-  float root;
-};
+// Steps of the log in flight per lane: the log row of step k+D and the
+// per-cell gathers of step k+D-1 are issued while step k computes.  One wave
+// per SIMD is all a 65536-drop wave fills, so the ring is the latency hiding.
+constexpr int kReplayDepth = 8;
 
 __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
+  constexpr int D = kReplayDepth;
   LaneStats S;
   long long walked = 0, skipped = 0;
   const size_t stride = (size_t)w.total_drops;
@@ -348,55 +355,66 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
     if (dl.y == kEndSkipped) skipped += 1;
     if (dl.y <= kEndExit) {
       walked += 1;
-      const int32_t* __restrict__ offset = w.offset + T.cell_off;
-      const float* __restrict__ scale = w.scale + T.cell_off;
+      const int2* __restrict__ osc = w.osc + T.cell_off;
       const float* __restrict__ root = T.has_root ? w.root + T.cell_off : nullptr;
       const float dep = T.P.depositionRate, e = T.P.evapRate;
       float sed = 0.0f, vol = 1.0f;
       double gain = 0.0;
       long long wdh = 0;
-      // two-deep prefetch: the log of step k+2 and the gathers of step k+1
-      // are in flight while step k computes
-      // (rows k >= n of the log are not this drop's: never read them)
-      ReplayStep cur{}, nxt{};
-      int4 L2 = make_int4(0, 0, 0, 0);
-      int tk2 = 0;
-      if (n > 0) {
-        cur.L = w.slog[g];
-        cur.tk = w.ticket[g];
-        cur.off = offset[cur.L.x];
-        cur.s = scale[cur.L.x];
-        cur.root = root ? root[cur.L.x] : 0.0f;
-      }
-      if (n > 1) {
-        nxt.L = w.slog[stride + g];
-        nxt.tk = w.ticket[stride + g];
-      }
-      for (int k = 0; k < n; ++k) {
-        if (k + 2 < n) {
-          const size_t kl = (size_t)(k + 2) * stride + g;
-          L2 = w.slog[kl];
-          tk2 = w.ticket[kl];
+      // ring slot u holds step k = k0 + u; rows k >= n are never read
+      int4 L[D];
+      int tk[D], off[D];
+      float sc[D], rt[D];
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        L[u] = make_int4(0, 0, 0, 0);
+        tk[u] = 0;
+        off[u] = 0;
+        sc[u] = 0.0f;
+        rt[u] = 0.0f;
+        if (u < n) {
+          L[u] = w.slog[(size_t)u * stride + g];
+          tk[u] = w.ticket[(size_t)u * stride + g];
         }
-        if (k + 1 < n) {
-          nxt.off = offset[nxt.L.x];
-          nxt.s = scale[nxt.L.x];
-          nxt.root = root ? root[nxt.L.x] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u + 1 < D; ++u) {
+        if (u < n) {
+          const int2 o = osc[L[u].x];
+          off[u] = o.x;
+          sc[u] = __int_as_float(o.y);
+          rt[u] = root ? root[L[u].x] : 0.0f;
         }
-        float effD = dep * (1.0f - cur.root);  // water.h:86-87
-        if (effD < 0) effD = 0;
-        const float cdiff = __int_as_float(cur.L.y) - sed;
-        const int32_t q = quant_h((effD * cdiff) * cur.s);
-        w.rec[cur.off + cur.tk] = make_int4(-q, quant_t(vol), cur.L.z, cur.L.w);
-        wdh -= q;
-        sed += (float)q * kHQInv;
-        const float s1 = evap_sed(sed, e);
-        gain += (double)s1 - (double)sed;
-        sed = s1;
-        vol = evap_vol(vol, e);
-        cur = nxt;
-        nxt.L = L2;
-        nxt.tk = tk2;
+      }
+      for (int k0 = 0; k0 < n; k0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+          const int k = k0 + u;
+          if (k < n) {
+            float effD = dep * (1.0f - rt[u]);  // water.h:86-87
+            if (effD < 0) effD = 0;
+            const float cdiff = __int_as_float(L[u].y) - sed;
+            const int32_t q = quant_h((effD * cdiff) * sc[u]);
+            w.rec[off[u] + tk[u]] = make_int4(-q, quant_t(vol), L[u].z, L[u].w);
+            wdh -= q;
+            sed += (float)q * kHQInv;
+            const float s1 = evap_sed(sed, e);
+            gain += (double)s1 - (double)sed;
+            sed = s1;
+            vol = evap_vol(vol, e);
+            if (k + D < n) {  // slot u now takes step k + D
+              L[u] = w.slog[(size_t)(k + D) * stride + g];
+              tk[u] = w.ticket[(size_t)(k + D) * stride + g];
+            }
+            const int v = (u + D - 1) % D;  // slot of step k + D - 1 (constant once unrolled)
+            if (k + D - 1 < n) {
+              const int2 o = osc[L[v].x];
+              off[v] = o.x;
+              sc[v] = __int_as_float(o.y);
+              rt[v] = root ? root[L[v].x] : 0.0f;
+            }
+          }
+        }
       }
       S.steps += n;
       if (dl.y == kEndExit) {
@@ -493,10 +511,11 @@ __global__ void __launch_bounds__(kScanBlock) scan_top_kernel(int32_t* bsum, int
   }
 }
 
-// bucket offsets and burst scale per cell
+// bucket offset and burst scale per cell; cascade sites where more steps
+// start than first steps (an arrival, water.h:151); clears the first counts
 __global__ void __launch_bounds__(kScanBlock) scan_cells_kernel(
-    const int32_t* __restrict__ count, const int32_t* __restrict__ bsum, size_t n,
-    int32_t* offset, float* scale, const float4* __restrict__ fld,
+    const int32_t* __restrict__ count, const int32_t* __restrict__ bsum, size_t n, int2* osc,
+    int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
     const float* __restrict__ root, const Terrain* __restrict__ ter, int nter) {
   __shared__ int tmp[kScanBlock];
   const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
@@ -504,12 +523,15 @@ __global__ void __launch_bounds__(kScanBlock) scan_cells_kernel(
   int total;
   const int ex = block_exclusive_scan(k, tmp, total);
   if (i >= n) return;
-  offset[i] = bsum[blockIdx.x] + ex;
   const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
   const shy_params& P = ter[t].P;
   float effD = P.depositionRate * (1.0f - (ter[t].has_root ? root[i] : 0.0f));
   if (effD < 0) effD = 0;
-  scale[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
+  osc[i] = make_int2(bsum[blockIdx.x] + ex,
+                     __float_as_int(burst_scale(k, effD, P.entrainment, fld[i].w)));
+  const int f = first[i];
+  if (k > f) visit[i] = 1;
+  if (f) first[i] = 0;
 }
 
 // Per cell: sum its bucket and terminal deposits; clear the tickets.  The
@@ -522,7 +544,7 @@ constexpr int kReduceCells = 256;
 constexpr int kReduceChunk = 2048;  // records per LDS chunk (32 KiB)
 
 __global__ void __launch_bounds__(kReduceCells) reduce_kernel(
-    int32_t* count, const int32_t* __restrict__ offset, const int4* __restrict__ rec,
+    int32_t* count, const int2* __restrict__ osc, const int4* __restrict__ rec,
     int32_t* deposit, size_t n, int32_t* dh, long long* td, long long* tmx, long long* tmy) {
   __shared__ int off[kReduceCells + 1];
   __shared__ unsigned long long acc[4][kReduceCells];
@@ -530,8 +552,8 @@ __global__ void __launch_bounds__(kReduceCells) reduce_kernel(
   const int t = threadIdx.x;
   const size_t c0 = (size_t)blockIdx.x * kReduceCells;
   const int m = (int)((n - c0) < (size_t)kReduceCells ? (n - c0) : (size_t)kReduceCells);
-  if (t < m) off[t] = offset[c0 + t];
-  if (t == m - 1) off[m] = offset[c0 + t] + count[c0 + t];
+  if (t < m) off[t] = osc[c0 + t].x;
+  if (t == m - 1) off[m] = osc[c0 + t].x + count[c0 + t];
 #pragma unroll
   for (int f = 0; f < 4; ++f) acc[f][t] = 0;
   __syncthreads();

@@ -58,6 +58,7 @@ def erf_discharge(d):
 
 
 def flow_world(n, seed, deposition=0.0, octaves=8, base_freq=0.0):
+    """A terrain with the deposition rate set to zero (flow only)."""
     p = Params()
     p.depositionRate = deposition
     w = World(n, n, seed=seed, params=p)
@@ -318,7 +319,7 @@ def evaluate(net, n=2048, seed=7, c0=10, K=20, W=10, ref=(200, 400), warm_cycles
     e_warm = {c0 + K: l1_rel(ws.pop("window_mean"), Q)}
     rec2 = Recorder(w2)
     hist = []
-    for c in range(c0 + K + 1, c0 + K + warm_cycles + 1):
+    for c in range(c0 + K + 1, c0 + K + warm_cycles + 1):  # the warm run's later curve
         w2.erode_cycles(1, drops, graph=False)
         hist.append(erf_discharge(rec2.grab()[..., 1]))
         if len(hist) > W:
@@ -364,7 +365,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-size", type=int, default=2048)
-    ap.add_argument("--eval-seed", type=int, default=7)
+    ap.add_argument("--eval-seeds", type=int, nargs="+", default=[7])
     ap.add_argument("--ref", type=int, nargs=2, default=(200, 400))
     ap.add_argument("--out", default="", help="write the trained weights here")
     ap.add_argument("--load", default="", help="skip training, load weights")
@@ -406,8 +407,11 @@ def main(argv=None):
         if args.out:
             save_weights(net, args.out)
     torch.cuda.empty_cache()
-    res["warm_start"] = evaluate(net, n=args.eval_size, seed=args.eval_seed, c0=args.c0,
-                                 ref=tuple(args.ref), log=log)
+    res["warm_start"] = [evaluate(net, n=args.eval_size, seed=s, c0=args.c0,
+                                  ref=tuple(args.ref), log=log) for s in args.eval_seeds]
+    sp = [r["speedup"] for r in res["warm_start"] if r["speedup"] is not None]
+    log(json.dumps({"speedups": sp, "mean_speedup": float(np.mean(sp)) if sp else None,
+                    "accepted": [r["gate"]["accepted"] for r in res["warm_start"]]}))
     return res
 
 

@@ -1,0 +1,86 @@
+"""GPU: what the trained surrogate gains, and the warm-start gate.
+
+The network (simplehydrology_amd/weights, trained by train_surrogate on 40
+simulator-made 2048x2048 flow maps) maps the state after 10 cold cycles to
+the long-run mean discharge/momentum.  These tests run on a held-out terrain
+of the training statistics (fBm base frequency 1/2048) at 1024x1024.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ts(shy):
+    from simplehydrology_amd import train_surrogate
+    return train_surrogate
+
+
+@pytest.fixture(scope="module")
+def trained(ts):
+    from simplehydrology_amd import surrogate
+    assert os.path.exists(ts.WEIGHTS), "trained weights missing"
+    return ts.load_weights(surrogate.UNet(seed=0), ts.WEIGHTS)
+
+
+N, SEED, C0 = 1024, 31337, 10
+
+
+def cold_run(ts, n=N, seed=SEED, c0=C0, t0=100, t1=200):
+    """state after c0 cycles, and the mean flow over [t0, t1)"""
+    w = ts.flow_world(n, seed, base_freq=1.0 / 2048)
+    drops = n * n // 16
+    rec = ts.Recorder(w)
+    w.erode_cycles(c0, drops, graph=True)
+    state = rec.grab().clone()
+    cells = w.cells()
+    w.erode_cycles(t0 - c0, drops, graph=True)
+    acc = torch.zeros(n, n, 3, device="cuda:0")
+    for _ in range(t1 - t0):
+        w.erode_cycles(1, drops, graph=False)
+        acc += rec.grab()[..., 1:4]
+    w.close()
+    return state, cells, acc / (t1 - t0)
+
+
+def test_trained_network_beats_the_state_it_starts_from(ts, trained):
+    """Discharge (as erf(0.4 d), node::discharge) predicted from the state
+    after 10 cycles is closer to the long-run mean than that state itself,
+    and an untrained network is far worse than both."""
+    from simplehydrology_amd import surrogate as sg
+    print("cold run", N, SEED, flush=True)
+    state, _, target = cold_run(ts)
+    T = ts.erf_discharge(target[..., 0])
+    d, _, _ = sg.predict_tiled(trained, *(state[..., k] for k in range(4)))
+    e_net = ts.l1_rel(ts.erf_discharge(d), T)
+    e_in = ts.l1_rel(ts.erf_discharge(state[..., 1]), T)
+    d0, _, _ = sg.predict_tiled(sg.UNet(seed=3), *(state[..., k] for k in range(4)))
+    e_raw = ts.l1_rel(ts.erf_discharge(d0), T)
+    print(f"L1 network {e_net:.4f}  input state {e_in:.4f}  untrained {e_raw:.4f}", flush=True)
+    assert e_net < 0.95 * e_in
+    assert e_raw > 1.5 * e_in
+
+
+def test_gate_accepts_trained_and_rejects_untrained_with_restore(ts, trained):
+    from simplehydrology_amd import surrogate as sg
+    drops = N * N // 16
+    for net, expect in ((trained, True), (sg.UNet(seed=3), False)):
+        print("warm start, trained" if expect else "warm start, untrained", flush=True)
+        w = ts.flow_world(N, SEED, base_freq=1.0 / 2048)
+        w.erode_cycles(C0, drops, graph=True)
+        before, cyc = w.cells(), w.cycle
+        r = ts.warm_start(net, w, drops, SEED, K=20, W=10)
+        print({k: v for k, v in r.items() if k != "window_mean"}, flush=True)
+        assert r["accepted"] == expect
+        if expect:
+            assert w.cycle == cyc + 20
+        else:
+            # rejected: the plain run's state before the import is back
+            assert w.cycle == cyc
+            assert np.array_equal(w.cells().view(np.uint32), before.view(np.uint32))
+        w.close()
