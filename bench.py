@@ -61,9 +61,13 @@ def roofline(tm: dict, cycles: int, steps: int, workload: str) -> dict:
             entry = json.load(f).get(workload, {}).get(dom)
         if entry:
             traffic = entry["bytes_per_launch"]
-    return {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": d["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
-            "bytes_per_step": d["bytes_per_step"], "kernels": per}
+    out = {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": d["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+           "bytes_per_step": d["bytes_per_step"], "kernels": per}
+    if traffic is not None:
+        out["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload's "
+                                 "default command, copy-calibrated: profiles/hbm_traffic.json")
+    return out
 
 
 def cpu_model() -> str:

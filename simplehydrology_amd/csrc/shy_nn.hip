@@ -215,25 +215,60 @@ __global__ void __launch_bounds__(64) conv_wgrad_kernel(WgradArgs a) {
       }
 }
 
-// dW = sum over splits in a fixed order (deterministic)
-__global__ void wgrad_reduce_kernel(const float* __restrict__ part, float* dw, int n, int splits) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// dW[i] = sum over splits of part[k][i] in a fixed order (deterministic):
+// 8 interleaved slices of k, each summed in ascending k, then the slice sums
+// in slice order.  32 outputs x 8 slices per block keep the loads coalesced
+// and give every output 8 independent chains.
+constexpr int kRedSlices = 8;
+
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restrict__ part,
+                                                           float* dw, int n, int splits) {
+  __shared__ float acc[kRedSlices][32];
+  const int e = threadIdx.x & 31, sl = threadIdx.x >> 5;
+  const int i = blockIdx.x * 32 + e;
   float s = 0.0f;
-  for (int k = 0; k < splits; ++k) s += part[(size_t)k * n + i];
-  dw[i] = s;
+  if (i < n)
+    for (int k = sl; k < splits; k += kRedSlices) s += part[(size_t)k * n + i];
+  acc[sl][e] = s;
+  __syncthreads();
+  if (sl == 0 && i < n) {
+    float t = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kRedSlices; ++q) t += acc[q][e];
+    dw[i] = t;
+  }
 }
 
-// db[co] = sum_p g[p][co]: per-block partial column sums, fixed order
-__global__ void bias_grad_kernel(const __bf16* __restrict__ g, float* part, int P, int C,
-                                 int rows_per_block) {
-  const int co = threadIdx.x;  // blockDim.x >= C
-  if (co >= C) return;
+// part[b][c] = sum of g[p][c] over block b's rows, in a fixed order: for
+// C <= 256 thread t covers channel t % C and every (256/C)-th row from row
+// t / C (coalesced: neighbouring threads read neighbouring channels), then
+// the row-group sums are added in group order.
+__global__ void __launch_bounds__(256) bias_grad_kernel(const __bf16* __restrict__ g, float* part,
+                                                        int P, int C, int rows_per_block) {
+  __shared__ float acc[256];
+  const int t = threadIdx.x;
   const int p0 = blockIdx.x * rows_per_block;
   const int p1 = min(P, p0 + rows_per_block);
-  float s = 0.0f;
-  for (int p = p0; p < p1; ++p) s += (float)g[(size_t)p * C + co];
-  part[(size_t)blockIdx.x * C + co] = s;
+  if (C <= 256) {
+    const int groups = 256 / C;
+    const int c = t % C, r = t / C;
+    float s = 0.0f;
+    if (r < groups)
+      for (int p = p0 + r; p < p1; p += groups) s += (float)g[(size_t)p * C + c];
+    acc[t] = s;
+    __syncthreads();
+    if (t < C) {
+      float u = 0.0f;
+      for (int q = 0; q < groups; ++q) u += acc[q * C + t];
+      part[(size_t)blockIdx.x * C + t] = u;
+    }
+  } else {
+    for (int c = t; c < C; c += 256) {
+      float s = 0.0f;
+      for (int p = p0; p < p1; ++p) s += (float)g[(size_t)p * C + c];
+      part[(size_t)blockIdx.x * C + c] = s;
+    }
+  }
 }
 
 __global__ void relu_bwd_kernel(const __bf16* __restrict__ y, const __bf16* __restrict__ g,
@@ -446,7 +481,7 @@ int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int 
   conv_wgrad_kernel<<<grid, 64, 0, s>>>(a);
   if (int r = nn_check("conv_wgrad")) return r;
   const int n = Cout * a.K;
-  wgrad_reduce_kernel<<<blocks(n), 256, 0, s>>>(part, dw, n, splits);
+  wgrad_reduce_kernel<<<(n + 31) / 32, 256, 0, s>>>(part, dw, n, splits);
   return nn_check("wgrad_reduce");
 }
 
@@ -456,10 +491,9 @@ int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P
   const int rows = (P + part_rows - 1) / part_rows;
   const int nb = (P + rows - 1) / rows;
   hipStream_t s = (hipStream_t)stream;
-  const int bs = (C + 63) / 64 * 64;
-  bias_grad_kernel<<<nb, bs, 0, s>>>((const __bf16*)g, part, P, C, rows);
+  bias_grad_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
   if (int r = nn_check("bias_grad")) return r;
-  wgrad_reduce_kernel<<<blocks(C), 256, 0, s>>>(part, db, C, nb);
+  wgrad_reduce_kernel<<<(C + 31) / 32, 256, 0, s>>>(part, db, C, nb);
   return nn_check("bias_grad_reduce");
 }
 

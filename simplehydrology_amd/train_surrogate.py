@@ -315,6 +315,11 @@ def evaluate(net, n=2048, seed=7, c0=10, K=20, W=10, ref=(200, 400), warm_cycles
     w2 = flow_world(n, seed)
     w2.upload_cells(state_c0)
     w2.set_seed(seed, c0)
+    # one untimed inference first: module load and the allocator's first
+    # allocations are paid once per process, not per warm start
+    st = Recorder(w2).grab()
+    sg.predict_tiled(net, st[..., 0], st[..., 1], st[..., 2], st[..., 3])
+    torch.cuda.synchronize()
     ws = warm_start(net, w2, drops, seed, K=K, W=W, gate_l1=gate_l1)
     e_warm = {c0 + K: l1_rel(ws.pop("window_mean"), Q)}
     rec2 = Recorder(w2)
