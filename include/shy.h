@@ -34,7 +34,8 @@ typedef enum shy_status {
   SHY_ERR_HIP = -2,      /* HIP runtime error */
   SHY_ERR_NODEV = -3,    /* no GPU visible */
   SHY_ERR_NONFINITE = -4,/* input map holds NaN or Inf */
-  SHY_ERR_STATE = -5     /* call not valid in the context's current state */
+  SHY_ERR_STATE = -5,    /* call not valid in the context's current state */
+  SHY_ERR_IO = -6        /* file could not be read or written, or is not a map dump */
 } shy_status;
 
 /* Parameter set: the reference's static members, same names and defaults.
@@ -176,6 +177,15 @@ int shy_download_cells(shy_ctx* ctx, float* cells8);
 int shy_upload_field(shy_ctx* ctx, int field, const float* src);
 int shy_download_field(shy_ctx* ctx, int field, float* dst);
 
+/* The same transfer in the reference's multi-tile pool order
+   (source/cellpool.h:171-176, 323-340, 421-426): a square map of side
+   mapsize*tileres is mapsize^2 nodes, node (i, j) at index i*mapsize + j
+   holding cells [i*tileres, (i+1)*tileres) x [j*tileres, (j+1)*tileres)
+   x-major.  The engine itself holds one seamless grid, so drops cross node
+   borders exactly as map.get routes them. */
+int shy_upload_cells_tiled(shy_ctx* ctx, int tileres, const float* pool8);
+int shy_download_cells_tiled(shy_ctx* ctx, int tileres, float* pool8);
+
 /* quad::map::normal (source/cellpool.h:181-204, 445-447) of every cell,
    computed on the device from the current heights: 3 floats (x, y, z) per
    cell, x-major. */
@@ -193,6 +203,35 @@ int shy_generate_fbm(shy_ctx* ctx, uint64_t seed, int octaves, float base_freq,
    (h - min)/(max - min) with min <= 0 <= max.  The reference's world is
    512x512 with tileres 512 (cellpool.h:167-176).  Writes heights only. */
 int shy_generate_reference(shy_ctx* ctx, int seed, int tileres);
+
+/* Render-facing outputs of the reference's frame loop, x-major, 4 floats per
+   cell (SimpleHydrology.cpp:341-354): dischargeMap = (water_rgb, erf(0.4
+   discharge)) and momentumMap = (0.5(1+erf(mx)), 0.5(1+erf(my)), 0.5, 1).
+   Either output may be NULL. */
+int shy_download_textures(shy_ctx* ctx, const float* water_rgb, float* discharge_rgba,
+                          float* momentum_rgba);
+/* updatenode's per-vertex mesh data (source/cellpool.h:286-304), 12 floats
+   per cell, x-major: position (x, 80 h, y), normal, tangent T-P, bitangent
+   B-P (T = the +x neighbour, B = the +y neighbour; heights off the map read
+   0, as quad::node::height does). */
+int shy_download_vertices(shy_ctx* ctx, float* vertex12);
+
+/* Map dumps: this header, then nx*ny cells of 8 floats in the quad::cell
+   order (source/cellpool.h:207-220), x-major, host byte order.  Load needs a
+   context of the same size; it restores the cells, the parameters and the
+   spawn stream (seed, cycle), so a loaded map continues bit for bit. */
+#define SHY_MAP_MAGIC "SHYMAP01"
+#define SHY_MAP_VERSION 1u
+typedef struct shy_map_header {
+  char magic[8];    /* SHY_MAP_MAGIC, no terminator */
+  uint32_t version; /* SHY_MAP_VERSION */
+  int32_t nx, ny;
+  uint32_t cycle;   /* the next World::erode call's cycle number */
+  uint64_t seed;
+  shy_params params;
+} shy_map_header;   /* 72 bytes */
+int shy_save_map(shy_ctx* ctx, const char* path);
+int shy_load_map(shy_ctx* ctx, const char* path);
 
 /* World::erode(cycles) (source/world.h:54-88): one cycle of `drops` spawns.
    Wave-synchronous: every drop reads the map as it stood at the start of the

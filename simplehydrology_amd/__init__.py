@@ -23,7 +23,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libshy.so")
 
-SHY_OK, SHY_ERR_ARG, SHY_ERR_HIP, SHY_ERR_NODEV, SHY_ERR_NONFINITE, SHY_ERR_STATE = 0, -1, -2, -3, -4, -5
+SHY_OK, SHY_ERR_ARG, SHY_ERR_HIP, SHY_ERR_NODEV, SHY_ERR_NONFINITE, SHY_ERR_STATE, SHY_ERR_IO = \
+    0, -1, -2, -3, -4, -5, -6
 FIELDS = ("height", "discharge", "momentumx", "momentumy", "discharge_track",
           "momentumx_track", "momentumy_track", "rootdensity", "erf_discharge")
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
@@ -52,6 +53,42 @@ class Params(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MapHeader(C.Structure):
+    """shy_map_header of include/shy.h (72 bytes)."""
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("nx", C.c_int32),
+                ("ny", C.c_int32), ("cycle", C.c_uint32), ("seed", C.c_uint64),
+                ("params", Params)]
+
+
+MAP_MAGIC, MAP_VERSION = b"SHYMAP01", 1
+
+
+def write_map(path: str, cells, params: Params | None = None, seed: int = 1, cycle: int = 0):
+    """Write a map dump (the format shy_save_map writes) from AoS cells (nx, ny, 8)."""
+    a = np.ascontiguousarray(cells, np.float32)
+    if a.ndim != 3 or a.shape[2] != 8:
+        raise ValueError("cells must be (nx, ny, 8)")
+    hd = MapHeader(MAP_MAGIC, MAP_VERSION, a.shape[0], a.shape[1], cycle, seed,
+                   params if params is not None else Params())
+    with open(path, "wb") as f:
+        f.write(bytes(hd))
+        f.write(a.tobytes())
+
+
+def read_map(path: str):
+    """(header, cells (nx, ny, 8) float32) of a map dump; no GPU needed."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    if len(raw) < C.sizeof(MapHeader):
+        raise ValueError(f"{path}: too short for a map dump")
+    hd = MapHeader.from_buffer_copy(raw[:C.sizeof(MapHeader)])
+    if hd.magic != MAP_MAGIC or hd.version != MAP_VERSION:
+        raise ValueError(f"{path}: not a map dump")
+    n = hd.nx * hd.ny * 8
+    body = np.frombuffer(raw, np.float32, count=n, offset=C.sizeof(MapHeader))
+    return hd, body.reshape(hd.nx, hd.ny, 8).copy()
 
 
 class DropStep(C.Structure):
@@ -126,6 +163,12 @@ def lib():
             "shy_download_field": (i32, [vp, i32, vp]),
             "shy_generate_fbm": (i32, [vp, u64, i32, C.c_float, C.c_float, C.c_float]),
             "shy_generate_reference": (i32, [vp, i32, i32]),
+            "shy_download_textures": (i32, [vp, vp, vp, vp]),
+            "shy_download_vertices": (i32, [vp, vp]),
+            "shy_save_map": (i32, [vp, C.c_char_p]),
+            "shy_upload_cells_tiled": (i32, [vp, i32, vp]),
+            "shy_download_cells_tiled": (i32, [vp, i32, vp]),
+            "shy_load_map": (i32, [vp, C.c_char_p]),
             "shy_terrain_generate_reference": (i32, [vp, i32, i32, i32]),
             "shy_erode": (i32, [vp, i32]),
             "shy_erode_cycles": (i32, [vp, i32, i32, i32]),
@@ -289,6 +332,40 @@ class World:
         out = np.empty((self.nx, self.ny, 3), np.float32)
         _check(self._ctx, lib().shy_download_normals(self._ctx, out.ctypes.data))
         return out
+
+    def textures(self, water_rgb=(0.2, 0.4, 1.0)):
+        """(dischargeMap, momentumMap) of the reference's frame loop, (nx, ny, 4) each."""
+        dis = np.empty((self.nx, self.ny, 4), np.float32)
+        mom = np.empty((self.nx, self.ny, 4), np.float32)
+        rgb = np.asarray(water_rgb, np.float32)
+        _check(self._ctx, lib().shy_download_textures(self._ctx, rgb.ctypes.data, dis.ctypes.data,
+                                                       mom.ctypes.data))
+        return dis, mom
+
+    def vertices(self) -> np.ndarray:
+        """updatenode's vertex data: (nx, ny, 4, 3) = position, normal, tangent, bitangent."""
+        out = np.empty((self.nx, self.ny, 4, 3), np.float32)
+        _check(self._ctx, lib().shy_download_vertices(self._ctx, out.ctypes.data))
+        return out
+
+    def save(self, path: str):
+        """Map dump (header + AoS cells, see shy.h); restores with ``load``."""
+        _check(self._ctx, lib().shy_save_map(self._ctx, os.fsencode(path)))
+
+    def load(self, path: str):
+        _check(self._ctx, lib().shy_load_map(self._ctx, os.fsencode(path)))
+
+    def cells_tiled(self, tileres: int = 512) -> np.ndarray:
+        """Cells in the reference's multi-tile pool order: (mapsize^2, tileres, tileres, 8)."""
+        m = self.nx // tileres
+        out = np.empty((m * m, tileres, tileres, 8), np.float32)
+        _check(self._ctx, lib().shy_download_cells_tiled(self._ctx, tileres, out.ctypes.data))
+        return out
+
+    def upload_cells_tiled(self, pool, tileres: int = 512):
+        m = self.nx // tileres
+        a = _f32(pool, (m * m, tileres, tileres, 8))
+        _check(self._ctx, lib().shy_upload_cells_tiled(self._ctx, tileres, a.ctypes.data))
 
     def upload_cells(self, cells):
         a = _f32(cells, (self.nx, self.ny, 8))

@@ -413,6 +413,36 @@ int time_cycles(shy_ctx* c, int cycles, float* ms) {
   return SHY_OK;
 }
 
+// device scratch of `bytes`, a kernel filling it, one copy to the host
+template <class Launch>
+int download_via(shy_ctx* c, const char* what, size_t bytes, void* host, Launch launch) {
+  void* d = nullptr;
+  HIPCHK(c, hipMalloc(&d, bytes));
+  launch(d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return SHY_OK;
+}
+
+// node-major pool order of a mapsize x mapsize node grid (cellpool.h:323-340,
+// 421-426): node (i, j) is index i*mapsize + j, its tileres^2 cells x-major
+int check_tiled(shy_ctx* c, int tileres, const char* what) {
+  const Terrain& T = c->ter[0];
+  if (tileres < 1 || T.nx != T.ny || T.nx % tileres)
+    return fail(c, SHY_ERR_ARG, std::string(what) +
+                                    ": the map must be square with a side a multiple of tileres");
+  return SHY_OK;
+}
+
+size_t tiled_index(int x, int y, int tileres, int mapsize) {
+  const int i = x / tileres, j = y / tileres;
+  const size_t node = (size_t)i * mapsize + j;
+  return node * (size_t)tileres * tileres + (size_t)(x % tileres) * tileres + (y % tileres);
+}
+
 }  // namespace
 
 extern "C" {
@@ -793,6 +823,118 @@ int shy_download_normals(shy_ctx* c, float* xyz) {
   hipFree(d);
   if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("shy_download_normals: ") + hipGetErrorString(e));
   return SHY_OK;
+}
+
+int shy_download_textures(shy_ctx* c, const float* water_rgb, float* discharge_rgba,
+                          float* momentum_rgba) {
+  RETURN_IF(need_single(c, "shy_download_textures"));
+  if (!discharge_rgba && !momentum_rgba) return SHY_OK;
+  if (discharge_rgba && !water_rgb)
+    return fail(c, SHY_ERR_ARG, "shy_download_textures: water_rgb is NULL");
+  hipSetDevice(c->device);
+  const size_t n = c->n, grid = (n + 255) / 256;
+  const float wr = water_rgb ? water_rgb[0] : 0.0f, wg = water_rgb ? water_rgb[1] : 0.0f,
+              wb = water_rgb ? water_rgb[2] : 0.0f;
+  if (discharge_rgba)
+    RETURN_IF(download_via(c, "shy_download_textures", n * sizeof(float4), discharge_rgba,
+                           [&](void* d) {
+                             textures_kernel<<<grid, 256, 0, c->stream>>>(
+                                 c->fld, n, wr, wg, wb, c->erf_a, (float4*)d, nullptr);
+                           }));
+  if (momentum_rgba)
+    RETURN_IF(download_via(c, "shy_download_textures", n * sizeof(float4), momentum_rgba,
+                           [&](void* d) {
+                             textures_kernel<<<grid, 256, 0, c->stream>>>(
+                                 c->fld, n, 0.0f, 0.0f, 0.0f, c->erf_a, nullptr, (float4*)d);
+                           }));
+  return SHY_OK;
+}
+
+int shy_download_vertices(shy_ctx* c, float* vertex12) {
+  RETURN_IF(need_single(c, "shy_download_vertices"));
+  if (!vertex12) return fail(c, SHY_ERR_ARG, "shy_download_vertices: NULL argument");
+  hipSetDevice(c->device);
+  const Terrain& T = c->ter[0];
+  const size_t grid = (c->n + 255) / 256;
+  return download_via(c, "shy_download_vertices", 12 * c->n * sizeof(float), vertex12,
+                      [&](void* d) {
+                        vertices_kernel<<<grid, 256, 0, c->stream>>>(c->h[parity(c)], T.nx, T.ny,
+                                                                    (float*)d);
+                      });
+}
+
+int shy_save_map(shy_ctx* c, const char* path) {
+  RETURN_IF(need_single(c, "shy_save_map"));
+  if (!path) return fail(c, SHY_ERR_ARG, "shy_save_map: NULL path");
+  shy_map_header hd;
+  std::memset(&hd, 0, sizeof(hd));
+  std::memcpy(hd.magic, SHY_MAP_MAGIC, 8);
+  hd.version = SHY_MAP_VERSION;
+  hd.nx = c->ter[0].nx;
+  hd.ny = c->ter[0].ny;
+  hd.cycle = c->cycle;
+  hd.seed = c->ter[0].seed;
+  hd.params = c->ter[0].P;
+  std::vector<float> cells(8 * c->n);
+  RETURN_IF(shy_download_cells(c, cells.data()));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, SHY_ERR_IO, std::string("shy_save_map: cannot open ") + path);
+  const bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1 &&
+                  std::fwrite(cells.data(), sizeof(float), cells.size(), f) == cells.size();
+  const bool closed = std::fclose(f) == 0;
+  if (!ok || !closed) return fail(c, SHY_ERR_IO, std::string("shy_save_map: write failed: ") + path);
+  return SHY_OK;
+}
+
+int shy_load_map(shy_ctx* c, const char* path) {
+  RETURN_IF(need_single(c, "shy_load_map"));
+  if (!path) return fail(c, SHY_ERR_ARG, "shy_load_map: NULL path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, SHY_ERR_IO, std::string("shy_load_map: cannot open ") + path);
+  shy_map_header hd;
+  std::vector<float> cells(8 * c->n);
+  bool ok = std::fread(&hd, sizeof(hd), 1, f) == 1;
+  if (ok && (std::memcmp(hd.magic, SHY_MAP_MAGIC, 8) != 0 || hd.version != SHY_MAP_VERSION)) {
+    std::fclose(f);
+    return fail(c, SHY_ERR_IO, std::string("shy_load_map: not a map dump: ") + path);
+  }
+  if (ok && (hd.nx != c->ter[0].nx || hd.ny != c->ter[0].ny)) {
+    std::fclose(f);
+    return fail(c, SHY_ERR_ARG, "shy_load_map: map size differs from the context's");
+  }
+  ok = ok && std::fread(cells.data(), sizeof(float), cells.size(), f) == cells.size();
+  std::fclose(f);
+  if (!ok) return fail(c, SHY_ERR_IO, std::string("shy_load_map: short file: ") + path);
+  RETURN_IF(shy_set_params(c, &hd.params));
+  RETURN_IF(shy_upload_cells(c, cells.data()));
+  return shy_set_seed(c, hd.seed, hd.cycle);
+}
+
+int shy_download_cells_tiled(shy_ctx* c, int tileres, float* pool8) {
+  RETURN_IF(need_single(c, "shy_download_cells_tiled"));
+  if (!pool8) return fail(c, SHY_ERR_ARG, "shy_download_cells_tiled: NULL argument");
+  RETURN_IF(check_tiled(c, tileres, "shy_download_cells_tiled"));
+  const int n = c->ter[0].nx, mapsize = n / tileres;
+  std::vector<float> cells(8 * c->n);
+  RETURN_IF(shy_download_cells(c, cells.data()));
+  for (int x = 0; x < n; ++x)
+    for (int y = 0; y < n; ++y)
+      std::memcpy(pool8 + 8 * tiled_index(x, y, tileres, mapsize),
+                  cells.data() + 8 * ((size_t)x * n + y), 8 * sizeof(float));
+  return SHY_OK;
+}
+
+int shy_upload_cells_tiled(shy_ctx* c, int tileres, const float* pool8) {
+  RETURN_IF(need_single(c, "shy_upload_cells_tiled"));
+  if (!pool8) return fail(c, SHY_ERR_ARG, "shy_upload_cells_tiled: NULL argument");
+  RETURN_IF(check_tiled(c, tileres, "shy_upload_cells_tiled"));
+  const int n = c->ter[0].nx, mapsize = n / tileres;
+  std::vector<float> cells(8 * c->n);
+  for (int x = 0; x < n; ++x)
+    for (int y = 0; y < n; ++y)
+      std::memcpy(cells.data() + 8 * ((size_t)x * n + y),
+                  pool8 + 8 * tiled_index(x, y, tileres, mapsize), 8 * sizeof(float));
+  return shy_upload_cells(c, cells.data());
 }
 
 int shy_upload_cells(shy_ctx* c, const float* cells8) {

@@ -855,6 +855,46 @@ __global__ void normals_kernel(const float* __restrict__ h, int nx, int ny, floa
   out[3 * i + 2] = n2;
 }
 
+// The textures the reference's frame loop builds after each erode call
+// (SimpleHydrology.cpp:341-354): dischargeMap = (water colour, erf(0.4 d)),
+// momentumMap = (0.5 (1 + erf(mx)), 0.5 (1 + erf(my)), 0.5, 1).
+__global__ void textures_kernel(const float4* __restrict__ fld, size_t n, float wr, float wg,
+                                float wb, const double* __restrict__ erf_a, float4* dis,
+                                float4* mom) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 f = fld[i];
+  if (dis) dis[i] = make_float4(wr, wg, wb, f.w);
+  if (mom)
+    mom[i] = make_float4(0.5f * (1.0f + erf_f32(erf_a, f.x)), 0.5f * (1.0f + erf_f32(erf_a, f.y)),
+                         0.5f, 1.0f);
+}
+
+// updatenode's vertex (cellpool.h:286-304): P = (x, 80 h(x,y), y), the
+// normal, T - P and B - P with T = (x+1, 80 h(x+1,y), y), B = (x, 80 h(x,y+1),
+// y+1); a height off the map reads 0 (quad::node::height).
+__global__ void vertices_kernel(const float* __restrict__ h, int nx, int ny, float* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int x = (int)(i / (size_t)ny), y = (int)(i % (size_t)ny);
+  CellView c;
+  c.h = h[i];
+  c.xm = h[x > 0 ? i - ny : i];
+  c.xp = h[x + 1 < nx ? i + ny : i];
+  c.ym = h[y > 0 ? i - 1 : i];
+  c.yp = h[y + 1 < ny ? i + 1 : i];
+  float n0, n1, n2;
+  normal_from(c, x, y, nx, ny, n0, n1, n2);
+  const float py = kMapScale * c.h;
+  const float ty = kMapScale * (x + 1 < nx ? c.xp : 0.0f);
+  const float by = kMapScale * (y + 1 < ny ? c.yp : 0.0f);
+  float* o = out + 12 * i;
+  o[0] = (float)x; o[1] = py; o[2] = (float)y;
+  o[3] = n0; o[4] = n1; o[5] = n2;
+  o[6] = (float)(x + 1) - (float)x; o[7] = ty - py; o[8] = (float)y - (float)y;
+  o[9] = (float)x - (float)x; o[10] = by - py; o[11] = (float)(y + 1) - (float)y;
+}
+
 __global__ void refresh_erf_kernel(float4* fld, size_t n, const double* __restrict__ erf_a) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

@@ -42,6 +42,27 @@ def test_struct_layouts_match_header():
     assert C.sizeof(shy.DropStep) == 40
     assert C.sizeof(shy.DropState) == 28
     assert C.sizeof(shy.Stats) == 80
+    assert C.sizeof(shy.MapHeader) == 72
+
+
+def test_map_dump_format_roundtrip(tmp_path):
+    """The dump format (header + AoS cells) written and read without a GPU."""
+    import numpy as np
+    import simplehydrology_amd as shy
+    rng = np.random.default_rng(0)
+    cells = rng.random((13, 7, 8), dtype=np.float32)
+    p = shy.Params(evapRate=0.002)
+    path = str(tmp_path / "m.shymap")
+    shy.write_map(path, cells, p, seed=99, cycle=17)
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"SHYMAP01" and len(raw) == 72 + cells.nbytes
+    hd, back = shy.read_map(path)
+    assert (hd.nx, hd.ny, hd.seed, hd.cycle) == (13, 7, 99, 17)
+    assert hd.params.evapRate == np.float32(0.002)
+    assert np.array_equal(back, cells)
+    open(path, "wb").write(b"NOTAMAP!" + raw[8:])
+    with pytest.raises(ValueError):
+        shy.read_map(path)
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK),
