@@ -54,6 +54,16 @@ typedef struct shy_params {
   float settling;         /* 0.8   */
 } shy_params;
 
+/* Plant parameters: the Plant statics of source/vegetation.h:37-41, same
+   names and defaults. */
+typedef struct shy_plant_params {
+  float maxSize;       /* 1.5  */
+  float growRate;      /* 0.05 */
+  float maxSteep;      /* 0.8  */
+  float maxDischarge;  /* 0.3  */
+  float maxTreeHeight; /* 0.8  */
+} shy_plant_params;
+
 /* Cell fields, in the reference's quad::cell order (source/cellpool.h:207-220). */
 typedef enum shy_field {
   SHY_HEIGHT = 0,
@@ -232,6 +242,25 @@ typedef struct shy_map_header {
 } shy_map_header;   /* 72 bytes */
 int shy_save_map(shy_ctx* ctx, const char* path);
 int shy_load_map(shy_ctx* ctx, const char* path);
+
+/* Vegetation (source/vegetation.h:44-186), the other writer of the erosion's
+   inputs: plants raise the root density around them (Plant::root), and the
+   walk's deposition rate falls with it (water.h:86-87).  The plant list lives
+   on the device.  One shy_vegetation_grow call is one Vegetation::grow(): a
+   spawn attempt at a random cell, then every plant grows, may die and may
+   seed a child within 4 cells; the children are visited in the same call, as
+   the reference's loop visits plants appended during it.  Each generation
+   decides against the map and root density as they stood when it began;
+   root changes are summed exactly in units of 0.2 and applied per
+   generation; draws come from counter hashes of (seed, step, plant, purpose),
+   not rand().  Plants beyond `capacity` are dropped and counted. */
+void shy_default_plant_params(shy_plant_params* p);
+int shy_vegetation_enable(shy_ctx* ctx, int capacity, const shy_plant_params* p); /* NULL = defaults */
+int shy_vegetation_grow(shy_ctx* ctx);
+int shy_vegetation_count(shy_ctx* ctx, int* plants, int64_t* dropped);
+/* the plant list in order: x, y, size per plant */
+int shy_vegetation_download(shy_ctx* ctx, float* xys);
+int shy_vegetation_upload(shy_ctx* ctx, const float* xys, int n);
 
 /* World::erode(cycles) (source/world.h:54-88): one cycle of `drops` spawns.
    Wave-synchronous: every drop reads the map as it stood at the start of the

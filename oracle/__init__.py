@@ -45,6 +45,19 @@ class Params(C.Structure):
         return p
 
 
+class PlantParams(C.Structure):
+    """Plant statics (vegetation.h:37-41)."""
+    _fields_ = [(n, C.c_float) for n in ("maxSize", "growRate", "maxSteep", "maxDischarge",
+                                         "maxTreeHeight")]
+
+    @classmethod
+    def default(cls, **kw):
+        p = cls(1.5, 0.05, 0.8, 0.3, 0.8)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
 class Step(C.Structure):
     _fields_ = [("age", C.c_int32), ("status", C.c_int32), ("pos", C.c_float * 2),
                 ("speed", C.c_float * 2), ("volume", C.c_float), ("sediment", C.c_float),
@@ -105,6 +118,10 @@ def lib():
         L.orc_fnl_noise3.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float]
         L.orc_fnl_noise3.restype = C.c_float
         L.orc_fnl_world.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, fp]
+        L.orc_veg_grow.argtypes = [fp, fp, fp, C.c_int, C.c_int, i32p, i32p, fp, C.c_int, C.c_int,
+                                   C.POINTER(PlantParams), C.c_uint64, C.c_uint32,
+                                   C.POINTER(C.c_int64)]
+        L.orc_veg_grow.restype = C.c_int
         L.orc_os2_edge.argtypes = [C.c_int]
         L.orc_os2_edge.restype = C.c_int
         L.orc_normal.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
@@ -161,6 +178,29 @@ def fbm(seed: int, nx: int, ny: int, octaves: int = 8, base_freq: float | None =
     out = np.empty(nx * ny, np.float32)
     lib().orc_fbm(seed, nx, ny, octaves, float(base_freq), lacunarity, gain, out)
     return out.reshape(nx, ny)
+
+
+def veg_grow(h, d, root, plants, seed: int, step: int, cap: int = 1 << 20,
+             params: PlantParams | None = None):
+    """One Vegetation::grow() (generation semantics). ``root`` is updated in
+    place; returns (plants (n, 3) x, y, size, dropped)."""
+    nx, ny = h.shape
+    px = np.zeros(cap, np.int32)
+    py = np.zeros(cap, np.int32)
+    sz = np.zeros(cap, np.float32)
+    n = plants.shape[0]
+    px[:n] = plants[:, 0]
+    py[:n] = plants[:, 1]
+    sz[:n] = plants[:, 2]
+    dropped = C.c_int64(0)
+    P = params if params is not None else PlantParams.default()
+    assert root.dtype == np.float32 and root.flags.c_contiguous
+    m = lib().orc_veg_grow(np.ascontiguousarray(h, np.float32).reshape(-1),
+                           np.ascontiguousarray(d, np.float32).reshape(-1), root.reshape(-1),
+                           nx, ny, px, py, sz, n, cap, C.byref(P), seed, step,
+                           C.byref(dropped))
+    return np.stack([px[:m].astype(np.float32), py[:m].astype(np.float32), sz[:m]], -1), \
+        dropped.value
 
 
 def fnl_noise3(freq: float, x: float, y: float, z: float) -> float:

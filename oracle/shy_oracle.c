@@ -303,6 +303,128 @@ void orc_fnl_world(int nx, int ny, int seed, int tileres, float* out) {
   for (size_t i = 0; i < n; ++i) out[i] = (out[i] - mn) / (mx - mn);
 }
 
+/* ------------------------------------------------------------ vegetation */
+/* Vegetation::grow (vegetation.h:120-186) with the device's generation
+   semantics (see include/shy.h): spawn attempt, then generations of plants
+   that grow (:59-61), may die (:63-70) and may seed a child (:144-184), each
+   generation deciding against the map and root density as they stood when it
+   began; root changes in integer units of 0.2 (Plant::root's 1.0 / 0.6 / 0.4,
+   :87-117) applied per generation; counter-hash draws. */
+static uint32_t veg_draw(uint64_t key, uint32_t slot, int purpose, uint32_t m) {
+  const uint64_t r = orc_mix64(key ^ (((uint64_t)slot << 8) | (uint64_t)purpose));
+  return (uint32_t)(((r >> 32) * (uint64_t)m) >> 32);
+}
+
+static void veg_root(int32_t* units, int nx, int ny, int x, int y, int sign) {
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int u = (dx == 0 && dy == 0) ? 5 : (dx == 0 || dy == 0) ? 3 : 2;
+      const int X = x + dx, Y = y + dy;
+      if (X < 0 || Y < 0 || X >= nx || Y >= ny) continue;
+      units[(size_t)X * ny + Y] += sign * u;
+    }
+}
+
+static void veg_apply(float* root, int32_t* units, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (units[i]) {
+      root[i] = root[i] + 0.2f * (float)units[i];
+      units[i] = 0;
+    }
+}
+
+int orc_veg_grow(const float* h, const float* d, float* root, int nx, int ny, int32_t* px,
+                 int32_t* py, float* size, int count, int cap, const orc_plant_params* P,
+                 uint64_t seed, uint32_t step, int64_t* dropped) {
+  const size_t n = (size_t)nx * ny;
+  int32_t* units = (int32_t*)calloc(n, sizeof(int32_t));
+  int32_t* alive = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
+  int32_t* cflag = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
+  int32_t* cx = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
+  int32_t* cy = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
+  for (int k = 0; k < cap; ++k) alive[k] = 1;
+  const uint64_t key = orc_cycle_key(seed, 0xffffffffu, step);
+  float nrm[3];
+  { /* spawn attempt (:124-136) */
+    const int x = (int)veg_draw(key, 0xffffffffu, 0, (uint32_t)nx);
+    const int y = (int)veg_draw(key, 0xffffffffu, 1, (uint32_t)ny);
+    const size_t i = (size_t)x * ny + y;
+    orc_normal(h, nx, ny, x, y, nrm);
+    if (orc_erf(0.4f * d[i]) < P->maxDischarge && !(nrm[1] < P->maxSteep) &&
+        h[i] < P->maxTreeHeight) {
+      if (count < cap) {
+        px[count] = x;
+        py[count] = y;
+        size[count] = 0.0f;
+        alive[count] = 1;
+        count++;
+        veg_root(units, nx, ny, x, y, 1);
+      } else {
+        (*dropped)++;
+      }
+    }
+  }
+  veg_apply(root, units, n);
+  int g0 = 0, g1 = count;
+  for (int gen = 0; gen < 1024 && g1 > g0; ++gen) {
+    for (int k = g0; k < g1; ++k) {
+      cflag[k] = 0;
+      size[k] += P->growRate * (P->maxSize - size[k]);
+      const int x = px[k], y = py[k];
+      const size_t i = (size_t)x * ny + y;
+      if (orc_erf(0.4f * d[i]) >= P->maxDischarge || h[i] >= P->maxTreeHeight ||
+          veg_draw(key, (uint32_t)k, 2, 1000) == 0) {
+        alive[k] = 0;
+        veg_root(units, nx, ny, x, y, -1);
+        continue;
+      }
+      if (veg_draw(key, (uint32_t)k, 3, 20) != 0) continue;
+      const int X = x + (int)veg_draw(key, (uint32_t)k, 4, 9) - 4;
+      const int Y = y + (int)veg_draw(key, (uint32_t)k, 5, 9) - 4;
+      if (X < 0 || Y < 0 || X >= nx || Y >= ny) continue;
+      const size_t j = (size_t)X * ny + Y;
+      if (orc_erf(0.4f * d[j]) >= P->maxDischarge) continue;
+      const float r = (float)veg_draw(key, (uint32_t)k, 6, 1000);
+      if ((double)r / 1000.0 <= (double)root[j]) continue;
+      orc_normal(h, nx, ny, X, Y, nrm);
+      if (nrm[1] <= P->maxSteep) continue;
+      cflag[k] = 1;
+      cx[k] = X;
+      cy[k] = Y;
+    }
+    for (int k = g0; k < g1; ++k) {
+      if (!cflag[k]) continue;
+      if (count < cap) {
+        px[count] = cx[k];
+        py[count] = cy[k];
+        size[count] = 0.0f;
+        alive[count] = 1;
+        count++;
+        veg_root(units, nx, ny, cx[k], cy[k], 1);
+      } else {
+        (*dropped)++;
+      }
+    }
+    veg_apply(root, units, n);
+    g0 = g1;
+    g1 = count;
+  }
+  int m = 0;
+  for (int k = 0; k < count; ++k)
+    if (alive[k]) {
+      px[m] = px[k];
+      py[m] = py[k];
+      size[m] = size[k];
+      m++;
+    }
+  free(units);
+  free(alive);
+  free(cflag);
+  free(cx);
+  free(cy);
+  return m;
+}
+
 /* ------------------------------------------------------------ accessors */
 static inline int oob(int nx, int ny, int x, int y) {
   /* quad::map::oob, cellpool.h:413-419 */

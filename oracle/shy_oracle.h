@@ -95,6 +95,15 @@ float orc_fnl_noise3(float freq, float x, float y, float z);
 void orc_fnl_world(int nx, int ny, int seed, int tileres, float* out);
 int orc_os2_edge(int entry);
 
+/* Vegetation::grow with the device's generation semantics; returns the new
+   plant count (the list is compacted in place) */
+typedef struct orc_plant_params {
+  float maxSize, growRate, maxSteep, maxDischarge, maxTreeHeight; /* vegetation.h:37-41 */
+} orc_plant_params;
+int orc_veg_grow(const float* h, const float* d, float* root, int nx, int ny, int32_t* px,
+                 int32_t* py, float* size, int count, int cap, const orc_plant_params* P,
+                 uint64_t seed, uint32_t step, int64_t* dropped);
+
 /* normal of reference _normal (cellpool.h:181-204); h x-major floats */
 void orc_normal(const float* h, int nx, int ny, int x, int y, float* n3);
 

@@ -91,6 +91,17 @@ def read_map(path: str):
     return hd, body.reshape(hd.nx, hd.ny, 8).copy()
 
 
+class PlantParams(C.Structure):
+    """shy_plant_params: the Plant statics of source/vegetation.h:37-41."""
+    _fields_ = [(n, C.c_float) for n in ("maxSize", "growRate", "maxSteep", "maxDischarge",
+                                         "maxTreeHeight")]
+
+    def __init__(self, **kw):
+        super().__init__(1.5, 0.05, 0.8, 0.3, 0.8)
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
 class DropStep(C.Structure):
     _fields_ = [("age", C.c_int32), ("status", C.c_int32), ("pos", C.c_float * 2),
                 ("speed", C.c_float * 2), ("volume", C.c_float), ("sediment", C.c_float),
@@ -166,6 +177,12 @@ def lib():
             "shy_download_textures": (i32, [vp, vp, vp, vp]),
             "shy_download_vertices": (i32, [vp, vp]),
             "shy_save_map": (i32, [vp, C.c_char_p]),
+            "shy_default_plant_params": (None, [vp]),
+            "shy_vegetation_enable": (i32, [vp, i32, vp]),
+            "shy_vegetation_grow": (i32, [vp]),
+            "shy_vegetation_count": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_int64)]),
+            "shy_vegetation_download": (i32, [vp, vp]),
+            "shy_vegetation_upload": (i32, [vp, vp, i32]),
             "shy_upload_cells_tiled": (i32, [vp, i32, vp]),
             "shy_download_cells_tiled": (i32, [vp, i32, vp]),
             "shy_load_map": (i32, [vp, C.c_char_p]),
@@ -366,6 +383,29 @@ class World:
         m = self.nx // tileres
         a = _f32(pool, (m * m, tileres, tileres, 8))
         _check(self._ctx, lib().shy_upload_cells_tiled(self._ctx, tileres, a.ctypes.data))
+
+    # -- vegetation (vegetation.h:44-186) ----------------------------------
+    def vegetation_enable(self, capacity: int = 1 << 20, params: "PlantParams | None" = None):
+        p = params if params is not None else PlantParams()
+        _check(self._ctx, lib().shy_vegetation_enable(self._ctx, capacity, C.byref(p)))
+
+    def vegetation_grow(self, steps: int = 1):
+        """``Vegetation::grow()`` ``steps`` times."""
+        for _ in range(steps):
+            _check(self._ctx, lib().shy_vegetation_grow(self._ctx))
+
+    def plants(self) -> np.ndarray:
+        """The plant list in order: (n, 3) = x, y, size."""
+        n, dropped = C.c_int(), C.c_int64()
+        _check(self._ctx, lib().shy_vegetation_count(self._ctx, C.byref(n), C.byref(dropped)))
+        out = np.empty((n.value, 3), np.float32)
+        if n.value:
+            _check(self._ctx, lib().shy_vegetation_download(self._ctx, out.ctypes.data))
+        return out
+
+    def set_plants(self, xys):
+        a = np.ascontiguousarray(xys, np.float32).reshape(-1, 3)
+        _check(self._ctx, lib().shy_vegetation_upload(self._ctx, a.ctypes.data, a.shape[0]))
 
     def upload_cells(self, cells):
         a = _f32(cells, (self.nx, self.ny, 8))

@@ -75,6 +75,13 @@ struct shy_ctx {
   void* drop_buf = nullptr;
   hipStream_t stream = nullptr;
   shy_tuning tune{256, 0, 0};
+  // vegetation (single terrain)
+  int veg_cap = 0;
+  int32_t *veg_px = nullptr, *veg_py = nullptr, *veg_alive = nullptr, *veg_cflag = nullptr;
+  int32_t *veg_cx = nullptr, *veg_cy = nullptr, *veg_units = nullptr, *veg_st = nullptr;
+  float* veg_size = nullptr;
+  uint32_t veg_step = 0;
+  shy_plant_params veg_P{};
   hipGraphExec_t graph = nullptr;
   std::string err;
 };
@@ -478,7 +485,8 @@ int shy_destroy(shy_ctx* c) {
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
                   c->first, c->osc, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
-                  c->slog, c->dlog, c->tsteps, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
+                  c->slog, c->dlog, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
+                  c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -935,6 +943,171 @@ int shy_upload_cells_tiled(shy_ctx* c, int tileres, const float* pool8) {
       std::memcpy(cells.data() + 8 * ((size_t)x * n + y),
                   pool8 + 8 * tiled_index(x, y, tileres, mapsize), 8 * sizeof(float));
   return shy_upload_cells(c, cells.data());
+}
+
+void shy_default_plant_params(shy_plant_params* p) {
+  if (!p) return;
+  p->maxSize = 1.5f;
+  p->growRate = 0.05f;
+  p->maxSteep = 0.8f;
+  p->maxDischarge = 0.3f;
+  p->maxTreeHeight = 0.8f;
+}
+
+int shy_vegetation_enable(shy_ctx* c, int capacity, const shy_plant_params* p) {
+  RETURN_IF(need_single(c, "shy_vegetation_enable"));
+  if (capacity < 1) return fail(c, SHY_ERR_ARG, "shy_vegetation_enable: capacity must be >= 1");
+  shy_plant_params P;
+  shy_default_plant_params(&P);
+  if (p) P = *p;
+  const float v[] = {P.maxSize, P.growRate, P.maxSteep, P.maxDischarge, P.maxTreeHeight};
+  for (float x : v)
+    if (!std::isfinite(x)) return fail(c, SHY_ERR_ARG, "shy_vegetation_enable: non-finite parameter");
+  hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (capacity != c->veg_cap) {
+    void* old[] = {c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag, c->veg_cx, c->veg_cy,
+                   c->veg_size};
+    for (void* q : old)
+      if (q) hipFree(q);
+    c->veg_px = c->veg_py = c->veg_alive = c->veg_cflag = c->veg_cx = c->veg_cy = nullptr;
+    c->veg_size = nullptr;
+    c->veg_cap = 0;
+    const size_t b = (size_t)capacity * sizeof(int32_t);
+    HIPCHK(c, hipMalloc(&c->veg_px, b));
+    HIPCHK(c, hipMalloc(&c->veg_py, b));
+    HIPCHK(c, hipMalloc(&c->veg_alive, b));
+    HIPCHK(c, hipMalloc(&c->veg_cflag, b));
+    HIPCHK(c, hipMalloc(&c->veg_cx, b));
+    HIPCHK(c, hipMalloc(&c->veg_cy, b));
+    HIPCHK(c, hipMalloc(&c->veg_size, (size_t)capacity * sizeof(float)));
+    c->veg_cap = capacity;
+  }
+  if (!c->veg_units) HIPCHK(c, hipMalloc(&c->veg_units, c->n * sizeof(int32_t)));
+  if (!c->veg_st) HIPCHK(c, hipMalloc(&c->veg_st, 4 * sizeof(int32_t)));
+  HIPCHK(c, hipMemsetAsync(c->veg_units, 0, c->n * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->veg_st, 0, 4 * sizeof(int32_t), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->veg_P = P;
+  c->veg_step = 0;
+  if (!c->ter[0].has_root) {  // the walk now reads the root density
+    c->ter[0].has_root = 1;
+    RETURN_IF(layout(c));
+  }
+  return SHY_OK;
+}
+
+static VegArgs veg_args(shy_ctx* c) {
+  VegArgs a;
+  a.h = c->h[parity(c)];
+  a.fld = c->fld;
+  a.root = c->root;
+  a.units = c->veg_units;
+  a.nx = c->ter[0].nx;
+  a.ny = c->ter[0].ny;
+  a.px = c->veg_px;
+  a.py = c->veg_py;
+  a.size = c->veg_size;
+  a.alive = c->veg_alive;
+  a.cflag = c->veg_cflag;
+  a.cx = c->veg_cx;
+  a.cy = c->veg_cy;
+  a.st = c->veg_st;
+  a.cap = c->veg_cap;
+  a.P = c->veg_P;
+  return a;
+}
+
+static int veg_state(shy_ctx* c, int32_t st[4]) {
+  HIPCHK(c, hipMemcpyAsync(st, c->veg_st, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SHY_OK;
+}
+
+int shy_vegetation_grow(shy_ctx* c) {
+  RETURN_IF(need_single(c, "shy_vegetation_grow"));
+  if (!c->veg_cap) return fail(c, SHY_ERR_STATE, "shy_vegetation_grow: vegetation not enabled");
+  hipSetDevice(c->device);
+  const VegArgs a = veg_args(c);
+  const uint64_t key = cycle_key(c->ter[0].seed, 0xffffffffu, c->veg_step);
+  const size_t grid = (c->n + 255) / 256;
+  veg_spawn_kernel<<<1, 64, 0, c->stream>>>(a, key);
+  veg_apply_kernel<<<grid, 256, 0, c->stream>>>(c->root, c->veg_units, c->n);
+  HIPCHK(c, hipGetLastError());
+  int32_t st[4];
+  RETURN_IF(veg_state(c, st));
+  int g0 = 0, g1 = st[0];
+  for (int gen = 0; gen < 1024 && g1 > g0; ++gen) {
+    veg_gen_kernel<<<(g1 - g0 + 255) / 256, 256, 0, c->stream>>>(a, key, g0, g1);
+    veg_append_kernel<<<1, kScanBlock, 0, c->stream>>>(a, g0, g1);
+    veg_apply_kernel<<<grid, 256, 0, c->stream>>>(c->root, c->veg_units, c->n);
+    HIPCHK(c, hipGetLastError());
+    RETURN_IF(veg_state(c, st));
+    g0 = g1;
+    g1 = st[0];
+  }
+  veg_compact_kernel<<<1, kScanBlock, 0, c->stream>>>(a);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->veg_step++;
+  return SHY_OK;
+}
+
+int shy_vegetation_count(shy_ctx* c, int* plants, int64_t* dropped) {
+  RETURN_IF(need_single(c, "shy_vegetation_count"));
+  if (!c->veg_cap) return fail(c, SHY_ERR_STATE, "shy_vegetation_count: vegetation not enabled");
+  int32_t st[4];
+  RETURN_IF(veg_state(c, st));
+  if (plants) *plants = st[0];
+  if (dropped) *dropped = st[2];
+  return SHY_OK;
+}
+
+int shy_vegetation_download(shy_ctx* c, float* xys) {
+  int n = 0;
+  RETURN_IF(shy_vegetation_count(c, &n, nullptr));
+  if (!xys) return fail(c, SHY_ERR_ARG, "shy_vegetation_download: NULL argument");
+  std::vector<int32_t> x(n), y(n);
+  std::vector<float> sz(n);
+  if (n) {
+    HIPCHK(c, hipMemcpy(x.data(), c->veg_px, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(y.data(), c->veg_py, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(sz.data(), c->veg_size, n * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < n; ++k) {
+    xys[3 * k] = (float)x[k];
+    xys[3 * k + 1] = (float)y[k];
+    xys[3 * k + 2] = sz[k];
+  }
+  return SHY_OK;
+}
+
+int shy_vegetation_upload(shy_ctx* c, const float* xys, int n) {
+  RETURN_IF(need_single(c, "shy_vegetation_upload"));
+  if (!c->veg_cap) return fail(c, SHY_ERR_STATE, "shy_vegetation_upload: vegetation not enabled");
+  if (n < 0 || n > c->veg_cap || (n && !xys))
+    return fail(c, SHY_ERR_ARG, "shy_vegetation_upload: bad plant count or NULL list");
+  const int nx = c->ter[0].nx, ny = c->ter[0].ny;
+  std::vector<int32_t> x(n), y(n), alive(n, 1);
+  std::vector<float> sz(n);
+  for (int k = 0; k < n; ++k) {
+    x[k] = (int32_t)xys[3 * k];
+    y[k] = (int32_t)xys[3 * k + 1];
+    sz[k] = xys[3 * k + 2];
+    if (x[k] < 0 || y[k] < 0 || x[k] >= nx || y[k] >= ny || !std::isfinite(sz[k]))
+      return fail(c, SHY_ERR_ARG, "shy_vegetation_upload: plant off the map or size not finite");
+  }
+  hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n) {
+    HIPCHK(c, hipMemcpy(c->veg_px, x.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->veg_py, y.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->veg_size, sz.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->veg_alive, alive.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  const int32_t st[4] = {n, 0, 0, 0};
+  HIPCHK(c, hipMemcpy(c->veg_st, st, sizeof(st), hipMemcpyHostToDevice));
+  return SHY_OK;
 }
 
 int shy_upload_cells(shy_ctx* c, const float* cells8) {

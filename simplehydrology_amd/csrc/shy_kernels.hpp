@@ -903,6 +903,217 @@ __global__ void refresh_erf_kernel(float4* fld, size_t n, const double* __restri
   fld[i] = f;
 }
 
+// ----------------------------------------------------------- vegetation
+// Vegetation::grow (vegetation.h:120-186) on the device, generation by
+// generation: the spawn attempt, then every plant of the current generation
+// grows, may die (root -1) and may seed a child (root +1), all against the
+// map and root density as they stood when the generation began; the
+// children form the next generation of the same call (the reference's loop
+// also visits plants appended during it).  Root-density changes are integer
+// units of 0.2 (the stencil weights 1.0, 0.6, 0.4 are 5, 3, 2 units), summed
+// order-free and applied once per generation.  Random draws replace rand():
+// counter hashes of (seed, step, plant slot, purpose).
+
+struct VegArgs {
+  const float* __restrict__ h;
+  const float4* __restrict__ fld;  // .w = erf(0.4 discharge) = map.discharge
+  float* root;
+  int32_t* units;                  // pending root-density change, 0.2 units
+  int nx, ny;
+  int32_t* px;
+  int32_t* py;
+  float* size;
+  int32_t* alive;
+  int32_t* cflag;
+  int32_t* cx;
+  int32_t* cy;
+  int32_t* st;                     // {count, children of the last generation, dropped}
+  int cap;
+  shy_plant_params P;
+};
+
+enum VegDraw { kVegX = 0, kVegY = 1, kVegDie = 2, kVegTry = 3, kVegDx = 4, kVegDy = 5, kVegRoot = 6 };
+constexpr uint32_t kVegSpawnSlot = 0xffffffffu;
+
+__host__ __device__ inline uint32_t veg_draw(uint64_t key, uint32_t slot, int purpose, uint32_t m) {
+  const uint64_t r = mix64(key ^ (((uint64_t)slot << 8) | (uint64_t)purpose));
+  return (uint32_t)(((r >> 32) * (uint64_t)m) >> 32);
+}
+
+__device__ inline float veg_normal_y(const VegArgs& a, int x, int y) {
+  const int ny = a.ny;
+  const size_t i = (size_t)x * ny + y;
+  CellView c;
+  c.h = a.h[i];
+  c.xm = a.h[x > 0 ? i - ny : i];
+  c.xp = a.h[x + 1 < a.nx ? i + ny : i];
+  c.ym = a.h[y > 0 ? i - 1 : i];
+  c.yp = a.h[y + 1 < ny ? i + 1 : i];
+  float n0, n1, n2;
+  normal_from(c, x, y, a.nx, ny, n0, n1, n2);
+  return n1;
+}
+
+// Plant::root (vegetation.h:87-117): weights 1.0 / 0.6 / 0.4 = 5 / 3 / 2 units
+__device__ inline void veg_root(const VegArgs& a, int x, int y, int sign) {
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int u = (dx == 0 && dy == 0) ? 5 : (dx == 0 || dy == 0) ? 3 : 2;
+      const int X = x + dx, Y = y + dy;
+      if (off_map(X, Y, a.nx, a.ny)) continue;
+      __hip_atomic_fetch_add(&a.units[(size_t)X * a.ny + Y], sign * u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Vegetation::grow's spawn attempt (vegetation.h:124-136, Plant::spawn :72-85)
+__global__ void veg_spawn_kernel(VegArgs a, uint64_t key) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int x = (int)veg_draw(key, kVegSpawnSlot, kVegX, (uint32_t)a.nx);
+  const int y = (int)veg_draw(key, kVegSpawnSlot, kVegY, (uint32_t)a.ny);
+  const size_t i = (size_t)x * a.ny + y;
+  if (a.fld[i].w >= a.P.maxDischarge) return;
+  if (veg_normal_y(a, x, y) < a.P.maxSteep) return;
+  if (a.h[i] >= a.P.maxTreeHeight) return;
+  const int n = a.st[0];
+  if (n >= a.cap) {
+    a.st[2] += 1;
+    return;
+  }
+  a.px[n] = x;
+  a.py[n] = y;
+  a.size[n] = 0.0f;
+  a.alive[n] = 1;
+  a.st[0] = n + 1;
+  veg_root(a, x, y, 1);
+}
+
+// one generation [g0, g1): grow, die (Plant::die :63-70), seed a child (:144-184)
+__global__ void veg_gen_kernel(VegArgs a, uint64_t key, int g0, int g1) {
+  const int k = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g1) return;
+  a.cflag[k] = 0;
+  const shy_plant_params& P = a.P;
+  float sz = a.size[k];
+  sz += P.growRate * (P.maxSize - sz);  // Plant::grow, :59-61
+  a.size[k] = sz;
+  const int x = a.px[k], y = a.py[k];
+  const size_t i = (size_t)x * a.ny + y;
+  if (a.fld[i].w >= P.maxDischarge || a.h[i] >= P.maxTreeHeight ||
+      veg_draw(key, (uint32_t)k, kVegDie, 1000) == 0) {
+    a.alive[k] = 0;
+    veg_root(a, x, y, -1);
+    return;
+  }
+  if (veg_draw(key, (uint32_t)k, kVegTry, 20) != 0) return;
+  const int nx = x + (int)veg_draw(key, (uint32_t)k, kVegDx, 9) - 4;
+  const int ny = y + (int)veg_draw(key, (uint32_t)k, kVegDy, 9) - 4;
+  if (off_map(nx, ny, a.nx, a.ny)) return;
+  const size_t j = (size_t)nx * a.ny + ny;
+  if (a.fld[j].w >= P.maxDischarge) return;
+  const float r = (float)veg_draw(key, (uint32_t)k, kVegRoot, 1000);
+  if ((double)r / 1000.0 <= (double)a.root[j]) return;
+  if (veg_normal_y(a, nx, ny) <= P.maxSteep) return;
+  a.cflag[k] = 1;
+  a.cx[k] = nx;
+  a.cy[k] = ny;
+}
+
+// exclusive block scan (1024 threads) of v; returns the block total
+__device__ inline int veg_block_scan(int v, int* tmp, int& total) {
+  return block_exclusive_scan(v, tmp, total);
+}
+
+// appends the children of [g0, g1) in parent order (one block)
+__global__ void __launch_bounds__(kScanBlock) veg_append_kernel(VegArgs a, int g0, int g1) {
+  __shared__ int tmp[kScanBlock];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = a.st[0];
+  __syncthreads();
+  int added = 0, dropped = 0;
+  for (int c0 = g0; c0 < g1; c0 += kScanBlock) {
+    const int k = c0 + threadIdx.x;
+    const int f = k < g1 ? a.cflag[k] : 0;
+    int total;
+    const int off = veg_block_scan(f, tmp, total);
+    const int b = base;
+    if (f) {
+      const int slot = b + off;
+      if (slot < a.cap) {
+        a.px[slot] = a.cx[k];
+        a.py[slot] = a.cy[k];
+        a.size[slot] = 0.0f;
+        a.alive[slot] = 1;
+        veg_root(a, a.cx[k], a.cy[k], 1);
+        added += 1;
+      } else {
+        dropped += 1;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base = b + total;
+    __syncthreads();
+  }
+  // totals (slots are contiguous from the old count, so the new count is
+  // min(old + children, capacity))
+  int tot_add, tot_drop;
+  veg_block_scan(added, tmp, tot_add);
+  veg_block_scan(dropped, tmp, tot_drop);
+  if (threadIdx.x == 0) {
+    a.st[0] += tot_add;
+    a.st[1] = tot_add;
+    a.st[2] += tot_drop;
+  }
+}
+
+// root density += 0.2 x the generation's units; clears the units
+__global__ void veg_apply_kernel(float* root, int32_t* units, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int u = units[i];
+  if (u) {
+    root[i] += 0.2f * (float)u;
+    units[i] = 0;
+  }
+}
+
+// removes dead plants, keeping the order (one block)
+__global__ void __launch_bounds__(kScanBlock) veg_compact_kernel(VegArgs a) {
+  __shared__ int tmp[kScanBlock];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int n = a.st[0];
+  for (int c0 = 0; c0 < n; c0 += kScanBlock) {
+    const int k = c0 + threadIdx.x;
+    const int f = k < n ? a.alive[k] : 0;
+    int x = 0, y = 0;
+    float sz = 0.0f;
+    if (f) {
+      x = a.px[k];
+      y = a.py[k];
+      sz = a.size[k];
+    }
+    int total;
+    const int off = veg_block_scan(f, tmp, total);
+    const int b = base;
+    __syncthreads();
+    if (f) {
+      const int d = b + off;
+      a.px[d] = x;
+      a.py[d] = y;
+      a.size[d] = sz;
+      a.alive[d] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base = b + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.st[0] = base;
+}
+
 // ------------------------------------------------------------------ fBm
 __device__ inline float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
 

@@ -374,6 +374,60 @@ def test_fnl_world_properties():
     assert max(abs(x) for x in v) <= 1.0
 
 
+def _flat_world(n=16, h0=0.5):
+    h = np.full((n, n), h0, np.float32)
+    z = np.zeros((n, n), np.float32)
+    return h, z.copy(), z.copy()
+
+
+def test_vegetation_root_stencil_and_first_growth():
+    """On a flat, dry map the spawn attempt always succeeds: Plant::root(1.0)
+    writes 1.0 at the plant, 0.6 on the 4 edge neighbours and 0.4 on the 4
+    corners (vegetation.h:87-117), and the new plant grows once in the same
+    call, to growRate * maxSize (vegetation.h:59-61, 124-142)."""
+    h, d, root = _flat_world()
+    pl, dropped = O.veg_grow(h, d, root, np.zeros((0, 3), np.float32), seed=4, step=0)
+    assert dropped == 0 and pl.shape[0] == 1   # seed 4: one interior spawn, no child
+    x, y, s0 = pl[0]
+    assert s0 == np.float32(0.05) * np.float32(1.5)
+    x, y = int(x), int(y)
+    assert 1 <= x < 15 and 1 <= y < 15
+    w = root[x - 1:x + 2, y - 1:y + 2]
+    expect = np.array([[0.4, 0.6, 0.4], [0.6, 1.0, 0.6], [0.4, 0.6, 0.4]], np.float32)
+    assert np.array_equal(w, expect)
+    assert np.count_nonzero(root) == 9
+
+
+def test_vegetation_growth_recurrence_and_death_rules():
+    """A surviving plant follows size += 0.05 (1.5 - size) in float; a plant
+    on a river (erf(0.4 d) >= 0.3) or at height >= 0.8 dies at the next call
+    and its root contribution is removed exactly (Plant::die, :63-70)."""
+    h, d, root = _flat_world()
+    plants = np.array([[8, 8, 0.0]], np.float32)
+    size = np.float32(0.0)
+    for step in range(30):
+        pl, _ = O.veg_grow(h, d, root, plants, seed=9, step=step)
+        size = size + np.float32(0.05) * (np.float32(1.5) - size)
+        ours = pl[(pl[:, 0] == 8) & (pl[:, 1] == 8)]
+        if ours.shape[0] == 0:
+            break  # the 1/1000 random death took it
+        assert ours[0, 2] == size
+        plants = pl
+    # rivers and peaks kill
+    for kill in ("river", "peak"):
+        h, d, root = _flat_world()
+        plants = np.array([[5, 5, 0.3]], np.float32)
+        root[4:7, 4:7] = np.array([[0.4, 0.6, 0.4], [0.6, 1.0, 0.6], [0.4, 0.6, 0.4]], np.float32)
+        if kill == "river":
+            d[5, 5] = 1.0   # erf(0.4) = 0.43 >= 0.3
+        else:
+            h[5, 5] = 0.8
+        pl, _ = O.veg_grow(h, d, root, plants, seed=1, step=0)
+        assert not np.any((pl[:, 0] == 5) & (pl[:, 1] == 5))
+        # seed 1's spawn lands at (11, 6), clear of this stencil: exactly zero again
+        assert np.all(root[4:7, 4:7] == 0.0), kill
+
+
 def _hyps(h):
     return np.quantile(h, np.linspace(0, 1, 21))
 
