@@ -127,6 +127,115 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvArgs a) {
     }
 }
 
+// Forward (and dgrad) convolution for Cin and Cout multiples of 32, staged
+// through LDS.  A block owns a 16 (x) by 4*NT (y) pixel tile of one image and
+// 16*MT output channels.  Per 32-channel chunk of the input it stages the
+// tile's input patch with a 1-pixel halo (TAPS = 9) and the chunk's weights
+// for every tap, then each wave multiplies: output channels are the MFMA's
+// M operand (A = weights), pixels its N operand (B = patch), so a lane ends
+// with 4 consecutive channels of one pixel and stores them as one vector.
+// Wave w owns tile rows NT*w .. NT*w+NT-1 (one 16-pixel N tile each).
+constexpr int kTW = 16;
+constexpr int kPS = 40;  // LDS bf16 per patch pixel (32 + 8: spreads the banks)
+
+template <int TAPS, int MT, int NT>
+__global__ void __launch_bounds__(256) conv_lds_kernel(ConvArgs a) {
+  constexpr int kTH = 4 * NT;
+  constexpr int HALO = TAPS == 9 ? 1 : 0;
+  constexpr int PW = kTW + 2 * HALO, PH = kTH + 2 * HALO;
+  constexpr int WS = TAPS * 32 + 8;  // LDS bf16 per output channel of the weight chunk
+  __shared__ __bf16 patch[PH * PW * kPS];
+  __shared__ __bf16 wl[MT * 16 * WS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+  int b = blockIdx.x;
+  const int tx = b % tiles_x;
+  b /= tiles_x;
+  const int ty = b % tiles_y;
+  const int n = b / tiles_y;
+  const int x0 = tx * kTW, y0 = ty * kTH;
+  const int cb = blockIdx.y * MT * 16;
+  const int Cin = a.Cin, K = a.K;
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+    // patch: PH x PW pixels x 32 channels as 16-byte vectors (zero off the image)
+    for (int v = threadIdx.x; v < PH * PW * 4; v += 256) {
+      const int pix = v >> 2, q = v & 3;
+      const int py = pix / PW, px = pix % PW;
+      const int gy = y0 + py - HALO, gx = x0 + px - HALO;
+      bf16x8 val = zero8();
+      if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+        val = load8(a.x + ((size_t)(n * a.H + gy) * a.W + gx) * Cin + ci0 + q * 8);
+      *reinterpret_cast<bf16x8*>(&patch[pix * kPS + q * 8]) = val;
+    }
+    // weights: MT*16 channels x TAPS taps x 32 input channels
+    for (int v = threadIdx.x; v < MT * 16 * TAPS * 4; v += 256) {
+      const int co = v / (TAPS * 4), r = v % (TAPS * 4);
+      const int tap = r >> 2, q = r & 3;
+      bf16x8 val = zero8();
+      if (cb + co < a.Cout) val = load8(a.w + (size_t)(cb + co) * K + tap * Cin + ci0 + q * 8);
+      *reinterpret_cast<bf16x8*>(&wl[co * WS + tap * 32 + q * 8]) = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < TAPS; ++tap) {
+      const int dy = TAPS == 9 ? tap / 3 : 0, dx = TAPS == 9 ? tap % 3 : 0;
+      bf16x8 B[NT], A[MT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int row = wave * NT + t + dy, col = (lane & 15) + dx;
+        B[t] = *reinterpret_cast<const bf16x8*>(&patch[(row * PW + col) * kPS + (lane >> 4) * 8]);
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        A[m] = *reinterpret_cast<const bf16x8*>(&wl[(m * 16 + (lane & 15)) * WS + tap * 32 +
+                                                    (lane >> 4) * 8]);
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = mfma(A[m], B[t], acc[m][t]);
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds channels cb + 16m + 4(lane/16) + i of pixel (row, lane%16)
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
+    if (gy >= a.H || gx >= a.W) continue;
+    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int co = cb + m * 16 + (lane >> 4) * 4;
+      if (co >= a.Cout) continue;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
+      const size_t o = p * a.Cout + co;
+      if (a.res) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
+      }
+      if (a.relu) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+      }
+      if (a.y_f) {
+        *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 q;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
+        *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
+      }
+    }
+  }
+}
+
 // dgrad weights: Wf[ci][tap'*Cout + co] = W[co][(TAPS-1-tap')*Cin + ci]
 __global__ void flip_weights_kernel(const __bf16* __restrict__ w, __bf16* wf, int Cin, int Cout,
                                     int taps) {
@@ -353,6 +462,107 @@ __global__ void add_kernel(const __bf16* __restrict__ a, const __bf16* __restric
   y[i] = (__bf16)((float)a[i] + (float)b[i]);
 }
 
+// The same element-wise kernels on 8 channels (16 bytes) per thread, used
+// when every channel count and offset is a multiple of 8.
+__device__ inline void store8(__bf16* p, bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
+
+__global__ void relu_bwd_v8_kernel(const __bf16* __restrict__ y, const __bf16* __restrict__ g,
+                                   __bf16* out, size_t n8) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const bf16x8 a = load8(y + 8 * i), b = load8(g + 8 * i);
+  bf16x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = (float)a[k] > 0.0f ? b[k] : (__bf16)0.0f;
+  store8(out + 8 * i, r);
+}
+
+__global__ void add_v8_kernel(const __bf16* __restrict__ a, const __bf16* __restrict__ b,
+                              __bf16* y, size_t n8) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const bf16x8 u = load8(a + 8 * i), v = load8(b + 8 * i);
+  bf16x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = (__bf16)((float)u[k] + (float)v[k]);
+  store8(y + 8 * i, r);
+}
+
+__global__ void pool2_fwd_v8_kernel(const __bf16* __restrict__ x, __bf16* y, int N, int H, int W,
+                                    int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+  if (i >= (size_t)N * Ho * Wo * C8) return;
+  const int c = (int)(i % C8) * 8;
+  const size_t q = i / C8;
+  const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), nn = (int)(q / ((size_t)Wo * Ho));
+  const size_t b = ((size_t)(nn * H + 2 * yo) * W + 2 * xo) * C + c;
+  const bf16x8 v0 = load8(x + b), v1 = load8(x + b + C), v2 = load8(x + b + (size_t)W * C),
+               v3 = load8(x + b + (size_t)W * C + C);
+  bf16x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    r[k] = (__bf16)(0.25f * ((float)v0[k] + (float)v1[k] + (float)v2[k] + (float)v3[k]));
+  store8(y + q * C + c, r);
+}
+
+__global__ void pool2_bwd_v8_kernel(const __bf16* __restrict__ gy, __bf16* gx, int N, int H,
+                                    int W, int C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int C8 = C / 8;
+  if (i >= (size_t)N * H * W * C8) return;
+  const int c = (int)(i % C8) * 8;
+  const size_t q = i / C8;
+  const int x = (int)(q % W), y = (int)((q / W) % H), nn = (int)(q / ((size_t)W * H));
+  const int Ho = H / 2, Wo = W / 2;
+  const bf16x8 v = load8(gy + ((size_t)(nn * Ho + y / 2) * Wo + x / 2) * C + c);
+  bf16x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = (__bf16)(0.25f * (float)v[k]);
+  store8(gx + q * C + c, r);
+}
+
+__global__ void up2_fwd_v8_kernel(const __bf16* __restrict__ x, __bf16* y, int N, int H, int W,
+                                  int C, int Cy, int c_off) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Ho = 2 * H, Wo = 2 * W, C8 = C / 8;
+  if (i >= (size_t)N * Ho * Wo * C8) return;
+  const int c = (int)(i % C8) * 8;
+  const size_t q = i / C8;
+  const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), nn = (int)(q / ((size_t)Wo * Ho));
+  store8(y + q * Cy + c_off + c, load8(x + ((size_t)(nn * H + yo / 2) * W + xo / 2) * C + c));
+}
+
+__global__ void up2_bwd_v8_kernel(const __bf16* __restrict__ gy, __bf16* gx, int N, int H, int W,
+                                  int C, int Cy, int c_off) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int C8 = C / 8;
+  if (i >= (size_t)N * H * W * C8) return;
+  const int c = (int)(i % C8) * 8;
+  const size_t q = i / C8;
+  const int x = (int)(q % W), y = (int)((q / W) % H), nn = (int)(q / ((size_t)W * H));
+  const int Wo = 2 * W;
+  const size_t b = ((size_t)(nn * 2 * H + 2 * y) * Wo + 2 * x) * Cy + c_off + c;
+  const bf16x8 v0 = load8(gy + b), v1 = load8(gy + b + Cy), v2 = load8(gy + b + (size_t)Wo * Cy),
+               v3 = load8(gy + b + (size_t)Wo * Cy + Cy);
+  bf16x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    r[k] = (__bf16)((float)v0[k] + (float)v1[k] + (float)v2[k] + (float)v3[k]);
+  store8(gx + q * C + c, r);
+}
+
+__global__ void channels_copy_v8_kernel(const __bf16* __restrict__ src, __bf16* dst, size_t P,
+                                        int C, int Cy, int c_off, int dir) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int C8 = C / 8;
+  if (i >= P * C8) return;
+  const size_t p = i / C8;
+  const int c = (int)(i % C8) * 8;
+  if (dir == 0) store8(dst + p * Cy + c_off + c, load8(src + p * C + c));
+  else store8(dst + p * C + c, load8(src + p * Cy + c_off + c));
+}
+
 // masked MSE: loss partials and d(loss)/dy = 2 (y - t) m / count
 __global__ void mse_kernel(const float* __restrict__ y, const float* __restrict__ t,
                            const float* __restrict__ m, __bf16* g, float* part, size_t n,
@@ -441,6 +651,35 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
   const long long P = (long long)N * H * W;
   if (P > INT32_MAX / 2) return nn_fail("conv_fwd: too many pixels");
   hipStream_t s = (hipStream_t)stream;
+  if (Cin % 32 == 0 && Cout % 32 == 0) {
+    // 16x16 tiles (4 rows per wave) when they still give >= 1024 blocks,
+    // else 16x8
+    const int mt = Cout % 64 == 0 ? 4 : 2;
+    const unsigned gy = (unsigned)(Cout / (16 * mt));
+    const long long tiles16 = (long long)N * ((H + 15) / 16) * ((W + kTW - 1) / kTW);
+    const long long tiles8 = (long long)N * ((H + 7) / 8) * ((W + kTW - 1) / kTW);
+    if (tiles8 > INT32_MAX) return nn_fail("conv_fwd: too many tiles");
+    const bool big = tiles16 * gy >= 1024;
+    const dim3 g((unsigned)(big ? tiles16 : tiles8), gy);
+    if (mt == 4) {
+      if (big) {
+        if (taps == 9) conv_lds_kernel<9, 4, 4><<<g, 256, 0, s>>>(a);
+        else conv_lds_kernel<1, 4, 4><<<g, 256, 0, s>>>(a);
+      } else {
+        if (taps == 9) conv_lds_kernel<9, 4, 2><<<g, 256, 0, s>>>(a);
+        else conv_lds_kernel<1, 4, 2><<<g, 256, 0, s>>>(a);
+      }
+    } else {
+      if (big) {
+        if (taps == 9) conv_lds_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
+        else conv_lds_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
+      } else {
+        if (taps == 9) conv_lds_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
+        else conv_lds_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
+      }
+    }
+    return nn_check("conv_fwd");
+  }
   const unsigned gx = (unsigned)((P + 127) / 128);
   if (Cout > 16) {
     dim3 g(gx, (unsigned)((Cout + 31) / 32));
@@ -497,16 +736,29 @@ int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P
   return nn_check("bias_grad_reduce");
 }
 
+inline bool a16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 int shy_nn_relu_bwd(const void* y, const void* g, void* out, size_t n, void* stream) {
-  relu_bwd_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)y, (const __bf16*)g,
-                                                               (__bf16*)out, n);
+  if (n % 8 == 0 && a16(y) && a16(g) && a16(out))
+    relu_bwd_v8_kernel<<<blocks(n / 8), 256, 0, (hipStream_t)stream>>>(
+        (const __bf16*)y, (const __bf16*)g, (__bf16*)out, n / 8);
+  else
+    relu_bwd_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)y,
+                                                                 (const __bf16*)g, (__bf16*)out, n);
   return nn_check("relu_bwd");
 }
 
 int shy_nn_pool2(const void* x, void* y, int N, int H, int W, int C, int backward, void* stream) {
   if (H % 2 || W % 2) return nn_fail("pool2: H and W must be even");
   hipStream_t s = (hipStream_t)stream;
-  if (!backward)
+  const bool v8 = C % 8 == 0 && a16(x) && a16(y);
+  if (v8 && !backward)
+    pool2_fwd_v8_kernel<<<blocks((size_t)N * H * W * C / 32), 256, 0, s>>>(
+        (const __bf16*)x, (__bf16*)y, N, H, W, C);
+  else if (v8)
+    pool2_bwd_v8_kernel<<<blocks((size_t)N * H * W * C / 8), 256, 0, s>>>(
+        (const __bf16*)x, (__bf16*)y, N, H, W, C);
+  else if (!backward)
     pool2_fwd_kernel<<<blocks((size_t)N * H * W * C / 4), 256, 0, s>>>((const __bf16*)x, (__bf16*)y,
                                                                        N, H, W, C);
   else
@@ -519,7 +771,14 @@ int shy_nn_up2(const void* x, void* y, int N, int H, int W, int C, int Cy, int c
                void* stream) {
   if (c_off < 0 || c_off + C > Cy) return nn_fail("up2: channel range outside the target");
   hipStream_t s = (hipStream_t)stream;
-  if (!backward)
+  const bool v8 = C % 8 == 0 && Cy % 8 == 0 && c_off % 8 == 0 && a16(x) && a16(y);
+  if (v8 && !backward)
+    up2_fwd_v8_kernel<<<blocks((size_t)N * 4 * H * W * C / 8), 256, 0, s>>>(
+        (const __bf16*)x, (__bf16*)y, N, H, W, C, Cy, c_off);
+  else if (v8)
+    up2_bwd_v8_kernel<<<blocks((size_t)N * H * W * C / 8), 256, 0, s>>>(
+        (const __bf16*)x, (__bf16*)y, N, H, W, C, Cy, c_off);
+  else if (!backward)
     up2_fwd_kernel<<<blocks((size_t)N * 4 * H * W * C), 256, 0, s>>>(
         (const __bf16*)x, (__bf16*)y, N, H, W, C, Cy, c_off);
   else
@@ -531,14 +790,22 @@ int shy_nn_up2(const void* x, void* y, int N, int H, int W, int C, int Cy, int c
 int shy_nn_channels(const void* src, void* dst, size_t P, int C, int Cy, int c_off, int dir,
                     void* stream) {
   if (c_off < 0 || c_off + C > Cy) return nn_fail("channels: range outside the target");
-  channels_copy_kernel<<<blocks(P * C), 256, 0, (hipStream_t)stream>>>(
-      (const __bf16*)src, (__bf16*)dst, P, C, Cy, c_off, dir);
+  if (C % 8 == 0 && Cy % 8 == 0 && c_off % 8 == 0 && a16(src) && a16(dst))
+    channels_copy_v8_kernel<<<blocks(P * C / 8), 256, 0, (hipStream_t)stream>>>(
+        (const __bf16*)src, (__bf16*)dst, P, C, Cy, c_off, dir);
+  else
+    channels_copy_kernel<<<blocks(P * C), 256, 0, (hipStream_t)stream>>>(
+        (const __bf16*)src, (__bf16*)dst, P, C, Cy, c_off, dir);
   return nn_check("channels");
 }
 
 int shy_nn_add(const void* a, const void* b, void* y, size_t n, void* stream) {
-  add_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)a, (const __bf16*)b,
-                                                          (__bf16*)y, n);
+  if (n % 8 == 0 && a16(a) && a16(b) && a16(y))
+    add_v8_kernel<<<blocks(n / 8), 256, 0, (hipStream_t)stream>>>(
+        (const __bf16*)a, (const __bf16*)b, (__bf16*)y, n / 8);
+  else
+    add_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>((const __bf16*)a, (const __bf16*)b,
+                                                            (__bf16*)y, n);
   return nn_check("add");
 }
 

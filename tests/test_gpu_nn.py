@@ -41,7 +41,8 @@ def ref_conv(x_nhwc, w_rows, b, taps, cin, cout):
 
 @pytest.mark.parametrize("N,H,W,cin,cout,taps", [(2, 16, 24, 8, 32, 9), (1, 32, 32, 64, 96, 9),
                                                  (3, 8, 40, 224, 96, 9), (2, 16, 16, 32, 8, 1),
-                                                 (1, 13, 7, 16, 8, 9)])
+                                                 (1, 13, 7, 16, 8, 9), (1, 24, 20, 32, 64, 9),
+                                                 (2, 8, 16, 32, 64, 1), (1, 5, 9, 96, 128, 9)])
 def test_conv_forward_dgrad_wgrad(sg, N, H, W, cin, cout, taps):
     g = torch.Generator().manual_seed(cin * 7 + cout)
     x = bf(torch.randn(N, H, W, cin, generator=g))
