@@ -20,6 +20,7 @@
 //                      channel concat/split, bias gradient, MSE loss, Adam
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -322,6 +323,144 @@ __global__ void __launch_bounds__(64) conv_wgrad_kernel(WgradArgs a) {
         const int k = kb + c * 16 + (lane & 15);
         if (co < a.Cout && k < a.K) out[(size_t)co * a.K + k] = acc[r][c][i];
       }
+}
+
+// Weight gradient for Cin and Cout multiples of 32, TAPS = 9, staged through
+// LDS: dW[co][tap*Cin + ci] = sum_p g[p][co] x[p + tap][ci].  The reduction
+// runs over pixels, so both MFMA operands need 8 consecutive pixels of one
+// channel per lane: each 16x8 pixel tile is staged transposed (channel-major)
+// -- G^T [16*MT][128 pixels] and X^T [32][10 rows][24] (the tile with its
+// 1-pixel halo) -- four pixels per 8-byte LDS write.  A block owns 16*MT
+// output channels x one 32-channel input chunk x all 9 taps (18 N tiles of
+// 16), wave w the N tiles w, w+4, ...; it sums its range of pixel tiles into
+// one FP32 partial.  The +1 / +2 pixel shifts of the taps read two aligned
+// 16-byte rows of X^T and realign in registers.
+constexpr int kWgTW = 16, kWgTH = 8, kWgPix = kWgTW * kWgTH;
+constexpr int kXR = 24;  // X^T row: 18 halo columns padded to 24
+
+__device__ inline bf16x8 shift_pixels(bf16x8 lo, bf16x8 hi, int dx) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  if (dx == 0) return lo;
+  const u32x4 a = __builtin_bit_cast(u32x4, lo), b = __builtin_bit_cast(u32x4, hi);
+  u32x4 r;
+  if (dx == 2) {
+    r = u32x4{a[1], a[2], a[3], b[0]};
+  } else {  // one bf16 to the left: v_alignbyte of neighbouring dwords
+    r[0] = __builtin_amdgcn_alignbyte(a[1], a[0], 2);
+    r[1] = __builtin_amdgcn_alignbyte(a[2], a[1], 2);
+    r[2] = __builtin_amdgcn_alignbyte(a[3], a[2], 2);
+    r[3] = __builtin_amdgcn_alignbyte(b[0], a[3], 2);
+  }
+  return __builtin_bit_cast(bf16x8, r);
+}
+
+struct WgArgs {
+  const __bf16* __restrict__ x;
+  const __bf16* __restrict__ g;
+  float* part;  // [splits][Cout][K]
+  int N, H, W, Cin, Cout, K;
+  int tiles_x, tiles_y, tiles;
+  int tiles_per_split;
+};
+
+template <int MT>
+__global__ void __launch_bounds__(256) wgrad_lds_kernel(WgArgs a) {
+  constexpr int NCO = 16 * MT;
+  constexpr int GS = kWgPix + 8;  // G^T row (pixels) with padding
+  __shared__ __bf16 gt[NCO * GS];
+  __shared__ __bf16 xt[32 * (kWgTH + 2) * kXR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cb = blockIdx.x * NCO, ci0 = blockIdx.y * 32;
+  const int t_begin = blockIdx.z * a.tiles_per_split;
+  const int t_end = min(a.tiles, t_begin + a.tiles_per_split);
+  const int Cin = a.Cin, Cout = a.Cout;
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  f32x4 acc[MT][5];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = t_begin; t < t_end; ++t) {
+    const int tx = t % a.tiles_x, ty = (t / a.tiles_x) % a.tiles_y;
+    const int n = t / (a.tiles_x * a.tiles_y);
+    const int x0 = tx * kWgTW, y0 = ty * kWgTH;
+    // G^T: task = (4 pixels of a row, 8 channels); pixels off the image are 0
+    for (int task = threadIdx.x; task < (kWgPix / 4) * (NCO / 8); task += 256) {
+      const int pq = task % (kWgPix / 4), cq = task / (kWgPix / 4);
+      const int pix = pq * 4, row = pix / kWgTW, col = pix % kWgTW;
+      bf16x8 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gy = y0 + row, gx = x0 + col + u;
+        v[u] = zero8();
+        if (gy < a.H && gx < a.W)
+          v[u] = load8(a.g + ((size_t)(n * a.H + gy) * a.W + gx) * Cout + cb + cq * 8);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        bf16x4 w4 = {v[0][c], v[1][c], v[2][c], v[3][c]};
+        *reinterpret_cast<bf16x4*>(&gt[(cq * 8 + c) * GS + pix]) = w4;
+      }
+    }
+    // X^T: task = (row of the halo tile, 4 halo columns, 8 channels)
+    for (int task = threadIdx.x; task < (kWgTH + 2) * 5 * 4; task += 256) {
+      const int r = task % (kWgTH + 2), rest = task / (kWgTH + 2);
+      const int cg = rest % 5, cq = rest / 5;
+      bf16x8 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int hc = cg * 4 + u;  // halo column 0..19 (18, 19 are padding)
+        const int gy = y0 + r - 1, gx = x0 + hc - 1;
+        v[u] = zero8();
+        if (hc < kWgTW + 2 && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+          v[u] = load8(a.x + ((size_t)(n * a.H + gy) * a.W + gx) * Cin + ci0 + cq * 8);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        bf16x4 w4 = {v[0][c], v[1][c], v[2][c], v[3][c]};
+        *reinterpret_cast<bf16x4*>(&xt[((cq * 8 + c) * (kWgTH + 2) + r) * kXR + cg * 4]) = w4;
+      }
+    }
+    __syncthreads();
+    // 4 k-steps of 32 pixels (tile rows 2s, 2s+1)
+#pragma unroll
+    for (int s = 0; s < kWgPix / 32; ++s) {
+      const int grp = lane >> 4;  // 8-pixel group of this lane
+      const int R = 2 * s + (grp >> 1), c0 = (grp & 1) * 8;
+      bf16x8 A[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        A[m] = *reinterpret_cast<const bf16x8*>(&gt[(m * 16 + (lane & 15)) * GS + R * kWgTW + c0]);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        const int nt = wave + 4 * j;  // N tile: tap nt/2, channels 16*(nt%2) + 0..15
+        if (nt >= 18) continue;
+        const int tap = nt >> 1, dy = tap / 3, dx = tap % 3;
+        const int ci = (nt & 1) * 16 + (lane & 15);
+        const __bf16* row = &xt[(ci * (kWgTH + 2) + R + dy) * kXR + c0];
+        const bf16x8 lo = *reinterpret_cast<const bf16x8*>(row);
+        const bf16x8 hi = *reinterpret_cast<const bf16x8*>(row + 8);
+        const bf16x8 B = shift_pixels(lo, hi, dx);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[m][j] = mfma(A[m], B, acc[m][j]);
+      }
+    }
+    __syncthreads();
+  }
+  float* out = a.part + (size_t)blockIdx.z * Cout * a.K;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int nt = wave + 4 * j;
+    if (nt >= 18) continue;
+    const int k = (nt >> 1) * Cin + ci0 + (nt & 1) * 16 + (lane & 15);
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = cb + m * 16 + (lane >> 4) * 4 + i;
+        out[(size_t)co * a.K + k] = acc[m][j][i];
+      }
+  }
 }
 
 // dW[i] = sum over splits of part[k][i] in a fixed order (deterministic):
@@ -706,7 +845,35 @@ int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int 
   if (!x || !g || !dw || !part || part_splits < 1) return nn_fail("conv_wgrad: bad pointers");
   if (Cin % 8 || Cout % 8 || (taps != 9 && taps != 1)) return nn_fail("conv_wgrad: channels must be multiples of 8");
   const long long P = (long long)N * H * W;
+  if (taps == 9 && Cin % 32 == 0 && Cout % 32 == 0) {
+    WgArgs b;
+    b.x = (const __bf16*)x;
+    b.g = (const __bf16*)g;
+    b.part = part;
+    b.N = N; b.H = H; b.W = W; b.Cin = Cin; b.Cout = Cout; b.K = 9 * Cin;
+    b.tiles_x = (W + kWgTW - 1) / kWgTW;
+    b.tiles_y = (H + kWgTH - 1) / kWgTH;
+    const long long tiles = (long long)N * b.tiles_x * b.tiles_y;
+    if (tiles > INT32_MAX) return nn_fail("conv_wgrad: too many tiles");
+    b.tiles = (int)tiles;
+    const int mt = Cout % 64 == 0 ? 4 : 2;
+    const int bxy = (Cout / (16 * mt)) * (Cin / 32);
+    // about 2048 blocks, at most part_splits partials, at least one tile each
+    long long splits = std::max(1LL, std::min<long long>((2048 + bxy - 1) / bxy, tiles));
+    splits = std::min<long long>(splits, part_splits);
+    b.tiles_per_split = (int)((tiles + splits - 1) / splits);
+    splits = (tiles + b.tiles_per_split - 1) / b.tiles_per_split;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((unsigned)(Cout / (16 * mt)), (unsigned)(Cin / 32), (unsigned)splits);
+    if (mt == 4) wgrad_lds_kernel<4><<<grid, 256, 0, s>>>(b);
+    else wgrad_lds_kernel<2><<<grid, 256, 0, s>>>(b);
+    if (int r = nn_check("conv_wgrad")) return r;
+    const int n = Cout * b.K;
+    wgrad_reduce_kernel<<<(n + 31) / 32, 256, 0, s>>>(part, dw, n, (int)splits);
+    return nn_check("wgrad_reduce");
+  }
   int chunk = (int)((P + part_splits - 1) / part_splits);
+
   chunk = (chunk + 31) / 32 * 32;
   const int splits = (int)((P + chunk - 1) / chunk);
   WgradArgs a;
