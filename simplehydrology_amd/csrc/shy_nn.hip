@@ -475,8 +475,18 @@ __global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restri
   const int e = threadIdx.x & 31, sl = threadIdx.x >> 5;
   const int i = blockIdx.x * 32 + e;
   float s = 0.0f;
-  if (i < n)
-    for (int k = sl; k < splits; k += kRedSlices) s += part[(size_t)k * n + i];
+  if (i < n) {
+    // eight loads in flight per iteration, added in ascending k as before
+    int k = sl;
+    for (; k + 7 * kRedSlices < splits; k += 8 * kRedSlices) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(k + u * kRedSlices) * n + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; k < splits; k += kRedSlices) s += part[(size_t)k * n + i];
+  }
   acc[sl][e] = s;
   __syncthreads();
   if (sl == 0 && i < n) {
@@ -858,8 +868,9 @@ int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int 
     b.tiles = (int)tiles;
     const int mt = Cout % 64 == 0 ? 4 : 2;
     const int bxy = (Cout / (16 * mt)) * (Cin / 32);
-    // about 2048 blocks, at most part_splits partials, at least one tile each
-    long long splits = std::max(1LL, std::min<long long>((2048 + bxy - 1) / bxy, tiles));
+    // about 1024 blocks (4 waves per CU), at most part_splits partials, at
+    // least one tile each: fewer partials to sum afterwards
+    long long splits = std::max(1LL, std::min<long long>((1024 + bxy - 1) / bxy, tiles));
     splits = std::min<long long>(splits, part_splits);
     b.tiles_per_split = (int)((tiles + splits - 1) / splits);
     splits = (tiles + b.tiles_per_split - 1) / b.tiles_per_split;
