@@ -21,6 +21,8 @@
 //   slog        int4    [step][drop] step log of the count pass
 //   rec         int4    per-step commit records, bucketed by start cell
 //   dlog        int4    [drop] how each walk ended
+//   pack        float4x4 per cell: the count walk's one-line record of the
+//                       cycle's start (5 heights, flow, root), by pack_kernel
 // A device table of Terrain entries maps drops, cells and cascade tiles to
 // their terrain, so every kernel covers the whole ensemble in one launch.
 // Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle
@@ -63,6 +65,7 @@ struct shy_ctx {
   int4* slog = nullptr;  // [step][drop] step log of the count pass
   size_t wave_cap = 0;   // steps the ticket log, step log and buckets can hold
   int4* dlog = nullptr;  // [drop] how each walk ended
+  float4* pack = nullptr;  // per cell, 4 x float4: the count walk's record
   size_t dlog_cap = 0;
   long long* tsteps = nullptr;
   DeviceState* state = nullptr;
@@ -237,6 +240,7 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.rec = c->rec;
   w.slog = c->slog;
   w.dlog = c->dlog;
+  w.pack = c->pack;
   w.deposit = c->deposit;
   w.visit = c->visit[p];
   w.state = c->state;
@@ -267,6 +271,8 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   const int nter = (int)c->ter.size();
   const bool walk = c->total_drops > 0;
   if (walk) {
+    pack_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, n, c->h[p],
+                                                                     c->fld, c->root, c->pack);
     walk_kernel<<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
@@ -486,7 +492,7 @@ int shy_destroy(shy_ctx* c) {
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
                   c->first, c->osc, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
                   c->slog, c->dlog, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
-                  c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
+                  c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -550,6 +556,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->count, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc count");
   if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
   if ((e = hipMalloc(&c->osc, n * sizeof(int2))) != hipSuccess) return bail(e, "hipMalloc osc");
+  if ((e = hipMalloc(&c->pack, 4 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");

@@ -88,6 +88,7 @@ struct WaveArgs {
   int4* rec;
   int4* slog;         // [step][drop] {start cell, c_eq bits, qt(vol sx), qt(vol sy)}
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
+  const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
   int32_t* deposit;
   uint8_t* visit;
   DeviceState* state;
@@ -116,6 +117,7 @@ struct WalkArgs {
   int32_t* count;                  // steps starting at each cell (tickets)
   int32_t* ticket;                 // [step][drop] ticket log
   int4* slog;                      // [step][drop] step log (count pass)
+  const float4* __restrict__ pack; // packed cells, or nullptr (read h, fld, root)
   uint8_t* visit;                  // cascade sites
   DeviceState* state;
   uint64_t seed;
@@ -166,7 +168,20 @@ __device__ inline void block_add_stats(long long (&v)[NC], long long* out) {
 }
 
 // ------------------------------------------------------------------ walk
+// One cache line per step: {h, h(x-1), h(x+1), h(y-1)}, {h(y+1), mx, my, d},
+// {erf(0.4 d), root, 0, 0}; neighbours off the map repeat the cell's own
+// height, as load_cell's clamped reads do (the normal never uses them).
+__device__ inline CellView load_packed(const float4* __restrict__ pk, int i) {
+  const float4 a = pk[4 * (size_t)i], b = pk[4 * (size_t)i + 1], c = pk[4 * (size_t)i + 2];
+  CellView v;
+  v.h = a.x; v.xm = a.y; v.xp = a.z; v.ym = a.w;
+  v.yp = b.x; v.mx = b.y; v.my = b.z; v.d = b.w;
+  v.erfd = c.x; v.root = c.y;
+  return v;
+}
+
 __device__ inline CellView load_cell(const WalkArgs& a, int x, int y, int i) {
+  if (a.pack) return load_packed(a.pack, i);
   CellView c;
   const int ny = a.ny;
   c.h = a.h[i];
@@ -280,6 +295,28 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   return steps;
 }
 
+// The walk's per-cell record for this cycle (load_packed), every cell of
+// every terrain: 5 heights of the normal's stencil, the flow float4, root.
+__global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
+                            const float* __restrict__ h, const float4* __restrict__ fld,
+                            const float* __restrict__ root, float4* pack) {
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int t = find_terrain(nter, (long long)g, [&](int q) { return ter[q].cell_off; });
+  const Terrain& T = ter[t];
+  const size_t i = g - (size_t)T.cell_off;
+  const int ny = T.ny, x = (int)(i / (size_t)ny), y = (int)(i % (size_t)ny);
+  const float* hh = h + T.cell_off;
+  const float hc = hh[i];
+  const float xm = hh[x > 0 ? i - ny : i], xp = hh[x + 1 < T.nx ? i + ny : i];
+  const float ym = hh[y > 0 ? i - 1 : i], yp = hh[y + 1 < ny ? i + 1 : i];
+  const float4 f = fld[g];
+  const float r = T.has_root ? root[g] : 0.0f;
+  pack[4 * g] = make_float4(hc, xm, xp, ym);
+  pack[4 * g + 1] = make_float4(yp, f.x, f.y, f.z);
+  pack[4 * g + 2] = make_float4(f.w, r, 0.0f, 0.0f);
+}
+
 // Count pass over every drop of every terrain; grid-stride, so the grid may
 // be smaller than the wave (a persistent launch).
 __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
@@ -301,6 +338,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.count = w.count + T.cell_off;
       a.ticket = w.ticket;
       a.slog = w.slog;
+      a.pack = w.pack ? w.pack + 4 * T.cell_off : nullptr;
       a.nx = T.nx;
       a.ny = T.ny;
       a.drops = w.total_drops;  // log stride
