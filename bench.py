@@ -54,19 +54,30 @@ def roofline(tm: dict, cycles: int, steps: int, workload: str) -> dict:
                      "achieved": steps / cycles * b / avg_s / 1e9}
     dom = max(per, key=lambda k: per[k]["ms_avg"])
     d = per[dom]
-    traffic = None
+    traffic, source = None, None
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
-            entry = json.load(f).get(workload, {}).get(dom)
+            wl = json.load(f).get(workload, {})
+        entry = wl.get(dom)
         if entry:
-            traffic = entry["bytes_per_launch"]
+            live = steps / cycles
+            source = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated: "
+                      "profiles/hbm_traffic.json")
+            if abs(live / wl["steps_per_launch"] - 1.0) < 1e-9:
+                traffic = entry["bytes_per_launch"]
+                source += " (same command, same particle-steps per launch)"
+            else:
+                # The counters were read on a run with another number of
+                # particle-steps per launch: scale by the measured bytes per step.
+                traffic = entry["bytes_per_step"] * live
+                source += (f" (measured {entry['bytes_per_step']:.1f} B per particle-step at "
+                           f"{wl['steps_per_launch']:.4g} steps/launch, scaled to this run)")
     out = {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": d["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
            "bytes_per_step": d["bytes_per_step"], "kernels": per}
-    if traffic is not None:
-        out["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload's "
-                                 "default command, copy-calibrated: profiles/hbm_traffic.json")
+    if source:
+        out["traffic_source"] = source
     return out
 
 

@@ -239,6 +239,12 @@ def _torch_sync():
         torch.cuda.synchronize()
 
 
+def _cycle_times(ms) -> dict:
+    """Slots of shy_time_cycles: total, then the five kernel phases."""
+    return {"ms_total": ms[0], "ms_pack": ms[1], "ms_count_walk": ms[2],
+            "ms_scan": ms[3], "ms_replay": ms[4], "ms_reduce_cascade": ms[5]}
+
+
 def _check(ctx, rc: int):
     if rc != SHY_OK:
         msg = lib().shy_last_error(ctx)
@@ -490,10 +496,9 @@ class World:
     def time_cycles(self, cycles: int, drops: int) -> dict:
         """Run ``cycles`` erode calls with HIP events between kernels on the
         context's stream; returns milliseconds per kernel, summed."""
-        ms = (C.c_float * 5)()
+        ms = (C.c_float * 6)()
         _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, drops, ms))
-        return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scan": ms[2],
-                "ms_replay": ms[3], "ms_reduce_cascade": ms[4]}
+        return _cycle_times(ms)
 
 
 class Drop:
@@ -581,7 +586,6 @@ class Ensemble(World):
         return out
 
     def time_cycles(self, cycles: int) -> dict:
-        ms = (C.c_float * 5)()
+        ms = (C.c_float * 6)()
         _check(self._ctx, lib().shy_time_cycles(self._ctx, cycles, -1, ms))
-        return {"ms_total": ms[0], "ms_count_walk": ms[1], "ms_scan": ms[2],
-                "ms_replay": ms[3], "ms_reduce_cascade": ms[4]}
+        return _cycle_times(ms)

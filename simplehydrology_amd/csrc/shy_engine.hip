@@ -259,8 +259,8 @@ int launch_cascade(shy_ctx* c, const CascArgs& a) {
 }
 
 // One cycle of every terrain.  ev, when given, receives events after the
-// count walk, the scan and the commit walk.  The wave buffers must already
-// hold wave_steps(c).
+// walk-record pack, the count walk, the scan and the commit replay.  The wave
+// buffers must already hold wave_steps(c).
 int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   WaveArgs w = wave_args(c, p);
   const int bs = c->tune.walk_block;
@@ -273,10 +273,14 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (walk) {
     pack_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, n, c->h[p],
                                                                      c->fld, c->root, c->pack);
-    walk_kernel<<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  if (walk) {
+    walk_kernel<<<grid, bs, 0, c->stream>>>(w);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   if (walk) {
     scan_sums_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, n, c->bsum);
     scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->bsum, nb);
@@ -285,13 +289,13 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
                                                        nter);
     HIPCHK(c, hipGetLastError());
   }
-  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (walk) {
     replay_kernel<<<(c->total_drops + kReplayBlock - 1) / kReplayBlock, kReplayBlock, 0,
                     c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
-  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   reduce_kernel<<<(n + kReduceCells - 1) / kReduceCells, kReduceCells, 0, c->stream>>>(
       c->count, c->osc, c->rec, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   HIPCHK(c, hipGetLastError());
@@ -386,7 +390,8 @@ int erode_many(shy_ctx* c, int cycles, bool use_graph) {
 
 int time_cycles(shy_ctx* c, int cycles, float* ms) {
   RETURN_IF(ensure_wave(c));
-  const size_t ne = 4 * (size_t)cycles + 1;
+  constexpr int kSlots = 5;  // pack, count walk, scan, replay, reduce + stencil
+  const size_t ne = kSlots * (size_t)cycles + 1;
   std::vector<hipEvent_t> ev(ne, nullptr);
   auto cleanup = [&]() {
     for (hipEvent_t e : ev)
@@ -400,20 +405,20 @@ int time_cycles(shy_ctx* c, int cycles, float* ms) {
   }
   hipError_t e = hipEventRecord(ev[0], c->stream);
   for (int k = 0; k < cycles && e == hipSuccess; ++k) {
-    int r = launch_cycle(c, parity(c), &ev[4 * k + 1]);
+    int r = launch_cycle(c, parity(c), &ev[kSlots * k + 1]);
     if (r) {
       cleanup();
       return r;
     }
     c->cycle += 1;
-    e = hipEventRecord(ev[4 * k + 4], c->stream);
+    e = hipEventRecord(ev[kSlots * k + kSlots], c->stream);
   }
   if (e == hipSuccess) e = hipEventSynchronize(ev.back());
-  float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  float part[kSlots] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   for (int k = 0; k < cycles && e == hipSuccess; ++k) {
-    for (int s = 0; s < 4 && e == hipSuccess; ++s) {
+    for (int s = 0; s < kSlots && e == hipSuccess; ++s) {
       float t = 0.0f;
-      e = hipEventElapsedTime(&t, ev[4 * k + s], ev[4 * k + s + 1]);
+      e = hipEventElapsedTime(&t, ev[kSlots * k + s], ev[kSlots * k + s + 1]);
       part[s] += t;
     }
   }
@@ -422,7 +427,7 @@ int time_cycles(shy_ctx* c, int cycles, float* ms) {
   cleanup();
   if (e != hipSuccess) return fail(c, SHY_ERR_HIP, std::string("shy_time_cycles: ") + hipGetErrorString(e));
   ms[0] = tot;
-  for (int s = 0; s < 4; ++s) ms[1 + s] = part[s];
+  for (int s = 0; s < kSlots; ++s) ms[1 + s] = part[s];
   return SHY_OK;
 }
 
@@ -1385,7 +1390,7 @@ int shy_set_walk_block(shy_ctx* c, int threads) {
   return shy_set_tuning(c, &t);
 }
 
-int shy_time_cycles(shy_ctx* c, int cycles, int drops, float ms[5]) {
+int shy_time_cycles(shy_ctx* c, int cycles, int drops, float ms[6]) {
   if (!c || !ms || cycles < 1) return fail(c, SHY_ERR_ARG, "shy_time_cycles: bad argument");
   hipSetDevice(c->device);
   if (drops >= 0) {
@@ -1441,7 +1446,7 @@ int shy_autotune(shy_ctx* c, int cycles, shy_tuning* chosen, float* best_ms) {
   for (const shy_tuning& t : cand) {
     c->tune = t;
     RETURN_IF(layout(c));
-    float ms[5];
+    float ms[6];
     RETURN_IF(time_cycles(c, 1, ms));  // warm-up
     RETURN_IF(restore());
     RETURN_IF(time_cycles(c, cycles, ms));
