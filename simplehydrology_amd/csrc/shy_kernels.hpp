@@ -239,12 +239,14 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       *end_cell = i0;
       break;
     }
-    // step k of this drop (k = age) holds ticket t on its start cell
+    // step k of this drop (k = age) holds ticket t on its start cell.  The
+    // ticket is stored only after the next cell's record has been loaded:
+    // vector memory operations complete in issue order, so one wait then
+    // covers the atomic's round trip and the load together.
     const size_t klog = (size_t)age * a.drops + lane_drop;
-    if (COUNT) {
-      a.ticket[klog] =
-          __hip_atomic_fetch_add(&a.count[i0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    int tk = 0;
+    if (COUNT)
+      tk = __hip_atomic_fetch_add(&a.count[i0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ++steps;
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
@@ -261,6 +263,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       // everything of the step that the snapshot fixes; the tracks
       // (water.h:115-117) use the volume before this step's evaporation
       a.slog[klog] = make_int4(i0, __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
+      a.ticket[klog] = tk;
     }
     if (TRACE) {
       const float ero = effD * (c_eq - sed);  // water.h:128-132
