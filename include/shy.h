@@ -312,7 +312,8 @@ int shy_autotune(shy_ctx* ctx, int cycles, shy_tuning* chosen, float* best_ms);
    HIP events on the context's stream between kernels.  drops >= 0 sets the
    drops of a single-terrain context first; drops < 0 keeps every terrain's
    own.  ms[0] = wall time of the whole sequence; ms[1..5] = summed durations
-   of the walk-record pack, the count walk, the scan, the commit replay and the
+   of the walk-record pack, the count walk, the partition of the step log by
+   cell tile with the per-tile counts, the commit replay and the per-tile
    reduce + cascade/blend stencil, in milliseconds. */
 int shy_time_cycles(shy_ctx* ctx, int cycles, int drops, float ms[6]);
 

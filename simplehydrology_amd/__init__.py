@@ -242,7 +242,7 @@ def _torch_sync():
 def _cycle_times(ms) -> dict:
     """Slots of shy_time_cycles: total, then the five kernel phases."""
     return {"ms_total": ms[0], "ms_pack": ms[1], "ms_count_walk": ms[2],
-            "ms_scan": ms[3], "ms_replay": ms[4], "ms_reduce_cascade": ms[5]}
+            "ms_partition": ms[3], "ms_replay": ms[4], "ms_reduce_cascade": ms[5]}
 
 
 def _check(ctx, rc: int):

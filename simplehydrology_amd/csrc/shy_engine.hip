@@ -7,20 +7,22 @@
 //   fld         float4  {momentumx, momentumy, discharge, erf(0.4 discharge)}
 //   root        float   root density
 //   visit[2]    uint8   cascade sites, ping-pong
-//   count       int32   walk steps per start cell (tickets); cleared by reduce
 //   first       int32   drops whose first step starts at the cell; cleared by
-//                       the scan
-//   osc         int2    {bucket start (prefix sum of count), burst scale} per cell
-//   dh          int32   summed walk commits per cell, 2^-20 (written by reduce)
-//   trk[3]      int64   summed tracks per cell, 2^-24 (written by reduce; stay
+//                       tile_count_kernel
+//   scl         float   burst scale per cell (tile_count_kernel)
+//   dh          int32   summed walk commits per cell, 2^-20 (tile_reduce_kernel)
+//   trk[3]      int64   summed tracks per cell, 2^-24 (tile_reduce_kernel; stay
 //                       readable until the next cycle, as the reference's
 //                       *_track fields do)
-//   deposit     int32   terminal deposits, 2^-20; cleared by reduce
-// and per wave (sized by total drops x (max maxAge + 1) steps):
-//   ticket      int32   [step][drop] ticket log of the count pass
-//   slog        int4    [step][drop] step log of the count pass
-//   rec         int4    per-step commit records, bucketed by start cell
-//   dlog        int4    [drop] how each walk ended
+//   deposit     int32   terminal deposits, 2^-20; cleared by tile_reduce_kernel
+// and per wave (sized by total drops x (max maxAge + 1) log slots):
+//   cel         int32   [step][drop] start cell of each step (count walk)
+//   slog        int4    [step][drop] step log of the count walk
+//   qb          int32   [step][drop] height commit of each step (replay)
+//   ent         int4    the valid slots grouped by cell tile (partition)
+//   vq          int32   [terrain][step] quantised volume track of a step
+//   dlog, wsteps int4, int32 [drop] how each walk ended, its steps
+//   hist        int32   [tile][block] partition counts, then offsets
 //   pack        float4x4 per cell: the count walk's one-line record of the
 //                       cycle's start (5 heights, flow, root), by pack_kernel
 // A device table of Terrain entries maps drops, cells and cascade tiles to
@@ -53,18 +55,30 @@ struct shy_ctx {
   float4* fld = nullptr;
   float* root = nullptr;
   uint8_t* visit[2] = {nullptr, nullptr};
-  int32_t* count = nullptr;
   int32_t* first = nullptr;
-  int2* osc = nullptr;
+  float* scl = nullptr;
   int32_t* dh = nullptr;
   long long* trk = nullptr;  // 3*n: d, mx, my
   int32_t* deposit = nullptr;
-  int32_t* bsum = nullptr;
-  int32_t* ticket = nullptr;
-  int4* rec = nullptr;
-  int4* slog = nullptr;  // [step][drop] step log of the count pass
-  size_t wave_cap = 0;   // steps the ticket log, step log and buckets can hold
+  int32_t* cel = nullptr;    // [step][drop] start cell log of the count walk
+  int4* slog = nullptr;      // [step][drop] step log of the count walk
+  int32_t* qb = nullptr;     // [step][drop] height commits of the replay
+  int4* ent = nullptr;       // partition entries in tile order: {slot, cell, tracks}
+  int32_t* vq = nullptr;     // [terrain][row] quantised volume track per step
+  size_t vq_cap = 0;
+  int vq_rows = 0;
+  bool vq_dirty = true;      // parameters changed since vq was built
+  size_t wave_cap = 0;       // log slots the wave buffers can hold
+  int32_t* hist = nullptr;   // [tile][block] partition counts, then offsets
+  int32_t* hbsum = nullptr;  // block sums of the hist scan
+  size_t hist_cap = 0;       // ints hist holds (hbsum holds hist_cap / 1024 + 1)
+  int part_blocks = 0;       // partition geometry of the current wave
+  int part_dwidth = 0;       // drops per partition block
+  int part_rows = 0;         // log rows
+  int part_window = 0;       // largest tile window of a partition block
+  int ntiles = 0;            // cell tiles of kTileCells cells
   int4* dlog = nullptr;  // [drop] how each walk ended
+  int32_t* wsteps = nullptr;  // [drop] steps of each walk
   float4* pack = nullptr;  // per cell, 4 x float4: the count walk's record
   size_t dlog_cap = 0;
   long long* tsteps = nullptr;
@@ -160,6 +174,9 @@ void drop_graph(shy_ctx* c) {
   c->graph = nullptr;
 }
 
+// Cell tiles the partition's LDS histogram can hold (128 KiB).
+constexpr int kPartMaxTiles = 32768;
+
 // Offsets of every terrain's cells, drops and cascade tiles; uploads the table.
 int layout(shy_ctx* c) {
   long long cells = 0, drops = 0, tiles = 0;
@@ -176,8 +193,12 @@ int layout(shy_ctx* c) {
     if (drops > INT32_MAX || tiles > INT32_MAX)
       return fail(c, SHY_ERR_ARG, "ensemble too large: drops and cascade tiles must stay below 2^31");
   }
+  if ((cells + kTileCells - 1) / kTileCells > kPartMaxTiles)
+    return fail(c, SHY_ERR_ARG, "context too large: at most 2^26 cells over all terrains");
   c->total_drops = (int)drops;
   c->total_tiles = (int)tiles;
+  c->ntiles = (int)((cells + kTileCells - 1) / kTileCells);
+  c->vq_dirty = true;
   drop_graph(c);
   if (c->d_ter) {
     HIPCHK(c, hipMemcpyAsync(c->d_ter, c->ter.data(), c->ter.size() * sizeof(Terrain),
@@ -193,34 +214,112 @@ size_t wave_steps(const shy_ctx* c) {
   return (size_t)c->total_drops * ((size_t)age + 1);
 }
 
-// Size the logs and buckets for the current drops (never inside a capture).
+// Partition blocks: at most kPartMaxBlocks, each a whole number of waves of
+// drops over all log rows (the [tile][block] matrix stays small).
+constexpr int kPartMaxBlocks = 512;
+
+// Largest tile window (tiles of the terrains of its first and last drop) of
+// any partition block.
+int part_max_window(const shy_ctx* c, int dwidth, int nblocks) {
+  int best = 1;
+  auto terrain_of = [&](int g) {
+    int lo = 0, hi = (int)c->ter.size() - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (c->ter[mid].drop_off <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  };
+  for (int b = 0; b < nblocks; ++b) {
+    const int g0 = b * dwidth;
+    const int g1 = std::min(g0 + dwidth, c->total_drops) - 1;
+    if (g1 < g0) continue;
+    const Terrain& t0 = c->ter[terrain_of(g0)];
+    const Terrain& t1 = c->ter[terrain_of(g1)];
+    const long long c1 = t1.cell_off + (long long)t1.nx * t1.ny - 1;
+    best = std::max(best, (int)((c1 >> kTileBits) - (t0.cell_off >> kTileBits) + 1));
+  }
+  return best;
+}
+
+// Size the logs, the partition and the per-drop buffers for the current
+// drops (never inside a capture).
 int ensure_wave(shy_ctx* c) {
   const size_t need = wave_steps(c);
   const size_t need_d = (size_t)std::max(c->total_drops, 1);
-  if (need > (size_t)INT32_MAX)
-    return fail(c, SHY_ERR_ARG, "drops x (maxAge+1) exceeds 2^31 steps per wave");
-  if (need <= c->wave_cap && need_d <= c->dlog_cap) return SHY_OK;
+  if (need > (size_t)INT32_MAX - 16384)  // slot indices plus a round stay in int
+    return fail(c, SHY_ERR_ARG, "drops x (maxAge+1) exceeds 2^31 log slots per wave");
+  const int drops = std::max(c->total_drops, 1);
+  const int rows = (int)(need / (size_t)drops);
+  const int per = (drops + kPartMaxBlocks - 1) / kPartMaxBlocks;
+  const int dwidth = std::max(kWaveDrops, (per + kWaveDrops - 1) / kWaveDrops * kWaveDrops);
+  const int nb = (drops + dwidth - 1) / dwidth;
+  const size_t hist_need = (size_t)c->ntiles * nb + 1;
+  if (nb != c->part_blocks || dwidth != c->part_dwidth || rows != c->part_rows)
+    drop_graph(c);  // a captured graph holds the old geometry
+  c->part_blocks = nb;
+  c->part_dwidth = dwidth;
+  c->part_rows = rows;
+  c->part_window = part_max_window(c, dwidth, nb);
+  {
+    // the volume-track table: rows = max maxAge + 1 over the terrains
+    float age = 0.0f;
+    for (const Terrain& t : c->ter) age = std::max(age, t.P.maxAge);
+    const int rows = (int)age + 1;
+    const size_t vq_need = (size_t)rows * c->ter.size();
+    if (vq_need > c->vq_cap) {
+      drop_graph(c);
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (c->vq) hipFree(c->vq);
+      c->vq = nullptr;
+      c->vq_cap = 0;
+      HIPCHK(c, hipMalloc(&c->vq, vq_need * sizeof(int32_t)));
+      c->vq_cap = vq_need;
+      c->vq_dirty = true;
+    }
+    if (c->vq_dirty || rows != c->vq_rows) {
+      if (rows != c->vq_rows) drop_graph(c);
+      c->vq_rows = rows;
+      const int nter = (int)c->ter.size();
+      volq_kernel<<<(nter + 63) / 64, 64, 0, c->stream>>>(c->d_ter, nter, rows, c->vq);
+      HIPCHK(c, hipGetLastError());
+      c->vq_dirty = false;
+    }
+  }
+  if (need <= c->wave_cap && need_d <= c->dlog_cap && hist_need <= c->hist_cap) return SHY_OK;
   drop_graph(c);  // a captured graph holds the old buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (need > c->wave_cap) {
-    if (c->ticket) hipFree(c->ticket);
-    if (c->rec) hipFree(c->rec);
-    if (c->slog) hipFree(c->slog);
-    c->ticket = nullptr;
-    c->rec = nullptr;
-    c->slog = nullptr;
+    void* old[] = {c->cel, c->slog, c->qb, c->ent};
+    for (void* q : old)
+      if (q) hipFree(q);
+    c->cel = c->qb = nullptr;
+    c->slog = c->ent = nullptr;
     c->wave_cap = 0;
-    HIPCHK(c, hipMalloc(&c->ticket, need * sizeof(int32_t)));
-    HIPCHK(c, hipMalloc(&c->rec, need * sizeof(int4)));
+    HIPCHK(c, hipMalloc(&c->cel, need * sizeof(int32_t)));
     HIPCHK(c, hipMalloc(&c->slog, need * sizeof(int4)));
+    HIPCHK(c, hipMalloc(&c->qb, need * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc(&c->ent, need * sizeof(int4)));
     c->wave_cap = need;
   }
   if (need_d > c->dlog_cap) {
     if (c->dlog) hipFree(c->dlog);
+    if (c->wsteps) hipFree(c->wsteps);
     c->dlog = nullptr;
+    c->wsteps = nullptr;
     c->dlog_cap = 0;
     HIPCHK(c, hipMalloc(&c->dlog, need_d * sizeof(int4)));
+    HIPCHK(c, hipMalloc(&c->wsteps, need_d * sizeof(int32_t)));
     c->dlog_cap = need_d;
+  }
+  if (hist_need > c->hist_cap) {
+    if (c->hist) hipFree(c->hist);
+    if (c->hbsum) hipFree(c->hbsum);
+    c->hist = c->hbsum = nullptr;
+    c->hist_cap = 0;
+    HIPCHK(c, hipMalloc(&c->hist, hist_need * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc(&c->hbsum, (hist_need / kScanBlock + 1) * sizeof(int32_t)));
+    c->hist_cap = hist_need;
   }
   return SHY_OK;
 }
@@ -233,13 +332,13 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.h = c->h[p];
   w.fld = c->fld;
   w.root = c->root;
-  w.count = c->count;
   w.first = c->first;
-  w.ticket = c->ticket;
-  w.osc = c->osc;
-  w.rec = c->rec;
+  w.cel = c->cel;
   w.slog = c->slog;
+  w.qb = c->qb;
+  w.scl = c->scl;
   w.dlog = c->dlog;
+  w.steps = c->wsteps;
   w.pack = c->pack;
   w.deposit = c->deposit;
   w.visit = c->visit[p];
@@ -259,15 +358,14 @@ int launch_cascade(shy_ctx* c, const CascArgs& a) {
 }
 
 // One cycle of every terrain.  ev, when given, receives events after the
-// walk-record pack, the count walk, the scan and the commit replay.  The wave
-// buffers must already hold wave_steps(c).
+// walk-record pack, the count walk, the partition + tile counts and the
+// commit replay.  ensure_wave(c) must have sized the wave buffers.
 int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   WaveArgs w = wave_args(c, p);
   const int bs = c->tune.walk_block;
   const int need = (c->total_drops + bs - 1) / bs;
   const int grid = c->tune.walk_grid > 0 ? std::min(c->tune.walk_grid, need) : need;
   const size_t n = c->n;
-  const int nb = (int)((n + kScanBlock - 1) / kScanBlock);
   const int nter = (int)c->ter.size();
   const bool walk = c->total_drops > 0;
   if (walk) {
@@ -281,14 +379,42 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  if (walk) {
-    scan_sums_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, n, c->bsum);
-    scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->bsum, nb);
-    scan_cells_kernel<<<nb, kScanBlock, 0, c->stream>>>(c->count, c->bsum, n, c->osc, c->first,
-                                                       c->visit[p], c->fld, c->root, c->d_ter,
-                                                       nter);
-    HIPCHK(c, hipGetLastError());
-  }
+  // partition of the valid log slots by cell tile (empty without drops)
+  PartArgs pa;
+  pa.cel = c->cel;
+  pa.slog = c->slog;
+  pa.steps = c->wsteps;
+  pa.ter = c->d_ter;
+  pa.nter = nter;
+  pa.hist = c->hist;
+  pa.ent = c->ent;
+  pa.rows = walk ? c->part_rows : 0;
+  pa.drops = std::max(c->total_drops, 1);
+  pa.dwidth = c->part_dwidth;
+  pa.nblocks = c->part_blocks;
+  pa.ntiles = c->ntiles;
+  pa.tb = kTileBits;
+  const size_t nh = (size_t)c->ntiles * c->part_blocks;
+  const int nbh = (int)((nh + kScanBlock - 1) / kScanBlock);
+  if (c->ntiles <= 4096)
+    part_hist_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  else
+    part_hist_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  scan_sums_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, nh, c->hbsum);
+  scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
+  scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);
+  if (c->part_window <= 1024)
+    part_scatter_staged_kernel<1024, 8192><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  else if (c->part_window <= 4096)
+    part_scatter_staged_kernel<4096, 4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  else if (c->ntiles <= 4096)
+    part_scatter_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  else
+    part_scatter_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  tile_count_kernel<<<c->ntiles, 1024, 0, c->stream>>>(c->hist, pa.nblocks, c->ent, n, c->scl,
+                                                       c->first, c->visit[p], c->fld, c->root,
+                                                       c->d_ter, nter);
+  HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (walk) {
     replay_kernel<<<(c->total_drops + kReplayBlock - 1) / kReplayBlock, kReplayBlock, 0,
@@ -296,8 +422,9 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
-  reduce_kernel<<<(n + kReduceCells - 1) / kReduceCells, kReduceCells, 0, c->stream>>>(
-      c->count, c->osc, c->rec, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
+  tile_reduce_kernel<<<c->ntiles, 1024, 0, c->stream>>>(
+      c->hist, pa.nblocks, c->ent, c->qb, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
+      c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   HIPCHK(c, hipGetLastError());
   CascArgs a;
   const int q = p ^ 1;
@@ -390,7 +517,7 @@ int erode_many(shy_ctx* c, int cycles, bool use_graph) {
 
 int time_cycles(shy_ctx* c, int cycles, float* ms) {
   RETURN_IF(ensure_wave(c));
-  constexpr int kSlots = 5;  // pack, count walk, scan, replay, reduce + stencil
+  constexpr int kSlots = 5;  // pack, count walk, partition + tile counts, replay, reduce + stencil
   const size_t ne = kSlots * (size_t)cycles + 1;
   std::vector<hipEvent_t> ev(ne, nullptr);
   auto cleanup = [&]() {
@@ -494,9 +621,9 @@ int shy_destroy(shy_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->count,
-                  c->first, c->osc, c->dh, c->trk, c->deposit, c->bsum, c->ticket, c->rec,
-                  c->slog, c->dlog, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
+  void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
+                  c->scl, c->dh, c->trk, c->deposit, c->cel, c->slog, c->qb, c->ent,
+                  c->vq, c->hist, c->hbsum, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)
@@ -551,21 +678,18 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
     shy_destroy(c);
     return SHY_ERR_ARG;
   }
-  const size_t nb = (n + kScanBlock - 1) / kScanBlock;
   for (int k = 0; k < 2; ++k) {
     if ((e = hipMalloc(&c->h[k], n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc h");
     if ((e = hipMalloc(&c->visit[k], n)) != hipSuccess) return bail(e, "hipMalloc visit");
   }
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
-  if ((e = hipMalloc(&c->count, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc count");
   if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
-  if ((e = hipMalloc(&c->osc, n * sizeof(int2))) != hipSuccess) return bail(e, "hipMalloc osc");
+  if ((e = hipMalloc(&c->scl, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc scl");
   if ((e = hipMalloc(&c->pack, 4 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");
-  if ((e = hipMalloc(&c->bsum, nb * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc bsum");
   if ((e = hipMalloc(&c->tsteps, nter * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc tsteps");
   if ((e = hipMalloc(&c->d_ter, nter * sizeof(Terrain))) != hipSuccess) return bail(e, "hipMalloc terrains");
   if ((e = hipMalloc(&c->state, sizeof(DeviceState))) != hipSuccess) return bail(e, "hipMalloc state");
@@ -579,9 +703,8 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   }
   hipMemsetAsync(c->fld, 0, n * sizeof(float4), c->stream);
   hipMemsetAsync(c->root, 0, n * sizeof(float), c->stream);
-  hipMemsetAsync(c->count, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->first, 0, n * sizeof(int32_t), c->stream);
-  hipMemsetAsync(c->osc, 0, n * sizeof(int2), c->stream);
+  hipMemsetAsync(c->scl, 0, n * sizeof(float), c->stream);
   hipMemsetAsync(c->dh, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
   hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);

@@ -4,18 +4,21 @@
 // drops that all read the map as it stood when the cycle began:
 //
 //  walk_kernel           one drop per lane walks its trajectory; for every
-//                        step it takes a ticket (relaxed int32 atomic with
-//                        return) on the step's start cell and logs, coalesced
-//                        as [step][drop], the ticket and the step's
-//                        snapshot-only quantities {start cell, c_eq, tracks};
-//                        per drop it logs how the walk ended, counts its
-//                        first step's cell and marks its end cell.  The
+//                        step it logs, coalesced as [step][drop], the start
+//                        cell and the step's snapshot-only quantities {c_eq,
+//                        tracks}; per drop it logs how the walk ended, counts
+//                        its first step's cell and marks its end cell.  The
 //                        cascade sites (arrival cells, water.h:151) follow
 //                        per cell: a cell is a site iff more steps start there
 //                        than first steps, or a walk ended there.
-//  scan_*_kernel         exclusive prefix sum of the per-cell step counts
-//                        (bucket offsets), the cascade-site marks, fused with
-//                        the per-cell burst scale
+//  part_*_kernel, scan   group the valid log slots by cell tile (2048 cells):
+//                        per-block LDS histograms over tiles, an exclusive
+//                        scan of the [tile][block] matrix, then an LDS-cursor
+//                        scatter of {slot, cell, tracks} into tile order.  No
+//                        global atomic per step: the device's atomic rate, not
+//                        its bandwidth, bounded a walk that took one.
+//  tile_count_kernel     per tile, the steps starting at each cell (LDS) ->
+//                        cascade sites and the burst scale
 //                        s = (1-(1-gamma)^k)/(gamma k), gamma = min(1, effD (1 +
 //                        entrainment erf(0.4 d))): the k drops through one cell
 //                        on the same snapshot commit together what k drops in
@@ -24,17 +27,16 @@
 //                        overshoot.
 //  replay_kernel         replays each drop's logged steps: the only chain left
 //                        is the sediment recurrence (arithmetic); the log loads
-//                        stream and the per-cell gathers (bucket offset, burst
-//                        scale) do not depend on the previous step, so they are
-//                        issued ahead.  Each step's commit -- erosion/deposition
-//                        effD*cdiff*s quantised once to int32 (2^-20, moved
-//                        between map and drop as that integer) and the
-//                        discharge/momentum tracks (int32, 2^-24) -- goes with
-//                        one plain 16-byte store into the start cell's bucket
-//                        at its ticket: no read-modify-write of the map per step.
-//  reduce_kernel         per cell, sums its bucket in integers (so the result
-//                        does not depend on the order drops ran) plus the
-//                        terminal deposits.
+//                        stream and the per-cell gathers (burst scale, root) do
+//                        not depend on the previous step, so they are issued
+//                        ahead.  Each step's erosion/deposition effD*cdiff*s is
+//                        quantised once to int32 (2^-20, moved between map and
+//                        drop as that integer) and stored in log order.
+//  tile_reduce_kernel    per tile, sums in LDS the height commits (gathered by
+//                        slot) and the discharge/momentum tracks (int32, 2^-24)
+//                        of its steps in integers (so the result does not
+//                        depend on the order drops ran), plus the terminal
+//                        deposits: no read-modify-write of the map per step.
 //  cascade_blend_kernel  2D-tiled stencil: applies the walk's commits, runs the
 //                        reference cascade (world.h:90-168) of every cell a drop
 //                        arrived at (water.h:151) as a weighted Jacobi sweep
@@ -81,13 +83,13 @@ struct WaveArgs {
   const float* __restrict__ h;
   const float4* __restrict__ fld;
   const float* __restrict__ root;
-  int32_t* count;
   int32_t* first;     // drops whose first step starts at the cell
-  int32_t* ticket;
-  const int2* __restrict__ osc;  // {bucket offset, burst-scale bits} per cell
-  int4* rec;
-  int4* slog;         // [step][drop] {start cell, c_eq bits, qt(vol sx), qt(vol sy)}
+  int32_t* cel;       // [step][drop] start cell of the step (index over all terrains)
+  int4* slog;         // [step][drop] {qt(vol), c_eq bits, qt(vol sx), qt(vol sy)}
+  int32_t* qb;        // [step][drop] height commit of the step, 2^-20 (replay)
+  const float* __restrict__ scl;  // burst scale per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
+  int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
   int32_t* deposit;
   uint8_t* visit;
@@ -114,9 +116,9 @@ struct WalkArgs {
   const float* __restrict__ h;
   const float4* __restrict__ fld;  // {momentumx, momentumy, discharge, erf(0.4*discharge)}
   const float* __restrict__ root;  // nullptr when the map has no root density
-  int32_t* count;                  // steps starting at each cell (tickets)
-  int32_t* ticket;                 // [step][drop] ticket log
+  int32_t* cel;                    // [step][drop] start cell log (count pass)
   int4* slog;                      // [step][drop] step log (count pass)
+  int cell_off;                    // the terrain's first cell over all terrains
   const float4* __restrict__ pack; // packed cells, or nullptr (read h, fld, root)
   uint8_t* visit;                  // cascade sites
   DeviceState* state;
@@ -215,7 +217,7 @@ enum WalkMode { kTrace = 0, kCount = 1 };
 // One drop from spawn to termination: Drop(pos) then while(drop.descend())
 // (source/world.h:74-76, source/water.h:58-156) on the cycle's snapshot.
 // kTrace records per-step states of one drop (unscaled, as Drop::descend);
-// kCount takes the tickets and writes the step log the replay commits from.
+// kCount writes the step log that the partition and the replay read.
 // Returns the steps taken; *end and *end_cell say how and where it ended.
 template <int MODE>
 __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, int* end,
@@ -239,14 +241,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       *end_cell = i0;
       break;
     }
-    // step k of this drop (k = age) holds ticket t on its start cell.  The
-    // ticket is stored only after the next cell's record has been loaded:
-    // vector memory operations complete in issue order, so one wait then
-    // covers the atomic's round trip and the load together.
-    const size_t klog = (size_t)age * a.drops + lane_drop;
-    int tk = 0;
-    if (COUNT)
-      tk = __hip_atomic_fetch_add(&a.count[i0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const size_t klog = (size_t)age * a.drops + lane_drop;  // step k = age of this drop
     ++steps;
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
@@ -262,8 +257,9 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     if (COUNT) {
       // everything of the step that the snapshot fixes; the tracks
       // (water.h:115-117) use the volume before this step's evaporation
-      a.slog[klog] = make_int4(i0, __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
-      a.ticket[klog] = tk;
+      a.cel[klog] = a.cell_off + i0;
+      a.slog[klog] = make_int4(quant_t(vol), __float_as_int(c_eq), quant_t(vol * sx),
+                               quant_t(vol * sy));
     }
     if (TRACE) {
       const float ero = effD * (c_eq - sed);  // water.h:128-132
@@ -338,8 +334,8 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.h = w.h + T.cell_off;
       a.fld = w.fld + T.cell_off;
       a.root = T.has_root ? w.root + T.cell_off : nullptr;
-      a.count = w.count + T.cell_off;
-      a.ticket = w.ticket;
+      a.cel = w.cel;
+      a.cell_off = (int)T.cell_off;
       a.slog = w.slog;
       a.pack = w.pack ? w.pack + 4 * T.cell_off : nullptr;
       a.nx = T.nx;
@@ -367,15 +363,17 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       }
     }
     w.dlog[g] = dl;
+    w.steps[g] = dl.x;
   }
 }
 
-// The commits of every drop from its step log (see the file comment).  The
-// arithmetic is the walk's, step for step: cdiff = c_eq - sediment, one
-// quantised amount q = quant_h(effD cdiff s) moves from the map into the
-// drop, then evaporation (water.h:128-136); the terminal deposit or the
-// export ends the drop (water.h:74-82, 139-142).
-// Steps of the log in flight per lane: the log row of step k+D and the
+// The commits of every drop from its step log.  The arithmetic is the
+// walk's, step for step: cdiff = c_eq - sediment, one quantised amount
+// q = quant_h(effD cdiff s) moves from the map into the drop, then
+// evaporation (water.h:128-136); the terminal deposit or the export ends the
+// drop (water.h:74-82, 139-142).  -q is written in log order (coalesced);
+// tile_reduce_kernel sums it per cell.
+// Steps of the log in flight per lane: the log rows of step k+D and the
 // per-cell gathers of step k+D-1 are issued while step k computes.  One wave
 // per SIMD is all a 65536-drop wave fills, so the ring is the latency hiding.
 constexpr int kReplayDepth = 8;
@@ -396,35 +394,32 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
     if (dl.y == kEndSkipped) skipped += 1;
     if (dl.y <= kEndExit) {
       walked += 1;
-      const int2* __restrict__ osc = w.osc + T.cell_off;
-      const float* __restrict__ root = T.has_root ? w.root + T.cell_off : nullptr;
+      const float* __restrict__ scl = w.scl;
+      const float* __restrict__ root = T.has_root ? w.root : nullptr;
       const float dep = T.P.depositionRate, e = T.P.evapRate;
       float sed = 0.0f, vol = 1.0f;
       double gain = 0.0;
       long long wdh = 0;
       // ring slot u holds step k = k0 + u; rows k >= n are never read
       int4 L[D];
-      int tk[D], off[D];
+      int cl[D];
       float sc[D], rt[D];
 #pragma unroll
       for (int u = 0; u < D; ++u) {
         L[u] = make_int4(0, 0, 0, 0);
-        tk[u] = 0;
-        off[u] = 0;
+        cl[u] = 0;
         sc[u] = 0.0f;
         rt[u] = 0.0f;
         if (u < n) {
           L[u] = w.slog[(size_t)u * stride + g];
-          tk[u] = w.ticket[(size_t)u * stride + g];
+          cl[u] = w.cel[(size_t)u * stride + g];
         }
       }
 #pragma unroll
       for (int u = 0; u + 1 < D; ++u) {
         if (u < n) {
-          const int2 o = osc[L[u].x];
-          off[u] = o.x;
-          sc[u] = __int_as_float(o.y);
-          rt[u] = root ? root[L[u].x] : 0.0f;
+          sc[u] = scl[cl[u]];
+          rt[u] = root ? root[cl[u]] : 0.0f;
         }
       }
       for (int k0 = 0; k0 < n; k0 += D) {
@@ -436,7 +431,7 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
             if (effD < 0) effD = 0;
             const float cdiff = __int_as_float(L[u].y) - sed;
             const int32_t q = quant_h((effD * cdiff) * sc[u]);
-            w.rec[off[u] + tk[u]] = make_int4(-q, quant_t(vol), L[u].z, L[u].w);
+            w.qb[(size_t)k * stride + g] = -q;
             wdh -= q;
             sed += (float)q * kHQInv;
             const float s1 = evap_sed(sed, e);
@@ -445,14 +440,12 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
             vol = evap_vol(vol, e);
             if (k + D < n) {  // slot u now takes step k + D
               L[u] = w.slog[(size_t)(k + D) * stride + g];
-              tk[u] = w.ticket[(size_t)(k + D) * stride + g];
+              cl[u] = w.cel[(size_t)(k + D) * stride + g];
             }
             const int v = (u + D - 1) % D;  // slot of step k + D - 1 (constant once unrolled)
             if (k + D - 1 < n) {
-              const int2 o = osc[L[v].x];
-              off[v] = o.x;
-              sc[v] = __int_as_float(o.y);
-              rt[v] = root ? root[L[v].x] : 0.0f;
+              sc[v] = scl[cl[v]];
+              rt[v] = root ? root[cl[v]] : 0.0f;
             }
           }
         }
@@ -552,99 +545,338 @@ __global__ void __launch_bounds__(kScanBlock) scan_top_kernel(int32_t* bsum, int
   }
 }
 
-// bucket offset and burst scale per cell; cascade sites where more steps
-// start than first steps (an arrival, water.h:151); clears the first counts
-__global__ void __launch_bounds__(kScanBlock) scan_cells_kernel(
-    const int32_t* __restrict__ count, const int32_t* __restrict__ bsum, size_t n, int2* osc,
-    int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
-    const float* __restrict__ root, const Terrain* __restrict__ ter, int nter) {
+// in-place exclusive scan of v[0, n) from the per-block sums of
+// scan_sums_kernel after scan_top_kernel; v[n] receives the total
+__global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(int32_t* v,
+                                                                const int32_t* __restrict__ bsum,
+                                                                size_t n) {
   __shared__ int tmp[kScanBlock];
   const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
-  const int k = i < n ? count[i] : 0;
+  const int x = i < n ? v[i] : 0;
   int total;
-  const int ex = block_exclusive_scan(k, tmp, total);
-  if (i >= n) return;
-  const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
-  const shy_params& P = ter[t].P;
-  float effD = P.depositionRate * (1.0f - (ter[t].has_root ? root[i] : 0.0f));
-  if (effD < 0) effD = 0;
-  osc[i] = make_int2(bsum[blockIdx.x] + ex,
-                     __float_as_int(burst_scale(k, effD, P.entrainment, fld[i].w)));
-  const int f = first[i];
-  if (k > f) visit[i] = 1;
-  if (f) first[i] = 0;
+  const int ex = block_exclusive_scan(x, tmp, total);
+  if (i < n) v[i] = bsum[blockIdx.x] + ex;
+  if (i + 1 == n) v[n] = bsum[blockIdx.x] + ex + x;
 }
 
-// Per cell: sum its bucket and terminal deposits; clear the tickets.  The
-// buckets of a block's kReduceCells cells are one contiguous record range.
-// It is staged through LDS in chunks with coalesced loads; each thread then
-// sums an equal slice of the chunk, flushing a run sum into the cell's LDS
-// accumulator whenever it crosses a bucket boundary, so a cell holding
-// thousands of records (a river) is shared by the whole block.
-constexpr int kReduceCells = 256;
-constexpr int kReduceChunk = 2048;  // records per LDS chunk (32 KiB)
+// ------------------------------------------------- partition by cell tile
+// The step log is [step][drop]; the per-cell sums need it grouped by cell.
+// Cells are cut into tiles of 2^tb consecutive indices (over all terrains).
+// Each of nblocks blocks owns dwidth consecutive drops over all rows of the
+// log (slot s of a block: row s / width, drop g0 + s % width):
+// part_hist_kernel counts its valid slots per tile in LDS, the scan turns
+// the [tile][block] matrix into exclusive offsets, and the scatter places
+// every valid slot at its tile's next free entry (an LDS cursor per tile),
+// one 16-byte entry per step.  Order inside a tile does not matter: every
+// consumer sums integers.  A block's drops belong to a few consecutive
+// terrains, so its entries fall in a short window of tiles.
+struct PartArgs {
+  const int32_t* __restrict__ cel;    // [slot] start cell
+  const int4* __restrict__ slog;      // [slot] {qt(vol), c_eq, qt(vol sx), qt(vol sy)}
+  const int32_t* __restrict__ steps;  // [drop] valid rows of the log
+  const Terrain* __restrict__ ter;
+  int nter;
+  int32_t* hist;   // [tile][block] counts, then exclusive offsets; [ntiles*nblocks] = total
+  int4* ent;       // [entry] {slot, start cell, qt(vol sx), qt(vol sy)}, tile order
+  int rows;        // log rows (0: no walk this cycle)
+  int drops;       // log stride
+  int dwidth;      // drops per block
+  int nblocks;
+  int ntiles;
+  int tb;          // log2 cells per tile
+};
 
-__global__ void __launch_bounds__(kReduceCells) reduce_kernel(
-    int32_t* count, const int2* __restrict__ osc, const int4* __restrict__ rec,
-    int32_t* deposit, size_t n, int32_t* dh, long long* td, long long* tmx, long long* tmy) {
-  __shared__ int off[kReduceCells + 1];
-  __shared__ unsigned long long acc[4][kReduceCells];
-  __shared__ int4 buf[kReduceChunk];
-  const int t = threadIdx.x;
-  const size_t c0 = (size_t)blockIdx.x * kReduceCells;
-  const int m = (int)((n - c0) < (size_t)kReduceCells ? (n - c0) : (size_t)kReduceCells);
-  if (t < m) off[t] = osc[c0 + t].x;
-  if (t == m - 1) off[m] = osc[c0 + t].x + count[c0 + t];
-#pragma unroll
-  for (int f = 0; f < 4; ++f) acc[f][t] = 0;
+// the block's drops and its slot count
+__device__ inline void part_block(const PartArgs& a, int& g0, int& width, int& nslots) {
+  g0 = blockIdx.x * a.dwidth;
+  width = a.drops - g0 < a.dwidth ? a.drops - g0 : a.dwidth;
+  if (width < 0) width = 0;
+  nslots = width * a.rows;
+}
+
+// slot s of the block -> log index k, or -1 when the drop took fewer steps
+__device__ inline int part_slot(const PartArgs& a, int g0, int width, int s) {
+  const int row = s / width, g = g0 + (s - row * width);
+  return row < a.steps[g] ? row * a.drops + g : -1;
+}
+
+template <int MAXT>
+__global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
+  __shared__ int h[MAXT];
+  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x) h[t] = 0;
   __syncthreads();
-  const int r0 = off[0], r1 = off[m];
-  constexpr int kPer = kReduceChunk / kReduceCells;
-  for (int base = r0; base < r1; base += kReduceChunk) {
-    const int len = r1 - base < kReduceChunk ? r1 - base : kReduceChunk;
-    for (int k = t; k < len; k += kReduceCells) buf[k] = rec[base + k];
+  int g0, width, ns;
+  part_block(a, g0, width, ns);
+  for (int s = (int)threadIdx.x; s < ns; s += 4 * blockDim.x) {
+    int c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = s + u * (int)blockDim.x;
+      const int k = x < ns ? part_slot(a, g0, width, x) : -1;
+      c[u] = k >= 0 ? a.cel[k] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) atomicAdd(&h[c[u] >> a.tb], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x)
+    a.hist[(size_t)t * a.nblocks + blockIdx.x] = h[t];
+}
+
+// Scatter without staging (any tile window): one 16-byte entry per step at
+// its tile's LDS cursor.
+template <int MAXT>
+__global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
+  __shared__ int cur[MAXT];
+  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x)
+    cur[t] = a.hist[(size_t)t * a.nblocks + blockIdx.x];
+  __syncthreads();
+  int g0, width, ns;
+  part_block(a, g0, width, ns);
+  for (int s = (int)threadIdx.x; s < ns; s += blockDim.x) {
+    const int k = part_slot(a, g0, width, s);
+    if (k >= 0) {
+      const int c = a.cel[k];
+      const int4 sl = a.slog[k];
+      const int e = atomicAdd(&cur[c >> a.tb], 1);
+      a.ent[e] = make_int4(k, c, sl.z, sl.w);
+    }
+  }
+}
+
+// the block's window of tiles: those of the terrains of its first and last drop
+__device__ inline void part_window(const PartArgs& a, int g0, int width, int& lo, int& n) {
+  auto dkey = [&](int q) { return (long long)a.ter[q].drop_off; };
+  const int t0 = find_terrain(a.nter, (long long)g0, dkey);
+  const int t1 = find_terrain(a.nter, (long long)(g0 + (width > 0 ? width : 1) - 1), dkey);
+  const long long c0 = a.ter[t0].cell_off;
+  const long long c1 = a.ter[t1].cell_off + (long long)a.ter[t1].nx * a.ter[t1].ny - 1;
+  lo = (int)(c0 >> a.tb);
+  n = (int)(c1 >> a.tb) - lo + 1;
+}
+
+// Exclusive scan of v[0, m) into out[0, m) by a block of 1024 threads,
+// m <= 4096 (each thread a run of up to four); wsum holds 16 ints.  Every
+// thread of the block must call it.
+__device__ inline void block_scan_runs(const int* v, int* out, int m, int* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int per = (m + 1023) >> 10;
+  int loc[4];
+  int sum = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i = tid * per + j;
+    loc[j] = j < per && i < m ? v[i] : 0;
+    sum += loc[j];
+  }
+  int x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  if (tid < 64) {
+    const int w = tid < 16 ? wsum[tid] : 0;
+    int z = w;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const int y = __shfl_up(z, o, 64);
+      if (lane >= o) z += y;
+    }
+    if (tid < 16) wsum[tid] = z - w;
+  }
+  __syncthreads();
+  int run = wsum[wv] + x - sum;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i = tid * per + j;
+    if (j < per && i < m) out[i] = run;
+    run += loc[j];
+  }
+}
+
+// The scatter with its writes grouped: each round of RS slots is ranked per
+// tile in LDS, staged in tile order and written out so that the round's
+// entries of one tile go to consecutive addresses.  LDS holds the block's
+// tile window (at most MAXT tiles, which the host checks) and RS entries.
+template <int MAXT, int RS>
+__global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
+  constexpr int PER = RS / 1024;
+  __shared__ int cur[MAXT];  // next entry of each tile of the window
+  __shared__ int cnt[MAXT];  // the round's entries per tile
+  __shared__ int off[MAXT];  // their exclusive offsets in the stage
+  __shared__ int4 stage[RS];
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x;
+  int g0, width, ns;
+  part_block(a, g0, width, ns);
+  if (ns == 0) return;  // uniform over the block
+  int wlo, wn;
+  part_window(a, g0, width, wlo, wn);
+  for (int t = tid; t < wn; t += 1024) cur[t] = a.hist[(size_t)(wlo + t) * a.nblocks + blockIdx.x];
+  for (int r0 = 0; r0 < ns; r0 += RS) {
+    for (int t = tid; t < wn; t += 1024) cnt[t] = 0;
     __syncthreads();
-    int k = t * kPer;
-    const int kend = k + kPer < len ? k + kPer : len;
-    if (k < kend) {
-      const int r = base + k;
-      // bucket j holding record r: the last j with off[j] <= r
-      int lo = 0, hi = m - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= r) lo = mid; else hi = mid - 1;
+    int4 ev[PER];
+    int tl[PER], rk[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int x = r0 + u * 1024 + tid;
+      tl[u] = -1;
+      ev[u] = make_int4(0, 0, 0, 0);
+      const int k = x < ns ? part_slot(a, g0, width, x) : -1;
+      if (k >= 0) {
+        const int c = a.cel[k];
+        const int4 sl = a.slog[k];
+        ev[u] = make_int4(k, c, sl.z, sl.w);
+        tl[u] = (c >> a.tb) - wlo;
       }
-      int j = lo;
-      long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-      for (; k < kend; ++k) {
-        while (base + k >= off[j + 1]) {
-          atomicAdd(&acc[0][j], (unsigned long long)s0);
-          atomicAdd(&acc[1][j], (unsigned long long)s1);
-          atomicAdd(&acc[2][j], (unsigned long long)s2);
-          atomicAdd(&acc[3][j], (unsigned long long)s3);
-          s0 = s1 = s2 = s3 = 0;
-          ++j;
-        }
-        const int4 v = buf[k];
-        s0 += v.x;
-        s1 += v.y;
-        s2 += v.z;
-        s3 += v.w;
-      }
-      atomicAdd(&acc[0][j], (unsigned long long)s0);
-      atomicAdd(&acc[1][j], (unsigned long long)s1);
-      atomicAdd(&acc[2][j], (unsigned long long)s2);
-      atomicAdd(&acc[3][j], (unsigned long long)s3);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) rk[u] = tl[u] >= 0 ? atomicAdd(&cnt[tl[u]], 1) : 0;
+    __syncthreads();
+    block_scan_runs(cnt, off, wn, wsum);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (tl[u] >= 0) stage[off[tl[u]] + rk[u]] = ev[u];
+    __syncthreads();
+    const int tot = off[wn - 1] + cnt[wn - 1];
+    for (int i = tid; i < tot; i += 1024) {
+      const int4 v = stage[i];
+      const int t = (v.y >> a.tb) - wlo;
+      a.ent[cur[t] + (i - off[t])] = v;
     }
     __syncthreads();
+    for (int t = tid; t < wn; t += 1024) cur[t] += cnt[t];
+    __syncthreads();
   }
-  if (t < m) {
-    const size_t i = c0 + t;
-    dh[i] = (int32_t)((long long)acc[0][t] + deposit[i]);
-    td[i] = (long long)acc[1][t];
-    tmx[i] = (long long)acc[2][t];
-    tmy[i] = (long long)acc[3][t];
-    count[i] = 0;
+}
+
+constexpr int kTileBits = 11;             // cells per tile = 2048
+constexpr int kTileCells = 1 << kTileBits;
+
+// Per tile: the walk steps starting at each cell (LDS histogram of the
+// tile's entries) -> burst scale; cascade sites where more steps start than
+// first steps (an arrival, water.h:151); clears the first-step counts.
+__global__ void __launch_bounds__(1024) tile_count_kernel(
+    const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent, size_t n,
+    float* scl, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
+    const float* __restrict__ root, const Terrain* __restrict__ ter, int nter) {
+  __shared__ int cnt[kTileCells];
+  for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) cnt[j] = 0;
+  __syncthreads();
+  const int base = blockIdx.x << kTileBits;
+  const int e0 = hist[(size_t)blockIdx.x * nblocks], e1 = hist[(size_t)(blockIdx.x + 1) * nblocks];
+  // four entries per thread in flight
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += 4 * blockDim.x) {
+    int c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = e + u * (int)blockDim.x;
+      c[u] = x < e1 ? ent[x].y - base : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) atomicAdd(&cnt[c[u]], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) {
+    const size_t i = (size_t)base + j;
+    if (i >= n) break;
+    const int k = cnt[j];
+    const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
+    const shy_params& P = ter[t].P;
+    float effD = P.depositionRate * (1.0f - (ter[t].has_root ? root[i] : 0.0f));
+    if (effD < 0) effD = 0;
+    scl[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
+    const int f = first[i];
+    if (k > f) visit[i] = 1;
+    if (f) first[i] = 0;
+  }
+}
+
+// quant_t of the volume a drop of terrain t carries into step k, for every
+// terrain and step: vol_0 = 1, vol_k+1 = evap_vol(vol_k) (water.h:136), the
+// same rounding as the walk.  [terrain][rows]
+__global__ void volq_kernel(const Terrain* __restrict__ ter, int nter, int rows, int32_t* vq) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nter) return;
+  const float e = ter[t].P.evapRate;
+  float vol = 1.0f;
+  for (int k = 0; k < rows; ++k) {
+    vq[(size_t)t * rows + k] = quant_t(vol);
+    vol = evap_vol(vol, e);
+  }
+}
+
+// Per tile: the height commits (gathered from the replay's log-order
+// output) and the tracks of every step starting at each cell, summed in
+// LDS in integers (so the result does not depend on the order drops ran),
+// plus the terminal deposits; clears the deposits.  The volume track of a
+// step is vq[terrain][step row].
+__global__ void __launch_bounds__(1024) tile_reduce_kernel(
+    const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent,
+    const int32_t* __restrict__ qb, const int32_t* __restrict__ vq, int rows, int drops,
+    const Terrain* __restrict__ ter, int nter, int32_t* deposit, size_t n, int32_t* dh,
+    long long* td, long long* tmx, long long* tmy) {
+  __shared__ unsigned long long acc[4][kTileCells];
+  for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) {
+    acc[0][j] = 0;
+    acc[1][j] = 0;
+    acc[2][j] = 0;
+    acc[3][j] = 0;
+  }
+  __syncthreads();
+  const int base = blockIdx.x << kTileBits;
+  // terrains of the tile's first and last cell: one terrain in nearly every tile
+  auto key = [&](int q) { return ter[q].cell_off; };
+  const int t_lo = find_terrain(nter, (long long)base, key);
+  const long long last = (long long)base + kTileCells - 1 < (long long)n ? base + kTileCells - 1
+                                                                          : (long long)n - 1;
+  const int t_hi = find_terrain(nter, last, key);
+  const int e0 = hist[(size_t)blockIdx.x * nblocks], e1 = hist[(size_t)(blockIdx.x + 1) * nblocks];
+  // four entries per thread in flight: the entry loads, then their gathers
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += 4 * blockDim.x) {
+    int4 p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = e + u * (int)blockDim.x;
+      p[u] = x < e1 ? ent[x] : make_int4(-1, 0, 0, 0);
+    }
+    int q[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      q[u] = 0;
+      v[u] = 0;
+      if (p[u].x >= 0) {
+        const int t = t_lo == t_hi ? t_lo : find_terrain(nter, (long long)p[u].y, key);
+        q[u] = qb[p[u].x];
+        v[u] = vq[(size_t)t * rows + p[u].x / drops];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (p[u].x >= 0) {
+        const int c = p[u].y - base;
+        atomicAdd(&acc[0][c], (unsigned long long)(long long)q[u]);
+        atomicAdd(&acc[1][c], (unsigned long long)(long long)v[u]);
+        atomicAdd(&acc[2][c], (unsigned long long)(long long)p[u].z);
+        atomicAdd(&acc[3][c], (unsigned long long)(long long)p[u].w);
+      }
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) {
+    const size_t i = (size_t)base + j;
+    if (i >= n) break;
+    dh[i] = (int32_t)((long long)acc[0][j] + deposit[i]);
+    td[i] = (long long)acc[1][j];
+    tmx[i] = (long long)acc[2][j];
+    tmy[i] = (long long)acc[3][j];
     deposit[i] = 0;
   }
 }
