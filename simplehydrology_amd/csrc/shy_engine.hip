@@ -18,7 +18,7 @@
 // and per wave (sized by total drops x (max maxAge + 1) log slots):
 //   cel         int32   [step][drop] start cell of each step (count walk)
 //   slog        int4    [step][drop] step log of the count walk
-//   qb          int32   [step][drop] height commit of each step (replay)
+//   qb          int32   [drop][step] height commit of each step (replay)
 //   ent         int4    the valid slots grouped by cell tile (partition)
 //   vq          int32   [terrain][step] quantised volume track of a step
 //   dlog, wsteps int4, int32 [drop] how each walk ended, its steps
@@ -62,7 +62,9 @@ struct shy_ctx {
   int32_t* deposit = nullptr;
   int32_t* cel = nullptr;    // [step][drop] start cell log of the count walk
   int4* slog = nullptr;      // [step][drop] step log of the count walk
-  int32_t* qb = nullptr;     // [step][drop] height commits of the replay
+  int32_t* qb = nullptr;     // [drop][step] height commits of the replay
+  size_t qb_cap = 0;         // ints qb holds
+  int qstride = 0;           // qb's stride per drop (log rows rounded up to 8)
   int4* ent = nullptr;       // partition entries in tile order: {slot, cell, tracks}
   int32_t* vq = nullptr;     // [terrain][row] quantised volume track per step
   size_t vq_cap = 0;
@@ -261,6 +263,10 @@ int ensure_wave(shy_ctx* c) {
   c->part_dwidth = dwidth;
   c->part_rows = rows;
   c->part_window = part_max_window(c, dwidth, nb);
+  const int qstride = (rows + 7) / 8 * 8;
+  const size_t qb_need = (size_t)c->total_drops * qstride;
+  if (qstride != c->qstride) drop_graph(c);
+  c->qstride = qstride;
   {
     // the volume-track table: rows = max maxAge + 1 over the terrains
     float age = 0.0f;
@@ -286,21 +292,29 @@ int ensure_wave(shy_ctx* c) {
       c->vq_dirty = false;
     }
   }
-  if (need <= c->wave_cap && need_d <= c->dlog_cap && hist_need <= c->hist_cap) return SHY_OK;
+  if (need <= c->wave_cap && need_d <= c->dlog_cap && hist_need <= c->hist_cap &&
+      qb_need <= c->qb_cap)
+    return SHY_OK;
   drop_graph(c);  // a captured graph holds the old buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (need > c->wave_cap) {
-    void* old[] = {c->cel, c->slog, c->qb, c->ent};
+    void* old[] = {c->cel, c->slog, c->ent};
     for (void* q : old)
       if (q) hipFree(q);
-    c->cel = c->qb = nullptr;
+    c->cel = nullptr;
     c->slog = c->ent = nullptr;
     c->wave_cap = 0;
     HIPCHK(c, hipMalloc(&c->cel, need * sizeof(int32_t)));
     HIPCHK(c, hipMalloc(&c->slog, need * sizeof(int4)));
-    HIPCHK(c, hipMalloc(&c->qb, need * sizeof(int32_t)));
     HIPCHK(c, hipMalloc(&c->ent, need * sizeof(int4)));
     c->wave_cap = need;
+  }
+  if (qb_need > c->qb_cap) {
+    if (c->qb) hipFree(c->qb);
+    c->qb = nullptr;
+    c->qb_cap = 0;
+    HIPCHK(c, hipMalloc(&c->qb, qb_need * sizeof(int32_t)));
+    c->qb_cap = qb_need;
   }
   if (need_d > c->dlog_cap) {
     if (c->dlog) hipFree(c->dlog);
@@ -336,6 +350,7 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.cel = c->cel;
   w.slog = c->slog;
   w.qb = c->qb;
+  w.qstride = c->qstride;
   w.scl = c->scl;
   w.dlog = c->dlog;
   w.steps = c->wsteps;
@@ -423,7 +438,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   tile_reduce_kernel<<<c->ntiles, 1024, 0, c->stream>>>(
-      c->hist, pa.nblocks, c->ent, c->qb, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
+      c->hist, pa.nblocks, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
       c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   HIPCHK(c, hipGetLastError());
   CascArgs a;

@@ -86,7 +86,8 @@ struct WaveArgs {
   int32_t* first;     // drops whose first step starts at the cell
   int32_t* cel;       // [step][drop] start cell of the step (index over all terrains)
   int4* slog;         // [step][drop] {qt(vol), c_eq bits, qt(vol sx), qt(vol sy)}
-  int32_t* qb;        // [step][drop] height commit of the step, 2^-20 (replay)
+  int32_t* qb;        // [drop][step] height commit of the step, 2^-20 (replay)
+  int qstride;        // qb's stride per drop: log rows rounded up to 8
   const float* __restrict__ scl;  // burst scale per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
@@ -371,8 +372,9 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
 // walk's, step for step: cdiff = c_eq - sediment, one quantised amount
 // q = quant_h(effD cdiff s) moves from the map into the drop, then
 // evaporation (water.h:128-136); the terminal deposit or the export ends the
-// drop (water.h:74-82, 139-142).  -q is written in log order (coalesced);
-// tile_reduce_kernel sums it per cell.
+// drop (water.h:74-82, 139-142).  -q is written drop-major, eight steps per
+// pair of 16-byte stores: a drop's consecutive steps often start in one cell
+// tile, so tile_reduce_kernel's gathers of them share lines.
 // Steps of the log in flight per lane: the log rows of step k+D and the
 // per-cell gathers of step k+D-1 are issued while step k computes.  One wave
 // per SIMD is all a 65536-drop wave fills, so the ring is the latency hiding.
@@ -422,16 +424,20 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
           rt[u] = root ? root[cl[u]] : 0.0f;
         }
       }
+      static_assert(D % 4 == 0, "the ring stores its commits as int4s");
+      int4* qdst = reinterpret_cast<int4*>(w.qb + (size_t)g * w.qstride);
       for (int k0 = 0; k0 < n; k0 += D) {
+        int qv[D];
 #pragma unroll
         for (int u = 0; u < D; ++u) {
           const int k = k0 + u;
+          qv[u] = 0;
           if (k < n) {
             float effD = dep * (1.0f - rt[u]);  // water.h:86-87
             if (effD < 0) effD = 0;
             const float cdiff = __int_as_float(L[u].y) - sed;
             const int32_t q = quant_h((effD * cdiff) * sc[u]);
-            w.qb[(size_t)k * stride + g] = -q;
+            qv[u] = -q;
             wdh -= q;
             sed += (float)q * kHQInv;
             const float s1 = evap_sed(sed, e);
@@ -449,6 +455,9 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
             }
           }
         }
+#pragma unroll
+        for (int u = 0; u < D; u += 4)
+          qdst[(k0 + u) >> 2] = make_int4(qv[u], qv[u + 1], qv[u + 2], qv[u + 3]);
       }
       S.steps += n;
       if (dl.y == kEndExit) {
@@ -813,14 +822,15 @@ __global__ void volq_kernel(const Terrain* __restrict__ ter, int nter, int rows,
   }
 }
 
-// Per tile: the height commits (gathered from the replay's log-order
+// Per tile: the height commits (gathered from the replay's drop-major
 // output) and the tracks of every step starting at each cell, summed in
 // LDS in integers (so the result does not depend on the order drops ran),
 // plus the terminal deposits; clears the deposits.  The volume track of a
 // step is vq[terrain][step row].
 __global__ void __launch_bounds__(1024) tile_reduce_kernel(
     const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent,
-    const int32_t* __restrict__ qb, const int32_t* __restrict__ vq, int rows, int drops,
+    const int32_t* __restrict__ qb, int qstride, const int32_t* __restrict__ vq, int rows,
+    int drops,
     const Terrain* __restrict__ ter, int nter, int32_t* deposit, size_t n, int32_t* dh,
     long long* td, long long* tmx, long long* tmy) {
   __shared__ unsigned long long acc[4][kTileCells];
@@ -854,8 +864,9 @@ __global__ void __launch_bounds__(1024) tile_reduce_kernel(
       v[u] = 0;
       if (p[u].x >= 0) {
         const int t = t_lo == t_hi ? t_lo : find_terrain(nter, (long long)p[u].y, key);
-        q[u] = qb[p[u].x];
-        v[u] = vq[(size_t)t * rows + p[u].x / drops];
+        const int row = p[u].x / drops;
+        q[u] = qb[(size_t)(p[u].x - row * drops) * qstride + row];
+        v[u] = vq[(size_t)t * rows + row];
       }
     }
 #pragma unroll
