@@ -173,7 +173,7 @@ __device__ inline void block_add_stats(long long (&v)[NC], long long* out) {
 // ------------------------------------------------------------------ walk
 // One cache line per step: {h, h(x-1), h(x+1), h(y-1)}, {h(y+1), mx, my, d},
 // {erf(0.4 d), root, 0, 0}; neighbours off the map repeat the cell's own
-// height, as load_cell's clamped reads do (the normal never uses them).
+// height, as load_unpacked's clamped reads do (the normal never uses them).
 __device__ inline CellView load_packed(const float4* __restrict__ pk, int i) {
   const float4 a = pk[4 * (size_t)i], b = pk[4 * (size_t)i + 1], c = pk[4 * (size_t)i + 2];
   CellView v;
@@ -183,8 +183,9 @@ __device__ inline CellView load_packed(const float4* __restrict__ pk, int i) {
   return v;
 }
 
-__device__ inline CellView load_cell(const WalkArgs& a, int x, int y, int i) {
-  if (a.pack) return load_packed(a.pack, i);
+// The same fields from the map arrays (the single-drop trace reads the live
+// map, which has no packed records).
+__device__ inline CellView load_unpacked(const WalkArgs& a, int x, int y, int i) {
   CellView c;
   const int ny = a.ny;
   c.h = a.h[i];
@@ -232,7 +233,22 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   float px = x0, py = y0, sx = 0.0f, sy = 0.0f, vol = 1.0f, sed = 0.0f;
   int age = 0, steps = 0;
   int nrec = 0;
-  CellView c = load_cell(a, ix, iy, i0);
+  // the log of step k is stored during step k+1, after that step's record
+  // load is issued: vector memory operations complete in issue order, so the
+  // load's wait then does not include the stores' acknowledgements.  The
+  // stores are unconditional (a conditional store makes every later wait a
+  // full one): step 0 first stores a placeholder into its own slot, which its
+  // real log overwrites one step later or, for a drop that takes no step,
+  // which no reader looks at (its step count is 0)
+  size_t pk = (size_t)lane_drop;
+  int pc = a.cell_off + i0;
+  int4 ps = make_int4(0, 0, 0, 0);
+  // the count walk reads the cycle's packed records, the trace the map arrays
+  auto load_cell = [&](int x, int y, int i) {
+    if constexpr (COUNT) return load_packed(a.pack, i);
+    else return load_unpacked(a, x, y, i);
+  };
+  CellView c = load_cell(ix, iy, i0);
   for (;;) {
     float n0, n1, n2;
     normal_from(c, ix, iy, nx, ny, n0, n1, n2);  // water.h:70
@@ -250,17 +266,22 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     const int jx = (int)px, jy = (int)py;
     const bool ex = off_map(jx, jy, nx, ny);
     const int j = jx * ny + jy;
-    CellView cn = c;
-    if (!ex) cn = load_cell(a, jx, jy, j);
+    // off the map the record is not used; loading the current cell's keeps
+    // the load unconditional, so its wait is counted exactly
+    const CellView cn = ex ? (COUNT ? load_cell(ix, iy, i0) : c) : load_cell(jx, jy, j);
+    if (COUNT) {
+      a.cel[pk] = pc;
+      a.slog[pk] = ps;
+    }
     const float h2 = ex ? (float)((double)c.h - 0.002) : cn.h;  // water.h:120-124
     float c_eq = (1.0f + P.entrainment * c.erfd) * (c.h - h2);  // water.h:127
     if (c_eq < 0) c_eq = 0;
     if (COUNT) {
       // everything of the step that the snapshot fixes; the tracks
       // (water.h:115-117) use the volume before this step's evaporation
-      a.cel[klog] = a.cell_off + i0;
-      a.slog[klog] = make_int4(quant_t(vol), __float_as_int(c_eq), quant_t(vol * sx),
-                               quant_t(vol * sy));
+      pk = klog;
+      pc = a.cell_off + i0;
+      ps = make_int4(quant_t(vol), __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
     }
     if (TRACE) {
       const float ero = effD * (c_eq - sed);  // water.h:128-132
@@ -290,6 +311,10 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     ++age;
     c = cn;
     ix = jx; iy = jy; i0 = j;
+  }
+  if (COUNT) {
+    a.cel[pk] = pc;
+    a.slog[pk] = ps;
   }
   if (TRACE) *tr_n = nrec;
   return steps;
