@@ -9,7 +9,6 @@
 //   visit[2]    uint8   cascade sites, ping-pong
 //   first       int32   drops whose first step starts at the cell; cleared by
 //                       tile_count_kernel
-//   scl         float   burst scale per cell (tile_count_kernel)
 //   dh          int32   summed walk commits per cell, 2^-20 (tile_reduce_kernel)
 //   trk[3]      int64   summed tracks per cell, 2^-24 (tile_reduce_kernel; stay
 //                       readable until the next cycle, as the reference's
@@ -25,6 +24,7 @@
 //   hist        int32   [tile][block] partition counts, then offsets
 //   pack        float4x4 per cell: the count walk's one-line record of the
 //                       cycle's start (5 heights, flow, root), by pack_kernel
+//   sr          float2  {burst scale, root density} per cell (tile_count_kernel)
 // A device table of Terrain entries maps drops, cells and cascade tiles to
 // their terrain, so every kernel covers the whole ensemble in one launch.
 // Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle
@@ -56,7 +56,7 @@ struct shy_ctx {
   float* root = nullptr;
   uint8_t* visit[2] = {nullptr, nullptr};
   int32_t* first = nullptr;
-  float* scl = nullptr;
+  float2* sr = nullptr;
   int32_t* dh = nullptr;
   long long* trk = nullptr;  // 3*n: d, mx, my
   int32_t* deposit = nullptr;
@@ -350,8 +350,8 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.cel = c->cel;
   w.slog = c->slog;
   w.qb = c->qb;
+  w.sr = c->sr;
   w.qstride = c->qstride;
-  w.scl = c->scl;
   w.dlog = c->dlog;
   w.steps = c->wsteps;
   w.pack = c->pack;
@@ -426,7 +426,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     part_scatter_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
   else
     part_scatter_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
-  tile_count_kernel<<<c->ntiles, 1024, 0, c->stream>>>(c->hist, pa.nblocks, c->ent, n, c->scl,
+  tile_count_kernel<<<c->ntiles, 1024, 0, c->stream>>>(c->hist, pa.nblocks, c->ent, n, c->sr,
                                                        c->first, c->visit[p], c->fld, c->root,
                                                        c->d_ter, nter);
   HIPCHK(c, hipGetLastError());
@@ -637,7 +637,7 @@ int shy_destroy(shy_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
-                  c->scl, c->dh, c->trk, c->deposit, c->cel, c->slog, c->qb, c->ent,
+                  c->sr, c->dh, c->trk, c->deposit, c->cel, c->slog, c->qb, c->ent,
                   c->vq, c->hist, c->hbsum, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
@@ -700,7 +700,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
-  if ((e = hipMalloc(&c->scl, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc scl");
+  if ((e = hipMalloc(&c->sr, n * sizeof(float2))) != hipSuccess) return bail(e, "hipMalloc sr");
   if ((e = hipMalloc(&c->pack, 4 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
@@ -719,7 +719,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   hipMemsetAsync(c->fld, 0, n * sizeof(float4), c->stream);
   hipMemsetAsync(c->root, 0, n * sizeof(float), c->stream);
   hipMemsetAsync(c->first, 0, n * sizeof(int32_t), c->stream);
-  hipMemsetAsync(c->scl, 0, n * sizeof(float), c->stream);
+  hipMemsetAsync(c->sr, 0, n * sizeof(float2), c->stream);
   hipMemsetAsync(c->dh, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
   hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);

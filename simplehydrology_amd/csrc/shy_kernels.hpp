@@ -88,7 +88,7 @@ struct WaveArgs {
   int4* slog;         // [step][drop] {qt(vol), c_eq bits, qt(vol sx), qt(vol sy)}
   int32_t* qb;        // [drop][step] height commit of the step, 2^-20 (replay)
   int qstride;        // qb's stride per drop: log rows rounded up to 8
-  const float* __restrict__ scl;  // burst scale per cell (tile_count_kernel)
+  const float2* __restrict__ sr;  // {burst scale, root density} per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
@@ -421,68 +421,66 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
     if (dl.y == kEndSkipped) skipped += 1;
     if (dl.y <= kEndExit) {
       walked += 1;
-      const float* __restrict__ scl = w.scl;
-      const float* __restrict__ root = T.has_root ? w.root : nullptr;
+      const float2* __restrict__ sr = w.sr;
       const float dep = T.P.depositionRate, e = T.P.evapRate;
       float sed = 0.0f, vol = 1.0f;
       double gain = 0.0;
       long long wdh = 0;
-      // ring slot u holds step k = k0 + u; rows k >= n are never read
-      int4 L[D];
-      int cl[D];
-      float sc[D], rt[D];
-#pragma unroll
-      for (int u = 0; u < D; ++u) {
-        L[u] = make_int4(0, 0, 0, 0);
-        cl[u] = 0;
-        sc[u] = 0.0f;
-        rt[u] = 0.0f;
-        if (u < n) {
-          L[u] = w.slog[(size_t)u * stride + g];
-          cl[u] = w.cel[(size_t)u * stride + g];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u + 1 < D; ++u) {
-        if (u < n) {
-          sc[u] = scl[cl[u]];
-          rt[u] = root ? root[cl[u]] : 0.0f;
-        }
-      }
-      static_assert(D % 4 == 0, "the ring stores its commits as int4s");
-      int4* qdst = reinterpret_cast<int4*>(w.qb + (size_t)g * w.qstride);
-      for (int k0 = 0; k0 < n; k0 += D) {
-        int qv[D];
+      if (n > 0) {
+        // ring slot u holds step k = k0 + u.  Every load is unconditional (the
+        // row index clamped to the drop's last step): a conditional load
+        // makes each later wait a full one, which would serialise the ring.
+        const int last = n - 1;
+        auto slot = [&](int k) { return (size_t)(k < last ? k : last) * stride + g; };
+        float cq[D], sc[D], rt[D];  // c_eq; burst scale and root density of the start cell
+        int cl[D];
 #pragma unroll
         for (int u = 0; u < D; ++u) {
-          const int k = k0 + u;
-          qv[u] = 0;
-          if (k < n) {
-            float effD = dep * (1.0f - rt[u]);  // water.h:86-87
-            if (effD < 0) effD = 0;
-            const float cdiff = __int_as_float(L[u].y) - sed;
-            const int32_t q = quant_h((effD * cdiff) * sc[u]);
-            qv[u] = -q;
-            wdh -= q;
-            sed += (float)q * kHQInv;
-            const float s1 = evap_sed(sed, e);
-            gain += (double)s1 - (double)sed;
-            sed = s1;
-            vol = evap_vol(vol, e);
-            if (k + D < n) {  // slot u now takes step k + D
-              L[u] = w.slog[(size_t)(k + D) * stride + g];
-              cl[u] = w.cel[(size_t)(k + D) * stride + g];
-            }
-            const int v = (u + D - 1) % D;  // slot of step k + D - 1 (constant once unrolled)
-            if (k + D - 1 < n) {
-              sc[v] = scl[cl[v]];
-              rt[v] = root ? root[cl[v]] : 0.0f;
-            }
-          }
+          cq[u] = __int_as_float(w.slog[slot(u)].y);
+          cl[u] = w.cel[slot(u)];
         }
 #pragma unroll
-        for (int u = 0; u < D; u += 4)
-          qdst[(k0 + u) >> 2] = make_int4(qv[u], qv[u + 1], qv[u + 2], qv[u + 3]);
+        for (int u = 0; u + 1 < D; ++u) {
+          const float2 r = sr[cl[u]];  // {burst scale, root density}
+          sc[u] = r.x;
+          rt[u] = r.y;
+        }
+        rt[D - 1] = 0.0f;
+        sc[D - 1] = 0.0f;
+        static_assert(D % 4 == 0, "the ring stores its commits as int4s");
+        int4* qdst = reinterpret_cast<int4*>(w.qb + (size_t)g * w.qstride);
+        for (int k0 = 0; k0 < n; k0 += D) {
+          int qv[D];
+#pragma unroll
+          for (int u = 0; u < D; ++u) {
+            const int k = k0 + u;
+            qv[u] = 0;
+            if (k < n) {
+              float effD = dep * (1.0f - rt[u]);  // water.h:86-87
+              if (effD < 0) effD = 0;
+              const float cdiff = cq[u] - sed;
+              const int32_t q = quant_h((effD * cdiff) * sc[u]);
+              qv[u] = -q;
+              wdh -= q;
+              sed += (float)q * kHQInv;
+              const float s1 = evap_sed(sed, e);
+              gain += (double)s1 - (double)sed;
+              sed = s1;
+              vol = evap_vol(vol, e);
+            }
+            // slot u takes step k + D; the gather of step k + D - 1 (slot v,
+            // whose row arrived one step ago) goes out now
+            cq[u] = __int_as_float(w.slog[slot(k + D)].y);
+            cl[u] = w.cel[slot(k + D)];
+            const int v = (u + D - 1) % D;  // constant once unrolled
+            const float2 r = sr[cl[v]];
+            sc[v] = r.x;
+            rt[v] = r.y;
+          }
+#pragma unroll
+          for (int u = 0; u < D; u += 4)
+            qdst[(k0 + u) >> 2] = make_int4(qv[u], qv[u + 1], qv[u + 2], qv[u + 3]);
+        }
       }
       S.steps += n;
       if (dl.y == kEndExit) {
@@ -794,11 +792,12 @@ constexpr int kTileBits = 11;             // cells per tile = 2048
 constexpr int kTileCells = 1 << kTileBits;
 
 // Per tile: the walk steps starting at each cell (LDS histogram of the
-// tile's entries) -> burst scale; cascade sites where more steps start than
+// tile's entries) -> burst scale, stored with the root density the walk
+// used as one 8-byte gather for the replay; cascade sites where more steps start than
 // first steps (an arrival, water.h:151); clears the first-step counts.
 __global__ void __launch_bounds__(1024) tile_count_kernel(
     const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent, size_t n,
-    float* scl, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
+    float2* sr, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
     const float* __restrict__ root, const Terrain* __restrict__ ter, int nter) {
   __shared__ int cnt[kTileCells];
   for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) cnt[j] = 0;
@@ -824,9 +823,10 @@ __global__ void __launch_bounds__(1024) tile_count_kernel(
     const int k = cnt[j];
     const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
     const shy_params& P = ter[t].P;
-    float effD = P.depositionRate * (1.0f - (ter[t].has_root ? root[i] : 0.0f));
+    const float rd = ter[t].has_root ? root[i] : 0.0f;
+    float effD = P.depositionRate * (1.0f - rd);
     if (effD < 0) effD = 0;
-    scl[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
+    sr[i] = make_float2(burst_scale(k, effD, P.entrainment, fld[i].w), rd);
     const int f = first[i];
     if (k > f) visit[i] = 1;
     if (f) first[i] = 0;
