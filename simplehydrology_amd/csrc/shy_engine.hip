@@ -411,7 +411,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   pa.tb = kTileBits;
   const size_t nh = (size_t)c->ntiles * c->part_blocks;
   const int nbh = (int)((nh + kScanBlock - 1) / kScanBlock);
-  if (c->ntiles <= 4096)
+  if (c->part_window <= 4096)
     part_hist_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
   else
     part_hist_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
@@ -419,7 +419,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
   scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);
   if (c->part_window <= 1024)
-    part_scatter_staged_kernel<1024, 8192><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+    part_scatter_staged_kernel<1024, 4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
   else if (c->part_window <= 4096)
     part_scatter_staged_kernel<4096, 4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
   else if (c->ntiles <= 4096)

@@ -632,13 +632,28 @@ __device__ inline int part_slot(const PartArgs& a, int g0, int width, int s) {
   return row < a.steps[g] ? row * a.drops + g : -1;
 }
 
+// the block's window of tiles: those of the terrains of its first and last drop
+__device__ inline void part_window(const PartArgs& a, int g0, int width, int& lo, int& n) {
+  auto dkey = [&](int q) { return (long long)a.ter[q].drop_off; };
+  const int t0 = find_terrain(a.nter, (long long)g0, dkey);
+  const int t1 = find_terrain(a.nter, (long long)(g0 + (width > 0 ? width : 1) - 1), dkey);
+  const long long c0 = a.ter[t0].cell_off;
+  const long long c1 = a.ter[t1].cell_off + (long long)a.ter[t1].nx * a.ter[t1].ny - 1;
+  lo = (int)(c0 >> a.tb);
+  n = (int)(c1 >> a.tb) - lo + 1;
+}
+
+// MAXT bounds the block's tile window (the host checks); the column is
+// written whole, zeros outside the window.
 template <int MAXT>
 __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
   __shared__ int h[MAXT];
-  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x) h[t] = 0;
-  __syncthreads();
   int g0, width, ns;
   part_block(a, g0, width, ns);
+  int wlo, wn;
+  part_window(a, g0, width, wlo, wn);
+  for (int t = threadIdx.x; t < wn; t += blockDim.x) h[t] = 0;
+  __syncthreads();
   for (int s = (int)threadIdx.x; s < ns; s += 4 * blockDim.x) {
     int c[4];
 #pragma unroll
@@ -649,11 +664,13 @@ __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (c[u] >= 0) atomicAdd(&h[c[u] >> a.tb], 1);
+      if (c[u] >= 0) atomicAdd(&h[(c[u] >> a.tb) - wlo], 1);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x)
-    a.hist[(size_t)t * a.nblocks + blockIdx.x] = h[t];
+  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x) {
+    const int r = t - wlo;
+    a.hist[(size_t)t * a.nblocks + blockIdx.x] = r >= 0 && r < wn ? h[r] : 0;
+  }
 }
 
 // Scatter without staging (any tile window): one 16-byte entry per step at
@@ -675,17 +692,6 @@ __global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
       a.ent[e] = make_int4(k, c, sl.z, sl.w);
     }
   }
-}
-
-// the block's window of tiles: those of the terrains of its first and last drop
-__device__ inline void part_window(const PartArgs& a, int g0, int width, int& lo, int& n) {
-  auto dkey = [&](int q) { return (long long)a.ter[q].drop_off; };
-  const int t0 = find_terrain(a.nter, (long long)g0, dkey);
-  const int t1 = find_terrain(a.nter, (long long)(g0 + (width > 0 ? width : 1) - 1), dkey);
-  const long long c0 = a.ter[t0].cell_off;
-  const long long c1 = a.ter[t1].cell_off + (long long)a.ter[t1].nx * a.ter[t1].ny - 1;
-  lo = (int)(c0 >> a.tb);
-  n = (int)(c1 >> a.tb) - lo + 1;
 }
 
 // Exclusive scan of v[0, m) into out[0, m) by a block of 1024 threads,
