@@ -15,8 +15,8 @@
 //                       *_track fields do)
 //   deposit     int32   terminal deposits, 2^-20; cleared by tile_reduce_kernel
 // and per wave (sized by total drops x (max maxAge + 1) log slots):
-//   cel         int32   [step][drop] start cell of each step (count walk)
-//   slog        int4    [step][drop] step log of the count walk
+//   slog        int4    [step][drop] step log of the count walk: start cell,
+//                       c_eq and the momentum tracks of each step
 //   qb          int32   [drop][step] height commit of each step (replay)
 //   ent         int4    the valid slots grouped by cell tile (partition)
 //   vq          int32   [terrain][step] quantised volume track of a step
@@ -60,7 +60,6 @@ struct shy_ctx {
   int32_t* dh = nullptr;
   long long* trk = nullptr;  // 3*n: d, mx, my
   int32_t* deposit = nullptr;
-  int32_t* cel = nullptr;    // [step][drop] start cell log of the count walk
   int4* slog = nullptr;      // [step][drop] step log of the count walk
   int32_t* qb = nullptr;     // [drop][step] height commits of the replay
   size_t qb_cap = 0;         // ints qb holds
@@ -298,13 +297,11 @@ int ensure_wave(shy_ctx* c) {
   drop_graph(c);  // a captured graph holds the old buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (need > c->wave_cap) {
-    void* old[] = {c->cel, c->slog, c->ent};
+    void* old[] = {c->slog, c->ent};
     for (void* q : old)
       if (q) hipFree(q);
-    c->cel = nullptr;
     c->slog = c->ent = nullptr;
     c->wave_cap = 0;
-    HIPCHK(c, hipMalloc(&c->cel, need * sizeof(int32_t)));
     HIPCHK(c, hipMalloc(&c->slog, need * sizeof(int4)));
     HIPCHK(c, hipMalloc(&c->ent, need * sizeof(int4)));
     c->wave_cap = need;
@@ -347,7 +344,6 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.fld = c->fld;
   w.root = c->root;
   w.first = c->first;
-  w.cel = c->cel;
   w.slog = c->slog;
   w.qb = c->qb;
   w.sr = c->sr;
@@ -396,7 +392,6 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   // partition of the valid log slots by cell tile (empty without drops)
   PartArgs pa;
-  pa.cel = c->cel;
   pa.slog = c->slog;
   pa.steps = c->wsteps;
   pa.ter = c->d_ter;
@@ -637,7 +632,7 @@ int shy_destroy(shy_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
-                  c->sr, c->dh, c->trk, c->deposit, c->cel, c->slog, c->qb, c->ent,
+                  c->sr, c->dh, c->trk, c->deposit, c->slog, c->qb, c->ent,
                   c->vq, c->hist, c->hbsum, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};

@@ -84,8 +84,7 @@ struct WaveArgs {
   const float4* __restrict__ fld;
   const float* __restrict__ root;
   int32_t* first;     // drops whose first step starts at the cell
-  int32_t* cel;       // [step][drop] start cell of the step (index over all terrains)
-  int4* slog;         // [step][drop] {qt(vol), c_eq bits, qt(vol sx), qt(vol sy)}
+  int4* slog;         // [step][drop] {start cell over all terrains, c_eq bits, qt(vol sx), qt(vol sy)}
   int32_t* qb;        // [drop][step] height commit of the step, 2^-20 (replay)
   int qstride;        // qb's stride per drop: log rows rounded up to 8
   const float2* __restrict__ sr;  // {burst scale, root density} per cell (tile_count_kernel)
@@ -117,7 +116,6 @@ struct WalkArgs {
   const float* __restrict__ h;
   const float4* __restrict__ fld;  // {momentumx, momentumy, discharge, erf(0.4*discharge)}
   const float* __restrict__ root;  // nullptr when the map has no root density
-  int32_t* cel;                    // [step][drop] start cell log (count pass)
   int4* slog;                      // [step][drop] step log (count pass)
   int cell_off;                    // the terrain's first cell over all terrains
   const float4* __restrict__ pack; // packed cells, or nullptr (read h, fld, root)
@@ -241,8 +239,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   // real log overwrites one step later or, for a drop that takes no step,
   // which no reader looks at (its step count is 0)
   size_t pk = (size_t)lane_drop;
-  int pc = a.cell_off + i0;
-  int4 ps = make_int4(0, 0, 0, 0);
+  int4 ps = make_int4(a.cell_off + i0, 0, 0, 0);
   // the count walk reads the cycle's packed records, the trace the map arrays
   auto load_cell = [&](int x, int y, int i) {
     if constexpr (COUNT) return load_packed(a.pack, i);
@@ -269,10 +266,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     // off the map the record is not used; loading the current cell's keeps
     // the load unconditional, so its wait is counted exactly
     const CellView cn = ex ? (COUNT ? load_cell(ix, iy, i0) : c) : load_cell(jx, jy, j);
-    if (COUNT) {
-      a.cel[pk] = pc;
-      a.slog[pk] = ps;
-    }
+    if (COUNT) a.slog[pk] = ps;
     const float h2 = ex ? (float)((double)c.h - 0.002) : cn.h;  // water.h:120-124
     float c_eq = (1.0f + P.entrainment * c.erfd) * (c.h - h2);  // water.h:127
     if (c_eq < 0) c_eq = 0;
@@ -280,8 +274,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       // everything of the step that the snapshot fixes; the tracks
       // (water.h:115-117) use the volume before this step's evaporation
       pk = klog;
-      pc = a.cell_off + i0;
-      ps = make_int4(quant_t(vol), __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
+      ps = make_int4(a.cell_off + i0, __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
     }
     if (TRACE) {
       const float ero = effD * (c_eq - sed);  // water.h:128-132
@@ -312,10 +305,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     c = cn;
     ix = jx; iy = jy; i0 = j;
   }
-  if (COUNT) {
-    a.cel[pk] = pc;
-    a.slog[pk] = ps;
-  }
+  if (COUNT) a.slog[pk] = ps;
   if (TRACE) *tr_n = nrec;
   return steps;
 }
@@ -360,7 +350,6 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.h = w.h + T.cell_off;
       a.fld = w.fld + T.cell_off;
       a.root = T.has_root ? w.root + T.cell_off : nullptr;
-      a.cel = w.cel;
       a.cell_off = (int)T.cell_off;
       a.slog = w.slog;
       a.pack = w.pack ? w.pack + 4 * T.cell_off : nullptr;
@@ -436,8 +425,9 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
         int cl[D];
 #pragma unroll
         for (int u = 0; u < D; ++u) {
-          cq[u] = __int_as_float(w.slog[slot(u)].y);
-          cl[u] = w.cel[slot(u)];
+          const int4 l = w.slog[slot(u)];
+          cl[u] = l.x;
+          cq[u] = __int_as_float(l.y);
         }
 #pragma unroll
         for (int u = 0; u + 1 < D; ++u) {
@@ -470,8 +460,9 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
             }
             // slot u takes step k + D; the gather of step k + D - 1 (slot v,
             // whose row arrived one step ago) goes out now
-            cq[u] = __int_as_float(w.slog[slot(k + D)].y);
-            cl[u] = w.cel[slot(k + D)];
+            const int4 l = w.slog[slot(k + D)];
+            cl[u] = l.x;
+            cq[u] = __int_as_float(l.y);
             const int v = (u + D - 1) % D;  // constant once unrolled
             const float2 r = sr[cl[v]];
             sc[v] = r.x;
@@ -603,8 +594,7 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(int32_t* v,
 // consumer sums integers.  A block's drops belong to a few consecutive
 // terrains, so its entries fall in a short window of tiles.
 struct PartArgs {
-  const int32_t* __restrict__ cel;    // [slot] start cell
-  const int4* __restrict__ slog;      // [slot] {qt(vol), c_eq, qt(vol sx), qt(vol sy)}
+  const int4* __restrict__ slog;      // [slot] {start cell, c_eq, qt(vol sx), qt(vol sy)}
   const int32_t* __restrict__ steps;  // [drop] valid rows of the log
   const Terrain* __restrict__ ter;
   int nter;
@@ -660,7 +650,7 @@ __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
     for (int u = 0; u < 4; ++u) {
       const int x = s + u * (int)blockDim.x;
       const int k = x < ns ? part_slot(a, g0, width, x) : -1;
-      c[u] = k >= 0 ? a.cel[k] : -1;
+      c[u] = k >= 0 ? a.slog[k].x : -1;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -686,8 +676,8 @@ __global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
   for (int s = (int)threadIdx.x; s < ns; s += blockDim.x) {
     const int k = part_slot(a, g0, width, s);
     if (k >= 0) {
-      const int c = a.cel[k];
       const int4 sl = a.slog[k];
+      const int c = sl.x;
       const int e = atomicAdd(&cur[c >> a.tb], 1);
       a.ent[e] = make_int4(k, c, sl.z, sl.w);
     }
@@ -767,8 +757,8 @@ __global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
       ev[u] = make_int4(0, 0, 0, 0);
       const int k = x < ns ? part_slot(a, g0, width, x) : -1;
       if (k >= 0) {
-        const int c = a.cel[k];
         const int4 sl = a.slog[k];
+        const int c = sl.x;
         ev[u] = make_int4(k, c, sl.z, sl.w);
         tl[u] = (c >> a.tb) - wlo;
       }
