@@ -78,6 +78,9 @@ struct shy_ctx {
   int part_rows = 0;         // log rows
   int part_window = 0;       // largest tile window of a partition block
   int ntiles = 0;            // cell tiles of kTileCells cells
+  int32_t* tcount = nullptr;    // per-part tile counts (kTileSplit parts per tile)
+  long long* tsum = nullptr;    // per-part tile sums
+  int tile_split = 1;           // kTileSplit when the part buffers are small, else 1
   int4* dlog = nullptr;  // [drop] how each walk ended
   int32_t* wsteps = nullptr;  // [drop] steps of each walk
   float4* pack = nullptr;  // per cell, 4 x float4: the count walk's record
@@ -218,6 +221,8 @@ size_t wave_steps(const shy_ctx* c) {
 // Partition blocks: at most kPartMaxBlocks, each a whole number of waves of
 // drops over all log rows (the [tile][block] matrix stays small).
 constexpr int kPartMaxBlocks = 512;
+constexpr int kTileSplit = 4;             // blocks per tile in the tile kernels
+constexpr int kTileSplitMaxTiles = 2048;
 
 // Largest tile window (tiles of the terrains of its first and last drop) of
 // any partition block.
@@ -289,6 +294,26 @@ int ensure_wave(shy_ctx* c) {
       volq_kernel<<<(nter + 63) / 64, 64, 0, c->stream>>>(c->d_ter, nter, rows, c->vq);
       HIPCHK(c, hipGetLastError());
       c->vq_dirty = false;
+    }
+  }
+  {
+    // split each tile's entries over kTileSplit blocks while the per-part
+    // buffers stay modest (up to 2048 tiles: 512 MiB of partial sums)
+    const int split = c->ntiles <= kTileSplitMaxTiles ? kTileSplit : 1;
+    if (split != c->tile_split || (split > 1 && !c->tcount)) {
+      drop_graph(c);
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (c->tcount) hipFree(c->tcount);
+      if (c->tsum) hipFree(c->tsum);
+      c->tcount = nullptr;
+      c->tsum = nullptr;
+      c->tile_split = 1;
+      if (split > 1) {
+        const size_t parts = (size_t)c->ntiles * split * kTileCells;
+        HIPCHK(c, hipMalloc(&c->tcount, parts * sizeof(int32_t)));
+        HIPCHK(c, hipMalloc(&c->tsum, 4 * parts * sizeof(long long)));
+        c->tile_split = split;
+      }
     }
   }
   if (need <= c->wave_cap && need_d <= c->dlog_cap && hist_need <= c->hist_cap &&
@@ -421,9 +446,18 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     part_scatter_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
   else
     part_scatter_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
-  tile_count_kernel<<<c->ntiles, 1024, 0, c->stream>>>(c->hist, pa.nblocks, c->ent, n, c->sr,
-                                                       c->first, c->visit[p], c->fld, c->root,
-                                                       c->d_ter, nter);
+  if (c->tile_split == kTileSplit) {
+    tile_count_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
+        c->hist, pa.nblocks, c->ent, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter,
+        nter, c->tcount);
+    tile_count_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+        c->tcount, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter);
+  } else {
+    tile_count_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(c->hist, pa.nblocks, c->ent, n,
+                                                            c->sr, c->first, c->visit[p],
+                                                            c->fld, c->root, c->d_ter, nter,
+                                                            nullptr);
+  }
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (walk) {
@@ -432,9 +466,17 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
-  tile_reduce_kernel<<<c->ntiles, 1024, 0, c->stream>>>(
-      c->hist, pa.nblocks, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
-      c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
+  if (c->tile_split == kTileSplit) {
+    tile_reduce_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
+        c->hist, pa.nblocks, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter,
+        nter, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, c->tsum);
+    tile_reduce_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+        c->tsum, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
+  } else {
+    tile_reduce_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(
+        c->hist, pa.nblocks, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter,
+        nter, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, nullptr);
+  }
   HIPCHK(c, hipGetLastError());
   CascArgs a;
   const int q = p ^ 1;
@@ -633,7 +675,7 @@ int shy_destroy(shy_ctx* c) {
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
                   c->sr, c->dh, c->trk, c->deposit, c->slog, c->qb, c->ent,
-                  c->vq, c->hist, c->hbsum, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
+                  c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)

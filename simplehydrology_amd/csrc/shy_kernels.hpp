@@ -789,17 +789,56 @@ constexpr int kTileCells = 1 << kTileBits;
 
 // Per tile: the walk steps starting at each cell (LDS histogram of the
 // tile's entries) -> burst scale, stored with the root density the walk
-// used as one 8-byte gather for the replay; cascade sites where more steps start than
-// first steps (an arrival, water.h:151); clears the first-step counts.
+// used as one 8-byte gather for the replay; cascade sites where more steps
+// start than first steps (an arrival, water.h:151); clears the first-step
+// counts.  With S > 1 each tile's entries are split over S blocks (block
+// t*S + s takes the s-th part), which write their counts to part[block] and
+// tile_count_finish_kernel adds them: the largest tiles hold several times
+// the mean's entries, and one block per tile would wait for them.
+
+// entry range of part s of tile t
+template <int S>
+__device__ inline void tile_part(const int32_t* __restrict__ hist, int nblocks, int t, int s,
+                                 int& a, int& b) {
+  const int e0 = hist[(size_t)t * nblocks], e1 = hist[(size_t)(t + 1) * nblocks];
+  if (S == 1) {
+    a = e0;
+    b = e1;
+  } else {
+    const long long len = e1 - e0;
+    a = e0 + (int)(len * s / S);
+    b = e0 + (int)(len * (s + 1) / S);
+  }
+}
+
+// burst scale, root density and cascade site of cell i, k steps starting there
+__device__ inline void cell_count_finish(size_t i, int k, float2* sr, int32_t* first,
+                                         uint8_t* visit, const float4* __restrict__ fld,
+                                         const float* __restrict__ root,
+                                         const Terrain* __restrict__ ter, int nter) {
+  const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
+  const shy_params& P = ter[t].P;
+  const float rd = ter[t].has_root ? root[i] : 0.0f;
+  float effD = P.depositionRate * (1.0f - rd);
+  if (effD < 0) effD = 0;
+  sr[i] = make_float2(burst_scale(k, effD, P.entrainment, fld[i].w), rd);
+  const int f = first[i];
+  if (k > f) visit[i] = 1;
+  if (f) first[i] = 0;
+}
+
+template <int S>
 __global__ void __launch_bounds__(1024) tile_count_kernel(
     const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent, size_t n,
     float2* sr, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
-    const float* __restrict__ root, const Terrain* __restrict__ ter, int nter) {
+    const float* __restrict__ root, const Terrain* __restrict__ ter, int nter, int32_t* part) {
   __shared__ int cnt[kTileCells];
   for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) cnt[j] = 0;
   __syncthreads();
-  const int base = blockIdx.x << kTileBits;
-  const int e0 = hist[(size_t)blockIdx.x * nblocks], e1 = hist[(size_t)(blockIdx.x + 1) * nblocks];
+  const int t = blockIdx.x / S, s = blockIdx.x % S;
+  const int base = t << kTileBits;
+  int e0, e1;
+  tile_part<S>(hist, nblocks, t, s, e0, e1);
   // four entries per thread in flight
   for (int e = e0 + (int)threadIdx.x; e < e1; e += 4 * blockDim.x) {
     int c[4];
@@ -814,19 +853,30 @@ __global__ void __launch_bounds__(1024) tile_count_kernel(
   }
   __syncthreads();
   for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) {
-    const size_t i = (size_t)base + j;
-    if (i >= n) break;
-    const int k = cnt[j];
-    const int t = find_terrain(nter, (long long)i, [&](int q) { return ter[q].cell_off; });
-    const shy_params& P = ter[t].P;
-    const float rd = ter[t].has_root ? root[i] : 0.0f;
-    float effD = P.depositionRate * (1.0f - rd);
-    if (effD < 0) effD = 0;
-    sr[i] = make_float2(burst_scale(k, effD, P.entrainment, fld[i].w), rd);
-    const int f = first[i];
-    if (k > f) visit[i] = 1;
-    if (f) first[i] = 0;
+    if (S > 1) {
+      part[(size_t)blockIdx.x * kTileCells + j] = cnt[j];
+    } else {
+      const size_t i = (size_t)base + j;
+      if (i >= n) break;
+      cell_count_finish(i, cnt[j], sr, first, visit, fld, root, ter, nter);
+    }
   }
+}
+
+// the S parts' counts of every cell (fixed order), then as tile_count_kernel
+template <int S>
+__global__ void tile_count_finish_kernel(const int32_t* __restrict__ part, size_t n, float2* sr,
+                                         int32_t* first, uint8_t* visit,
+                                         const float4* __restrict__ fld,
+                                         const float* __restrict__ root,
+                                         const Terrain* __restrict__ ter, int nter) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t t = i >> kTileBits, j = i & (kTileCells - 1);
+  int k = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) k += part[(t * S + s) * kTileCells + j];
+  cell_count_finish(i, k, sr, first, visit, fld, root, ter, nter);
 }
 
 // quant_t of the volume a drop of terrain t carries into step k, for every
@@ -847,13 +897,14 @@ __global__ void volq_kernel(const Terrain* __restrict__ ter, int nter, int rows,
 // output) and the tracks of every step starting at each cell, summed in
 // LDS in integers (so the result does not depend on the order drops ran),
 // plus the terminal deposits; clears the deposits.  The volume track of a
-// step is vq[terrain][step row].
+// step is vq[terrain][step row].  With S > 1 the tile's entries are split as
+// in tile_count_kernel and tile_reduce_finish_kernel adds the parts.
+template <int S>
 __global__ void __launch_bounds__(1024) tile_reduce_kernel(
     const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent,
     const int32_t* __restrict__ qb, int qstride, const int32_t* __restrict__ vq, int rows,
-    int drops,
-    const Terrain* __restrict__ ter, int nter, int32_t* deposit, size_t n, int32_t* dh,
-    long long* td, long long* tmx, long long* tmy) {
+    int drops, const Terrain* __restrict__ ter, int nter, int32_t* deposit, size_t n,
+    int32_t* dh, long long* td, long long* tmx, long long* tmy, long long* part) {
   __shared__ unsigned long long acc[4][kTileCells];
   for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) {
     acc[0][j] = 0;
@@ -862,14 +913,16 @@ __global__ void __launch_bounds__(1024) tile_reduce_kernel(
     acc[3][j] = 0;
   }
   __syncthreads();
-  const int base = blockIdx.x << kTileBits;
+  const int tile = blockIdx.x / S, s = blockIdx.x % S;
+  const int base = tile << kTileBits;
   // terrains of the tile's first and last cell: one terrain in nearly every tile
   auto key = [&](int q) { return ter[q].cell_off; };
   const int t_lo = find_terrain(nter, (long long)base, key);
   const long long last = (long long)base + kTileCells - 1 < (long long)n ? base + kTileCells - 1
                                                                           : (long long)n - 1;
   const int t_hi = find_terrain(nter, last, key);
-  const int e0 = hist[(size_t)blockIdx.x * nblocks], e1 = hist[(size_t)(blockIdx.x + 1) * nblocks];
+  int e0, e1;
+  tile_part<S>(hist, nblocks, tile, s, e0, e1);
   // four entries per thread in flight: the entry loads, then their gathers
   for (int e = e0 + (int)threadIdx.x; e < e1; e += 4 * blockDim.x) {
     int4 p[4];
@@ -903,14 +956,43 @@ __global__ void __launch_bounds__(1024) tile_reduce_kernel(
   }
   __syncthreads();
   for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) {
-    const size_t i = (size_t)base + j;
-    if (i >= n) break;
-    dh[i] = (int32_t)((long long)acc[0][j] + deposit[i]);
-    td[i] = (long long)acc[1][j];
-    tmx[i] = (long long)acc[2][j];
-    tmy[i] = (long long)acc[3][j];
-    deposit[i] = 0;
+    if (S > 1) {
+      long long* o = part + (size_t)blockIdx.x * 4 * kTileCells + j;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) o[(size_t)f * kTileCells] = (long long)acc[f][j];
+    } else {
+      const size_t i = (size_t)base + j;
+      if (i >= n) break;
+      dh[i] = (int32_t)((long long)acc[0][j] + deposit[i]);
+      td[i] = (long long)acc[1][j];
+      tmx[i] = (long long)acc[2][j];
+      tmy[i] = (long long)acc[3][j];
+      deposit[i] = 0;
+    }
   }
+}
+
+// the S parts' sums of every cell (fixed order), then as tile_reduce_kernel
+template <int S>
+__global__ void tile_reduce_finish_kernel(const long long* __restrict__ part, int32_t* deposit,
+                                          size_t n, int32_t* dh, long long* td, long long* tmx,
+                                          long long* tmy) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t t = i >> kTileBits, j = i & (kTileCells - 1);
+  long long a[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const long long* o = part + (t * S + s) * 4 * kTileCells + j;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+      a[f] = (long long)((unsigned long long)a[f] + (unsigned long long)o[(size_t)f * kTileCells]);
+  }
+  dh[i] = (int32_t)(a[0] + deposit[i]);
+  td[i] = a[1];
+  tmx[i] = a[2];
+  tmy[i] = a[3];
+  deposit[i] = 0;
 }
 
 __global__ void trace_kernel(WalkArgs a, float x0, float y0, shy_drop_step* out, int max_steps,
