@@ -77,6 +77,7 @@ struct shy_ctx {
   int part_dwidth = 0;       // drops per partition block
   int part_rows = 0;         // log rows
   int part_window = 0;       // largest tile window of a partition block
+  bool walk_counts = false;  // the walk builds the partition's tile counts
   int ntiles = 0;            // cell tiles of kTileCells cells
   int32_t* tcount = nullptr;    // per-part tile counts (kTileSplit parts per tile)
   long long* tsum = nullptr;    // per-part tile sums
@@ -258,8 +259,20 @@ int ensure_wave(shy_ctx* c) {
   const int drops = std::max(c->total_drops, 1);
   const int rows = (int)(need / (size_t)drops);
   const int per = (drops + kPartMaxBlocks - 1) / kPartMaxBlocks;
-  const int dwidth = std::max(kWaveDrops, (per + kWaveDrops - 1) / kWaveDrops * kWaveDrops);
+  int dwidth = std::max(kWaveDrops, (per + kWaveDrops - 1) / kWaveDrops * kWaveDrops);
+  // The walk itself counts steps per tile (an LDS histogram per block) when
+  // its blocks can be the partition blocks: one drop per thread (no
+  // persistent grid), a tile window that fits its LDS, and a [tile][block]
+  // matrix of modest size.
+  const int bs = c->tune.walk_block;
+  const int nb_walk = (drops + bs - 1) / bs;
+  const bool walk_counts = c->total_drops > 0 && c->tune.walk_grid == 0 &&
+                           (size_t)c->ntiles * nb_walk <= ((size_t)1 << 22) &&
+                           part_max_window(c, bs, nb_walk) <= kFuseMaxTiles;
+  if (walk_counts) dwidth = bs;
   const int nb = (drops + dwidth - 1) / dwidth;
+  if (walk_counts != c->walk_counts) drop_graph(c);
+  c->walk_counts = walk_counts;
   const size_t hist_need = (size_t)c->ntiles * nb + 1;
   if (nb != c->part_blocks || dwidth != c->part_dwidth || rows != c->part_rows)
     drop_graph(c);  // a captured graph holds the old geometry
@@ -375,6 +388,9 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.qstride = c->qstride;
   w.dlog = c->dlog;
   w.steps = c->wsteps;
+  w.hist = c->hist;
+  w.hist_nb = c->part_blocks;
+  w.ntiles = c->ntiles;
   w.pack = c->pack;
   w.deposit = c->deposit;
   w.visit = c->visit[p];
@@ -411,7 +427,10 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   if (walk) {
-    walk_kernel<<<grid, bs, 0, c->stream>>>(w);
+    if (c->walk_counts)
+      walk_kernel<true><<<grid, bs, 0, c->stream>>>(w);
+    else
+      walk_kernel<false><<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
@@ -431,10 +450,12 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   pa.tb = kTileBits;
   const size_t nh = (size_t)c->ntiles * c->part_blocks;
   const int nbh = (int)((nh + kScanBlock - 1) / kScanBlock);
-  if (c->part_window <= 4096)
-    part_hist_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
-  else
-    part_hist_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  if (!c->walk_counts) {
+    if (c->part_window <= 4096)
+      part_hist_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+    else
+      part_hist_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  }
   scan_sums_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, nh, c->hbsum);
   scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
   scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);

@@ -76,6 +76,12 @@ struct Terrain {
 };
 
 // Whole-ensemble arguments of the walk kernels (concatenated arrays).
+// The partition's cell tiles: 2^kTileBits consecutive cell indices over all
+// terrains.
+constexpr int kTileBits = 11;             // cells per tile = 2048
+constexpr int kTileCells = 1 << kTileBits;
+constexpr int kFuseMaxTiles = 4096;       // tile window of a walk block that counts in LDS
+
 struct WaveArgs {
   const Terrain* __restrict__ ter;
   int nter;
@@ -90,6 +96,9 @@ struct WaveArgs {
   const float2* __restrict__ sr;  // {burst scale, root density} per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
+  int32_t* hist;      // [tile][walk block] step counts, when the walk counts them
+  int hist_nb;        // walk blocks = partition blocks
+  int ntiles;
   const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
   int32_t* deposit;
   uint8_t* visit;
@@ -217,11 +226,17 @@ enum WalkMode { kTrace = 0, kCount = 1 };
 // One drop from spawn to termination: Drop(pos) then while(drop.descend())
 // (source/world.h:74-76, source/water.h:58-156) on the cycle's snapshot.
 // kTrace records per-step states of one drop (unscaled, as Drop::descend);
-// kCount writes the step log that the partition and the replay read.
+// kCount writes the step log that the partition and the replay read and
+// calls step_hook(start cell) for every step.
 // Returns the steps taken; *end and *end_cell say how and where it ended.
-template <int MODE>
+struct NoStepHook {
+  __device__ void operator()(int) const {}
+};
+
+template <int MODE, class Hook = NoStepHook>
 __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, int* end,
-                         int* end_cell, shy_drop_step* tr, int max_tr, int* tr_n) {
+                         int* end_cell, shy_drop_step* tr, int max_tr, int* tr_n,
+                         Hook step_hook = Hook()) {
   constexpr bool TRACE = MODE == kTrace;
   constexpr bool COUNT = MODE == kCount;
   const shy_params& P = a.P;
@@ -275,6 +290,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       // (water.h:115-117) use the volume before this step's evaporation
       pk = klog;
       ps = make_int4(a.cell_off + i0, __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
+      step_hook(a.cell_off + i0);
     }
     if (TRACE) {
       const float ero = effD * (c_eq - sed);  // water.h:128-132
@@ -334,7 +350,28 @@ __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
 
 // Count pass over every drop of every terrain; grid-stride, so the grid may
 // be smaller than the wave (a persistent launch).
+// With COUNT_TILES the grid is one block per partition block (one drop per
+// thread, no grid stride): the block counts its steps per tile of its
+// window in LDS and writes its column of the partition's [tile][block]
+// matrix, which part_hist_kernel would otherwise build from the log.
+template <bool COUNT_TILES>
 __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
+  __shared__ int th[COUNT_TILES ? kFuseMaxTiles : 1];
+  int wlo = 0, wn = 0;
+  if (COUNT_TILES) {
+    auto dkey = [&](int q) { return (long long)w.ter[q].drop_off; };
+    const int gb = blockIdx.x * blockDim.x;
+    const int gl = (gb + (int)blockDim.x < w.total_drops ? gb + (int)blockDim.x : w.total_drops) - 1;
+    const Terrain& t0 = w.ter[find_terrain(w.nter, (long long)gb, dkey)];
+    const Terrain& t1 = w.ter[find_terrain(w.nter, (long long)gl, dkey)];
+    wlo = (int)(t0.cell_off >> kTileBits);
+    wn = (int)((t1.cell_off + (long long)t1.nx * t1.ny - 1) >> kTileBits) - wlo + 1;
+    for (int j = threadIdx.x; j < wn; j += blockDim.x) th[j] = 0;
+    __syncthreads();
+  }
+  auto count_step = [&](int cell) {
+    if (COUNT_TILES) atomicAdd(&th[(cell >> kTileBits) - wlo], 1);
+  };
   const unsigned int cycle = w.state->cycle;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < w.total_drops;
        g += gridDim.x * blockDim.x) {
@@ -365,7 +402,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
         int end = 0, end_cell = 0;
         dl.x = walk_drop<kCount>(a, g, (float)(cid / (uint32_t)T.ny),
                                  (float)(cid % (uint32_t)T.ny), &end, &end_cell, nullptr, 0,
-                                 nullptr);
+                                 nullptr, count_step);
         dl.y = end;
         dl.z = end_cell;
         if (dl.x > 0) {
@@ -379,6 +416,13 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
     }
     w.dlog[g] = dl;
     w.steps[g] = dl.x;
+  }
+  if (COUNT_TILES) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < w.ntiles; t += blockDim.x) {
+      const int r = t - wlo;
+      w.hist[(size_t)t * w.hist_nb + blockIdx.x] = r >= 0 && r < wn ? th[r] : 0;
+    }
   }
 }
 
@@ -784,8 +828,6 @@ __global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
   }
 }
 
-constexpr int kTileBits = 11;             // cells per tile = 2048
-constexpr int kTileCells = 1 << kTileBits;
 
 // Per tile: the walk steps starting at each cell (LDS histogram of the
 // tile's entries) -> burst scale, stored with the root density the walk
