@@ -10,12 +10,12 @@ particle-steps/s over exactly --steps timed cycles, inputs resident in HBM.
 Prints one JSON line.  Roofline: the kernel with the largest measured time
 per cycle, against the HBM peak, with its algorithmic bytes per
 particle-step:
-  count walk (walk_kernel)  B_COUNT = 57: reads 5 heights and
-      {mx, my, d, erf(0.4 d)} of the step's cell = 36 B; writes the start-cell
-      log 4 B and the step log 16 B; cascade-site byte 1 B
-  commit replay (replay_kernel)  B_REPLAY = 28: reads the step log 16 B, the
-      start cell 4 B and the cell's burst scale 4 B; writes the height commit
-      4 B
+  count walk (walk_kernel)  B_COUNT = 53: reads 5 heights and
+      {mx, my, d, erf(0.4 d)} of the step's cell = 36 B; writes the 16-B step
+      record {start cell, c_eq, tracks}; cascade-site byte 1 B
+  commit replay (replay_kernel)  B_REPLAY = 20: reads the start cell and c_eq
+      of the step record 8 B and the cell's {burst scale, root density} 8 B;
+      writes the height commit 4 B
 ``traffic`` is the HBM bytes per launch of that kernel measured by rocprofv3
 PMC runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy-kernel calibration) on the
 same command, read from profiles/hbm_traffic.json when it holds an entry for
@@ -39,8 +39,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-B_COUNT = 57          # algorithmic bytes per particle-step, count walk
-B_REPLAY = 28         # algorithmic bytes per particle-step, commit replay
+B_COUNT = 53          # algorithmic bytes per particle-step, count walk
+B_REPLAY = 20         # algorithmic bytes per particle-step, commit replay
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNELS = (("walk_kernel", "ms_count_walk", B_COUNT), ("replay_kernel", "ms_replay", B_REPLAY))
 
