@@ -63,7 +63,7 @@ struct shy_ctx {
   int4* slog = nullptr;      // [step][drop] step log of the count walk
   int32_t* qb = nullptr;     // [drop][step] height commits of the replay
   size_t qb_cap = 0;         // ints qb holds
-  int qstride = 0;           // qb's stride per drop (log rows rounded up to 8)
+  int qstride = 0;           // qb's stride per drop (log rows rounded up to kReplayDepth)
   int4* ent = nullptr;       // partition entries in tile order: {slot, cell, tracks}
   int32_t* vq = nullptr;     // [terrain][row] quantised volume track per step
   size_t vq_cap = 0;
@@ -280,7 +280,7 @@ int ensure_wave(shy_ctx* c) {
   c->part_dwidth = dwidth;
   c->part_rows = rows;
   c->part_window = part_max_window(c, dwidth, nb);
-  const int qstride = (rows + 7) / 8 * 8;
+  const int qstride = (rows + kReplayDepth - 1) / kReplayDepth * kReplayDepth;
   const size_t qb_need = (size_t)c->total_drops * qstride;
   if (qstride != c->qstride) drop_graph(c);
   c->qstride = qstride;

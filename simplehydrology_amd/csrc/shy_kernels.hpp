@@ -92,7 +92,7 @@ struct WaveArgs {
   int32_t* first;     // drops whose first step starts at the cell
   int4* slog;         // [step][drop] {start cell over all terrains, c_eq bits, qt(vol sx), qt(vol sy)}
   int32_t* qb;        // [drop][step] height commit of the step, 2^-20 (replay)
-  int qstride;        // qb's stride per drop: log rows rounded up to 8
+  int qstride;        // qb's stride per drop: log rows rounded up to kReplayDepth
   const float2* __restrict__ sr;  // {burst scale, root density} per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
@@ -430,13 +430,13 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
 // walk's, step for step: cdiff = c_eq - sediment, one quantised amount
 // q = quant_h(effD cdiff s) moves from the map into the drop, then
 // evaporation (water.h:128-136); the terminal deposit or the export ends the
-// drop (water.h:74-82, 139-142).  -q is written drop-major, eight steps per
-// pair of 16-byte stores: a drop's consecutive steps often start in one cell
-// tile, so tile_reduce_kernel's gathers of them share lines.
+// drop (water.h:74-82, 139-142).  -q is written drop-major, four steps per
+// 16-byte store: a drop's consecutive steps often start in one cell tile, so
+// tile_reduce_kernel's gathers of them share lines.
 // Steps of the log in flight per lane: the log rows of step k+D and the
 // per-cell gathers of step k+D-1 are issued while step k computes.  One wave
 // per SIMD is all a 65536-drop wave fills, so the ring is the latency hiding.
-constexpr int kReplayDepth = 8;
+constexpr int kReplayDepth = 32;
 
 __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
   constexpr int D = kReplayDepth;
