@@ -25,6 +25,8 @@
 //   pack        float4x4 per cell: the count walk's one-line record of the
 //                       cycle's start (5 heights, flow, root), by pack_kernel
 //   sr          float2  {burst scale, root density} per cell (tile_count_kernel)
+//   tcount, tsum int32, int64x4  per-part tile counts and sums when a tile is
+//                       split over kTileSplit blocks
 // A device table of Terrain entries maps drops, cells and cascade tiles to
 // their terrain, so every kernel covers the whole ensemble in one launch.
 // Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle

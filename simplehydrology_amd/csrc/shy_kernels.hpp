@@ -4,9 +4,11 @@
 // drops that all read the map as it stood when the cycle began:
 //
 //  walk_kernel           one drop per lane walks its trajectory; for every
-//                        step it logs, coalesced as [step][drop], the start
-//                        cell and the step's snapshot-only quantities {c_eq,
-//                        tracks}; per drop it logs how the walk ended, counts
+//                        step it logs, coalesced as [step][drop], one 16-byte
+//                        record {start cell, c_eq, tracks} of what the
+//                        snapshot fixes (and, where its blocks are the
+//                        partition blocks, counts the step's cell tile in
+//                        LDS); per drop it logs how the walk ended, counts
 //                        its first step's cell and marks its end cell.  The
 //                        cascade sites (arrival cells, water.h:151) follow
 //                        per cell: a cell is a site iff more steps start there
@@ -31,7 +33,7 @@
 //                        not depend on the previous step, so they are issued
 //                        ahead.  Each step's erosion/deposition effD*cdiff*s is
 //                        quantised once to int32 (2^-20, moved between map and
-//                        drop as that integer) and stored in log order.
+//                        drop as that integer) and stored drop-major.
 //  tile_reduce_kernel    per tile, sums in LDS the height commits (gathered by
 //                        slot) and the discharge/momentum tracks (int32, 2^-24)
 //                        of its steps in integers (so the result does not
