@@ -21,7 +21,9 @@
 //   ent         int4    the valid slots grouped by cell tile (partition)
 //   vq          int32   [terrain][step] quantised volume track of a step
 //   dlog, wsteps int4, int32 [drop] how each walk ended, its steps
-//   hist        int32   [tile][block] partition counts, then offsets
+//   hist        int32   per terrain a [tile][block] count matrix of the
+//                       partition, then exclusive offsets over all of them
+//   tstart      int32   [tile] first partition entry of every cell tile
 //   pack        float4x4 per cell: the count walk's one-line record of the
 //                       cycle's start (5 heights, flow, root), by pack_kernel
 //   sr          float2  {burst scale, root density} per cell (tile_count_kernel)
@@ -72,13 +74,15 @@ struct shy_ctx {
   int vq_rows = 0;
   bool vq_dirty = true;      // parameters changed since vq was built
   size_t wave_cap = 0;       // log slots the wave buffers can hold
-  int32_t* hist = nullptr;   // [tile][block] partition counts, then offsets
+  int32_t* hist = nullptr;   // per-terrain [tile][block] partition counts, then offsets
   int32_t* hbsum = nullptr;  // block sums of the hist scan
   size_t hist_cap = 0;       // ints hist holds (hbsum holds hist_cap / 1024 + 1)
-  int part_blocks = 0;       // partition geometry of the current wave
-  int part_dwidth = 0;       // drops per partition block
+  int32_t* tstart = nullptr; // [tile] first partition entry, [ntiles] = all entries
+  int tstart_cap = 0;        // tiles tstart holds (plus one)
+  int part_blocks = 0;       // partition geometry of the current wave: blocks over all terrains
+  size_t part_entries = 0;   // count-matrix entries over all terrains
   int part_rows = 0;         // log rows
-  int part_window = 0;       // largest tile window of a partition block
+  int part_window = 0;       // most tiles of a terrain
   bool walk_counts = false;  // the walk builds the partition's tile counts
   int ntiles = 0;            // cell tiles of kTileCells cells
   int32_t* tcount = nullptr;    // per-part tile counts (kTileSplit parts per tile)
@@ -117,6 +121,9 @@ constexpr float kMaxAgeLimit = 1.0e6f;
 constexpr int kTileY[3] = {32, 64, 16};
 constexpr int kTileX[3] = {8, 4, 16};
 constexpr int kWaveDrops = 64;  // drop ranges are padded to whole wave64s
+// drop ranges of an ensemble are padded to 256 drops: walk blocks of 64, 128
+// or 256 threads then never span terrains
+constexpr int kDropAlign = 256;
 constexpr int kReplayBlock = 256;
 
 std::string g_noctx_err;
@@ -195,7 +202,8 @@ int layout(shy_ctx* c) {
     t.tile_off = (int)tiles;
     cells += (long long)t.nx * t.ny;
     // whole waves per terrain: the walk reads its terrain entry once per wave
-    drops += ((long long)t.drops + kWaveDrops - 1) / kWaveDrops * kWaveDrops;
+    const long long align = c->ter.size() > 1 ? kDropAlign : kWaveDrops;
+    drops += ((long long)t.drops + align - 1) / align * align;
     tiles += (long long)t.tiles_y * ((t.nx + tx - 1) / tx);
     if (drops > INT32_MAX || tiles > INT32_MAX)
       return fail(c, SHY_ERR_ARG, "ensemble too large: drops and cascade tiles must stay below 2^31");
@@ -227,28 +235,43 @@ constexpr int kPartMaxBlocks = 512;
 constexpr int kTileSplit = 4;             // blocks per tile in the tile kernels
 constexpr int kTileSplitMaxTiles = 2048;
 
-// Largest tile window (tiles of the terrains of its first and last drop) of
-// any partition block.
-int part_max_window(const shy_ctx* c, int dwidth, int nblocks) {
-  int best = 1;
-  auto terrain_of = [&](int g) {
-    int lo = 0, hi = (int)c->ter.size() - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) / 2;
-      if (c->ter[mid].drop_off <= g) lo = mid; else hi = mid - 1;
+// The partition geometry: per terrain its tiles, its blocks of dwidth drops
+// (dwidth = walk_block when the walk counts, else enough drops for at most
+// kPartMaxBlocks blocks per terrain) and the offset of its count matrix.
+// Fills the fields into t (when given); returns the count-matrix entries
+// over all terrains and sets blocks (over all terrains) and window (most
+// tiles of a terrain).
+size_t part_geometry(const shy_ctx* c, int walk_block, std::vector<Terrain>* t, int& blocks,
+                     int& window) {
+  size_t off = 0;
+  blocks = 0;
+  window = 1;
+  for (size_t k = 0; k < c->ter.size(); ++k) {
+    const Terrain& T = c->ter[k];
+    const long long end = (k + 1 < c->ter.size()) ? c->ter[k + 1].drop_off : c->total_drops;
+    const int padded = (int)(end - T.drop_off);
+    int dwidth = walk_block;
+    if (dwidth == 0) {
+      const int per = (padded + kPartMaxBlocks - 1) / kPartMaxBlocks;
+      dwidth = std::max(kWaveDrops, (per + kWaveDrops - 1) / kWaveDrops * kWaveDrops);
     }
-    return lo;
-  };
-  for (int b = 0; b < nblocks; ++b) {
-    const int g0 = b * dwidth;
-    const int g1 = std::min(g0 + dwidth, c->total_drops) - 1;
-    if (g1 < g0) continue;
-    const Terrain& t0 = c->ter[terrain_of(g0)];
-    const Terrain& t1 = c->ter[terrain_of(g1)];
-    const long long c1 = t1.cell_off + (long long)t1.nx * t1.ny - 1;
-    best = std::max(best, (int)((c1 >> kTileBits) - (t0.cell_off >> kTileBits) + 1));
+    const int nb = (padded + dwidth - 1) / dwidth;
+    const int lo = (int)(T.cell_off >> kTileBits);
+    const int tn = (int)((T.cell_off + (long long)T.nx * T.ny - 1) >> kTileBits) - lo + 1;
+    if (t) {
+      Terrain& o = (*t)[k];
+      o.part_off = (long long)off;
+      o.part_first = blocks;
+      o.part_blocks = nb;
+      o.part_dwidth = dwidth;
+      o.tile_lo = lo;
+      o.tile_n = tn;
+    }
+    off += (size_t)tn * nb;
+    blocks += nb;
+    window = std::max(window, tn);
   }
-  return best;
+  return off;
 }
 
 // Size the logs, the partition and the per-drop buffers for the current
@@ -260,28 +283,37 @@ int ensure_wave(shy_ctx* c) {
     return fail(c, SHY_ERR_ARG, "drops x (maxAge+1) exceeds 2^31 log slots per wave");
   const int drops = std::max(c->total_drops, 1);
   const int rows = (int)(need / (size_t)drops);
-  const int per = (drops + kPartMaxBlocks - 1) / kPartMaxBlocks;
-  int dwidth = std::max(kWaveDrops, (per + kWaveDrops - 1) / kWaveDrops * kWaveDrops);
   // The walk itself counts steps per tile (an LDS histogram per block) when
   // its blocks can be the partition blocks: one drop per thread (no
-  // persistent grid), a tile window that fits its LDS, and a [tile][block]
-  // matrix of modest size.
+  // persistent grid), blocks inside one terrain (one terrain, or a block
+  // size dividing the padded drop ranges), every terrain's tiles in its LDS
+  // and count matrices of modest size.
   const int bs = c->tune.walk_block;
-  const int nb_walk = (drops + bs - 1) / bs;
-  const bool walk_counts = c->total_drops > 0 && c->tune.walk_grid == 0 &&
-                           (size_t)c->ntiles * nb_walk <= ((size_t)1 << 22) &&
-                           part_max_window(c, bs, nb_walk) <= kFuseMaxTiles;
-  if (walk_counts) dwidth = bs;
-  const int nb = (drops + dwidth - 1) / dwidth;
-  if (walk_counts != c->walk_counts) drop_graph(c);
-  c->walk_counts = walk_counts;
-  const size_t hist_need = (size_t)c->ntiles * nb + 1;
-  if (nb != c->part_blocks || dwidth != c->part_dwidth || rows != c->part_rows)
+  int nb = 0, window = 0;
+  const bool fit = c->ter.size() == 1 || kDropAlign % bs == 0;
+  const bool walk_counts = c->total_drops > 0 && c->tune.walk_grid == 0 && fit &&
+                           part_geometry(c, bs, nullptr, nb, window) <= ((size_t)1 << 22) &&
+                           window <= kFuseMaxTiles;
+  std::vector<Terrain> geo = c->ter;
+  const size_t entries = part_geometry(c, walk_counts ? bs : 0, &geo, nb, window);
+  if (memcmp(geo.data(), c->ter.data(), geo.size() * sizeof(Terrain)) != 0) {
+    drop_graph(c);
+    c->ter = geo;
+    if (c->d_ter) {
+      HIPCHK(c, hipMemcpyAsync(c->d_ter, c->ter.data(), c->ter.size() * sizeof(Terrain),
+                               hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+  }
+  if (walk_counts != c->walk_counts || nb != c->part_blocks || rows != c->part_rows ||
+      entries != c->part_entries)
     drop_graph(c);  // a captured graph holds the old geometry
+  c->walk_counts = walk_counts;
   c->part_blocks = nb;
-  c->part_dwidth = dwidth;
+  c->part_entries = entries;
   c->part_rows = rows;
-  c->part_window = part_max_window(c, dwidth, nb);
+  c->part_window = window;
+  const size_t hist_need = entries + 1;
   const int qstride = (rows + kReplayDepth - 1) / kReplayDepth * kReplayDepth;
   const size_t qb_need = (size_t)c->total_drops * qstride;
   if (qstride != c->qstride) drop_graph(c);
@@ -330,6 +362,15 @@ int ensure_wave(shy_ctx* c) {
         c->tile_split = split;
       }
     }
+  }
+  if (c->ntiles > c->tstart_cap) {
+    drop_graph(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->tstart) hipFree(c->tstart);
+    c->tstart = nullptr;
+    c->tstart_cap = 0;
+    HIPCHK(c, hipMalloc(&c->tstart, ((size_t)c->ntiles + 1) * sizeof(int32_t)));
+    c->tstart_cap = c->ntiles;
   }
   if (need <= c->wave_cap && need_d <= c->dlog_cap && hist_need <= c->hist_cap &&
       qb_need <= c->qb_cap)
@@ -391,8 +432,6 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.dlog = c->dlog;
   w.steps = c->wsteps;
   w.hist = c->hist;
-  w.hist_nb = c->part_blocks;
-  w.ntiles = c->ntiles;
   w.pack = c->pack;
   w.deposit = c->deposit;
   w.visit = c->visit[p];
@@ -446,40 +485,39 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   pa.ent = c->ent;
   pa.rows = walk ? c->part_rows : 0;
   pa.drops = std::max(c->total_drops, 1);
-  pa.dwidth = c->part_dwidth;
-  pa.nblocks = c->part_blocks;
-  pa.ntiles = c->ntiles;
   pa.tb = kTileBits;
-  const size_t nh = (size_t)c->ntiles * c->part_blocks;
+  const size_t nh = c->part_entries;
   const int nbh = (int)((nh + kScanBlock - 1) / kScanBlock);
-  if (!c->walk_counts) {
-    if (c->part_window <= 4096)
-      part_hist_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  const int nb = c->part_blocks;
+  if (nh > 0) {
+    if (!c->walk_counts) {
+      if (c->part_window <= 4096)
+        part_hist_kernel<4096><<<nb, 1024, 0, c->stream>>>(pa);
+      else
+        part_hist_kernel<kPartMaxTiles><<<nb, 1024, 0, c->stream>>>(pa);
+    }
+    scan_sums_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, nh, c->hbsum);
+    scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
+    scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);
+    if (c->part_window <= 1024)
+      part_scatter_staged_kernel<1024, 4096><<<nb, 1024, 0, c->stream>>>(pa);
+    else if (c->part_window <= 4096)
+      part_scatter_staged_kernel<4096, 4096><<<nb, 1024, 0, c->stream>>>(pa);
     else
-      part_hist_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+      part_scatter_kernel<kPartMaxTiles><<<nb, 1024, 0, c->stream>>>(pa);
   }
-  scan_sums_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, nh, c->hbsum);
-  scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
-  scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);
-  if (c->part_window <= 1024)
-    part_scatter_staged_kernel<1024, 4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
-  else if (c->part_window <= 4096)
-    part_scatter_staged_kernel<4096, 4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
-  else if (c->ntiles <= 4096)
-    part_scatter_kernel<4096><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
-  else
-    part_scatter_kernel<kPartMaxTiles><<<pa.nblocks, 1024, 0, c->stream>>>(pa);
+  tile_start_kernel<<<(c->ntiles + 256) / 256, 256, 0, c->stream>>>(
+      c->d_ter, nter, c->ntiles, nh > 0 ? c->hist : nullptr, (long long)nh, c->tstart);
   if (c->tile_split == kTileSplit) {
     tile_count_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
-        c->hist, pa.nblocks, c->ent, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter,
-        nter, c->tcount);
+        c->tstart, c->ent, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter,
+        c->tcount);
     tile_count_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
         c->tcount, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter);
   } else {
-    tile_count_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(c->hist, pa.nblocks, c->ent, n,
-                                                            c->sr, c->first, c->visit[p],
-                                                            c->fld, c->root, c->d_ter, nter,
-                                                            nullptr);
+    tile_count_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(c->tstart, c->ent, n, c->sr,
+                                                            c->first, c->visit[p], c->fld,
+                                                            c->root, c->d_ter, nter, nullptr);
   }
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
@@ -491,14 +529,14 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   if (c->tile_split == kTileSplit) {
     tile_reduce_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
-        c->hist, pa.nblocks, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter,
-        nter, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, c->tsum);
+        c->tstart, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
+        c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, c->tsum);
     tile_reduce_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
         c->tsum, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   } else {
     tile_reduce_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(
-        c->hist, pa.nblocks, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter,
-        nter, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, nullptr);
+        c->tstart, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
+        c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, nullptr);
   }
   HIPCHK(c, hipGetLastError());
   CascArgs a;
@@ -698,7 +736,7 @@ int shy_destroy(shy_ctx* c) {
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
                   c->sr, c->dh, c->trk, c->deposit, c->slog, c->qb, c->ent,
-                  c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
+                  c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->tstart, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)

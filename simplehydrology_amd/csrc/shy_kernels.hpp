@@ -71,6 +71,12 @@ struct Terrain {
   int nx, ny;
   int drop_off, drops;
   int tile_off, tiles_y;
+  // the partition: the terrain's cell tiles [tile_lo, tile_lo + tile_n),
+  // its blocks [part_first, part_first + part_blocks) of part_dwidth drops,
+  // and its tile_n x part_blocks count matrix at part_off
+  long long part_off;
+  int part_first, part_blocks, part_dwidth;
+  int tile_lo, tile_n;
   unsigned long long seed;
   unsigned int id;  // terrain number in the spawn hash
   int has_root;
@@ -98,9 +104,7 @@ struct WaveArgs {
   const float2* __restrict__ sr;  // {burst scale, root density} per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
-  int32_t* hist;      // [tile][walk block] step counts, when the walk counts them
-  int hist_nb;        // walk blocks = partition blocks
-  int ntiles;
+  int32_t* hist;      // per-terrain [tile][walk block] step counts, when the walk counts them
   const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
   int32_t* deposit;
   uint8_t* visit;
@@ -354,20 +358,24 @@ __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
 // be smaller than the wave (a persistent launch).
 // With COUNT_TILES the grid is one block per partition block (one drop per
 // thread, no grid stride): the block counts its steps per tile of its
-// window in LDS and writes its column of the partition's [tile][block]
+// terrain in LDS and writes its column of the terrain's [tile][block]
 // matrix, which part_hist_kernel would otherwise build from the log.
 template <bool COUNT_TILES>
 __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
   __shared__ int th[COUNT_TILES ? kFuseMaxTiles : 1];
   int wlo = 0, wn = 0;
+  long long hbase = 0;
+  int hstride = 0;
   if (COUNT_TILES) {
-    auto dkey = [&](int q) { return (long long)w.ter[q].drop_off; };
-    const int gb = blockIdx.x * blockDim.x;
-    const int gl = (gb + (int)blockDim.x < w.total_drops ? gb + (int)blockDim.x : w.total_drops) - 1;
-    const Terrain& t0 = w.ter[find_terrain(w.nter, (long long)gb, dkey)];
-    const Terrain& t1 = w.ter[find_terrain(w.nter, (long long)gl, dkey)];
-    wlo = (int)(t0.cell_off >> kTileBits);
-    wn = (int)((t1.cell_off + (long long)t1.nx * t1.ny - 1) >> kTileBits) - wlo + 1;
+    // the block's drops lie in one terrain: the partition blocks are the walk
+    // blocks (part_dwidth = blockDim, which divides every padded drop range)
+    const Terrain& T = w.ter[find_terrain(
+        w.nter, (long long)blockIdx.x * blockDim.x,
+        [&](int q) { return (long long)w.ter[q].drop_off; })];
+    wlo = T.tile_lo;
+    wn = T.tile_n;
+    hbase = T.part_off + ((int)blockIdx.x - T.part_first);
+    hstride = T.part_blocks;
     for (int j = threadIdx.x; j < wn; j += blockDim.x) th[j] = 0;
     __syncthreads();
   }
@@ -421,10 +429,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
   }
   if (COUNT_TILES) {
     __syncthreads();
-    for (int t = threadIdx.x; t < w.ntiles; t += blockDim.x) {
-      const int r = t - wlo;
-      w.hist[(size_t)t * w.hist_nb + blockIdx.x] = r >= 0 && r < wn ? th[r] : 0;
-    }
+    for (int r = threadIdx.x; r < wn; r += blockDim.x) w.hist[hbase + (long long)r * hstride] = th[r];
   }
 }
 
@@ -631,35 +636,55 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(int32_t* v,
 // ------------------------------------------------- partition by cell tile
 // The step log is [step][drop]; the per-cell sums need it grouped by cell.
 // Cells are cut into tiles of 2^tb consecutive indices (over all terrains).
-// Each of nblocks blocks owns dwidth consecutive drops over all rows of the
-// log (slot s of a block: row s / width, drop g0 + s % width):
-// part_hist_kernel counts its valid slots per tile in LDS, the scan turns
-// the [tile][block] matrix into exclusive offsets, and the scatter places
-// every valid slot at its tile's next free entry (an LDS cursor per tile),
-// one 16-byte entry per step.  Order inside a tile does not matter: every
-// consumer sums integers.  A block's drops belong to a few consecutive
-// terrains, so its entries fall in a short window of tiles.
+// Each block owns dwidth consecutive drops over all rows of the log (slot s
+// of a block: row s / width, drop g0 + s % width), and its drops belong to
+// one terrain (dwidth divides every terrain's padded drop range), so its
+// steps fall in that terrain's tiles.  Every terrain owns a [tile][block]
+// count matrix of its tiles by its blocks, stored one after the other in
+// terrain order: part_hist_kernel (or the walk) fills the block's column in
+// LDS, the scan turns all matrices into exclusive offsets at once, and the
+// scatter places every valid slot at its tile's next free entry (an LDS
+// cursor per tile), one 16-byte entry per step.  A tile shared by two
+// terrains ends the first terrain's matrix and opens the next one's, so its
+// entries stay contiguous.  Order inside a tile does not matter: every
+// consumer sums integers.
 struct PartArgs {
   const int4* __restrict__ slog;      // [slot] {start cell, c_eq, qt(vol sx), qt(vol sy)}
   const int32_t* __restrict__ steps;  // [drop] valid rows of the log
   const Terrain* __restrict__ ter;
   int nter;
-  int32_t* hist;   // [tile][block] counts, then exclusive offsets; [ntiles*nblocks] = total
+  int32_t* hist;   // per terrain [tile][block] counts, then exclusive offsets; [total] = entries
   int4* ent;       // [entry] {slot, start cell, qt(vol sx), qt(vol sy)}, tile order
   int rows;        // log rows (0: no walk this cycle)
   int drops;       // log stride
-  int dwidth;      // drops per block
-  int nblocks;
-  int ntiles;
   int tb;          // log2 cells per tile
 };
 
-// the block's drops and its slot count
-__device__ inline void part_block(const PartArgs& a, int& g0, int& width, int& nslots) {
-  g0 = blockIdx.x * a.dwidth;
-  width = a.drops - g0 < a.dwidth ? a.drops - g0 : a.dwidth;
-  if (width < 0) width = 0;
-  nslots = width * a.rows;
+// A partition block: its drops [g0, g0 + width), its slots (width x rows),
+// its column of its terrain's count matrix (base, stride = the terrain's
+// blocks) and its terrain's tiles [wlo, wlo + wn).
+struct PartBlock {
+  int g0, width, nslots;
+  long long base;
+  int stride;
+  int wlo, wn;
+};
+
+__device__ inline PartBlock part_block(const PartArgs& a) {
+  const int b = blockIdx.x;
+  const Terrain& T = a.ter[find_terrain(a.nter, (long long)b,
+                                        [&](int q) { return (long long)a.ter[q].part_first; })];
+  const int j = b - T.part_first;
+  const int left = T.drops - j * T.part_dwidth;
+  PartBlock p;
+  p.g0 = T.drop_off + j * T.part_dwidth;
+  p.width = left < T.part_dwidth ? (left > 0 ? left : 0) : T.part_dwidth;
+  p.nslots = p.width * a.rows;
+  p.base = T.part_off + j;
+  p.stride = T.part_blocks;
+  p.wlo = T.tile_lo;
+  p.wn = T.tile_n;
+  return p;
 }
 
 // slot s of the block -> log index k, or -1 when the drop took fewer steps
@@ -668,27 +693,13 @@ __device__ inline int part_slot(const PartArgs& a, int g0, int width, int s) {
   return row < a.steps[g] ? row * a.drops + g : -1;
 }
 
-// the block's window of tiles: those of the terrains of its first and last drop
-__device__ inline void part_window(const PartArgs& a, int g0, int width, int& lo, int& n) {
-  auto dkey = [&](int q) { return (long long)a.ter[q].drop_off; };
-  const int t0 = find_terrain(a.nter, (long long)g0, dkey);
-  const int t1 = find_terrain(a.nter, (long long)(g0 + (width > 0 ? width : 1) - 1), dkey);
-  const long long c0 = a.ter[t0].cell_off;
-  const long long c1 = a.ter[t1].cell_off + (long long)a.ter[t1].nx * a.ter[t1].ny - 1;
-  lo = (int)(c0 >> a.tb);
-  n = (int)(c1 >> a.tb) - lo + 1;
-}
-
-// MAXT bounds the block's tile window (the host checks); the column is
-// written whole, zeros outside the window.
+// MAXT bounds the tiles of any terrain (the host checks).
 template <int MAXT>
 __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
   __shared__ int h[MAXT];
-  int g0, width, ns;
-  part_block(a, g0, width, ns);
-  int wlo, wn;
-  part_window(a, g0, width, wlo, wn);
-  for (int t = threadIdx.x; t < wn; t += blockDim.x) h[t] = 0;
+  const PartBlock col = part_block(a);
+  const int g0 = col.g0, width = col.width, ns = col.nslots;
+  for (int t = threadIdx.x; t < col.wn; t += blockDim.x) h[t] = 0;
   __syncthreads();
   for (int s = (int)threadIdx.x; s < ns; s += 4 * blockDim.x) {
     int c[4];
@@ -700,13 +711,11 @@ __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (c[u] >= 0) atomicAdd(&h[(c[u] >> a.tb) - wlo], 1);
+      if (c[u] >= 0) atomicAdd(&h[(c[u] >> a.tb) - col.wlo], 1);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x) {
-    const int r = t - wlo;
-    a.hist[(size_t)t * a.nblocks + blockIdx.x] = r >= 0 && r < wn ? h[r] : 0;
-  }
+  for (int r = threadIdx.x; r < col.wn; r += blockDim.x)
+    a.hist[col.base + (long long)r * col.stride] = h[r];
 }
 
 // Scatter without staging (any tile window): one 16-byte entry per step at
@@ -714,20 +723,45 @@ __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
 template <int MAXT>
 __global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
   __shared__ int cur[MAXT];
-  for (int t = threadIdx.x; t < a.ntiles; t += blockDim.x)
-    cur[t] = a.hist[(size_t)t * a.nblocks + blockIdx.x];
+  const PartBlock col = part_block(a);
+  const int g0 = col.g0, width = col.width, ns = col.nslots;
+  for (int r = threadIdx.x; r < col.wn; r += blockDim.x)
+    cur[r] = a.hist[col.base + (long long)r * col.stride];
   __syncthreads();
-  int g0, width, ns;
-  part_block(a, g0, width, ns);
   for (int s = (int)threadIdx.x; s < ns; s += blockDim.x) {
     const int k = part_slot(a, g0, width, s);
     if (k >= 0) {
       const int4 sl = a.slog[k];
       const int c = sl.x;
-      const int e = atomicAdd(&cur[c >> a.tb], 1);
+      const int e = atomicAdd(&cur[(c >> a.tb) - col.wlo], 1);
       a.ent[e] = make_int4(k, c, sl.z, sl.w);
     }
   }
+}
+
+// First entry of every tile in the scanned count matrices: tile t starts at
+// its row in the terrain holding its first cell (a tile shared by
+// neighbouring terrains continues in the next terrain's first row, which
+// follows in memory; a terrain without blocks has an empty matrix whose
+// offset is the next one's); start[ntiles] = all entries.  Without hist
+// (no entries) every start is 0.
+__global__ void tile_start_kernel(const Terrain* __restrict__ ter, int nter, int ntiles,
+                                  const int32_t* __restrict__ hist, long long total_pos,
+                                  int32_t* start) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  if (!hist) {
+    start[t] = 0;
+    return;
+  }
+  if (t == ntiles) {
+    start[t] = hist[total_pos];
+    return;
+  }
+  const int q = find_terrain(nter, (long long)t << kTileBits,
+                             [&](int k) { return ter[k].cell_off; });
+  const Terrain& T = ter[q];
+  start[t] = hist[T.part_off + (long long)(t - T.tile_lo) * T.part_blocks];
 }
 
 // Exclusive scan of v[0, m) into out[0, m) by a block of 1024 threads,
@@ -785,12 +819,11 @@ __global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
   __shared__ int4 stage[RS];
   __shared__ int wsum[16];
   const int tid = threadIdx.x;
-  int g0, width, ns;
-  part_block(a, g0, width, ns);
+  const PartBlock col = part_block(a);
+  const int g0 = col.g0, width = col.width, ns = col.nslots;
   if (ns == 0) return;  // uniform over the block
-  int wlo, wn;
-  part_window(a, g0, width, wlo, wn);
-  for (int t = tid; t < wn; t += 1024) cur[t] = a.hist[(size_t)(wlo + t) * a.nblocks + blockIdx.x];
+  const int wlo = col.wlo, wn = col.wn;
+  for (int t = tid; t < wn; t += 1024) cur[t] = a.hist[col.base + (long long)t * col.stride];
   for (int r0 = 0; r0 < ns; r0 += RS) {
     for (int t = tid; t < wn; t += 1024) cnt[t] = 0;
     __syncthreads();
@@ -842,9 +875,8 @@ __global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
 
 // entry range of part s of tile t
 template <int S>
-__device__ inline void tile_part(const int32_t* __restrict__ hist, int nblocks, int t, int s,
-                                 int& a, int& b) {
-  const int e0 = hist[(size_t)t * nblocks], e1 = hist[(size_t)(t + 1) * nblocks];
+__device__ inline void tile_part(const int32_t* __restrict__ start, int t, int s, int& a, int& b) {
+  const int e0 = start[t], e1 = start[t + 1];
   if (S == 1) {
     a = e0;
     b = e1;
@@ -873,7 +905,7 @@ __device__ inline void cell_count_finish(size_t i, int k, float2* sr, int32_t* f
 
 template <int S>
 __global__ void __launch_bounds__(1024) tile_count_kernel(
-    const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent, size_t n,
+    const int32_t* __restrict__ start, const int4* __restrict__ ent, size_t n,
     float2* sr, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
     const float* __restrict__ root, const Terrain* __restrict__ ter, int nter, int32_t* part) {
   __shared__ int cnt[kTileCells];
@@ -882,7 +914,7 @@ __global__ void __launch_bounds__(1024) tile_count_kernel(
   const int t = blockIdx.x / S, s = blockIdx.x % S;
   const int base = t << kTileBits;
   int e0, e1;
-  tile_part<S>(hist, nblocks, t, s, e0, e1);
+  tile_part<S>(start, t, s, e0, e1);
   // four entries per thread in flight
   for (int e = e0 + (int)threadIdx.x; e < e1; e += 4 * blockDim.x) {
     int c[4];
@@ -945,7 +977,7 @@ __global__ void volq_kernel(const Terrain* __restrict__ ter, int nter, int rows,
 // in tile_count_kernel and tile_reduce_finish_kernel adds the parts.
 template <int S>
 __global__ void __launch_bounds__(1024) tile_reduce_kernel(
-    const int32_t* __restrict__ hist, int nblocks, const int4* __restrict__ ent,
+    const int32_t* __restrict__ start, const int4* __restrict__ ent,
     const int32_t* __restrict__ qb, int qstride, const int32_t* __restrict__ vq, int rows,
     int drops, const Terrain* __restrict__ ter, int nter, int32_t* deposit, size_t n,
     int32_t* dh, long long* td, long long* tmx, long long* tmy, long long* part) {
@@ -966,7 +998,7 @@ __global__ void __launch_bounds__(1024) tile_reduce_kernel(
                                                                           : (long long)n - 1;
   const int t_hi = find_terrain(nter, last, key);
   int e0, e1;
-  tile_part<S>(hist, nblocks, tile, s, e0, e1);
+  tile_part<S>(start, tile, s, e0, e1);
   // four entries per thread in flight: the entry loads, then their gathers
   for (int e = e0 + (int)threadIdx.x; e < e1; e += 4 * blockDim.x) {
     int4 p[4];

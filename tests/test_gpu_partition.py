@@ -1,12 +1,14 @@
 """The step-log partition's code paths, bit-exact against the oracle.
 
 The cycle groups its step log by 2048-cell tile.  Which scatter runs depends
-on the largest tile window of a partition block (the tiles of the terrains of
-its drops): up to 1024 tiles (maps up to 2^21 cells; every other GPU test),
-up to 4096 tiles (2048x2048 here) and the unstaged scatter beyond that
-(4096x4096 here).  Also: a cycle without drops (empty partition, the tile
-reduce still writes zero commits and tracks) and a parameter change between
-cycles (the per-terrain volume-track table is rebuilt).
+on the most tiles of one terrain: up to 1024 tiles (maps up to 2^21 cells;
+every other GPU test), up to 4096 tiles (2048x2048 here) and the unstaged
+scatter beyond that (4096x4096 here).  Also: a cycle without drops (empty
+partition, the tile reduce still writes zero commits and tracks), a
+parameter change between cycles (the per-terrain volume-track table is
+rebuilt), and ensembles whose terrains share tiles, fit inside one tile or
+have no drops, with the walk counting the tiles (blocks of 64 and 256) and
+without (blocks of 512 span terrains; a persistent walk grid).
 """
 import numpy as np
 import pytest
@@ -63,3 +65,45 @@ def test_partition_empty_and_parameter_change(shy, orc):
     P3.maxAge = 700.0  # more log rows than the buffers held
     run_pair(shy, orc, 256, 9, [(4096, P1), (0, None), (4096, None), (4096, P2),
                                 (4096, None), (4096, P3), (4096, None)])
+
+
+def ensemble_vs_oracle(shy, orc, specs, cycles, tuning):
+    e = shy.Ensemble(specs)
+    for t, sp in enumerate(specs):
+        e.generate_fbm(t, sp["seed"], 8)
+    if tuning:
+        e.set_tuning(**tuning)
+    print("partition ensemble", [(s["nx"], s["ny"], s["drops"]) for s in specs], tuning,
+          flush=True)
+    e.erode(cycles, graph=False)
+    steps = e.terrain_steps()
+    for t, sp in enumerate(specs):
+        nx, ny = sp["nx"], sp["ny"]
+        h = orc.fbm(sp["seed"], nx, ny)
+        z = np.zeros_like(h)
+        d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+        P = orc.Params.default()
+        st = 0
+        for c in range(cycles):
+            st += orc.wave_cycle(h, d, mx, my, root, erfd, P, sp["seed"], c, sp["drops"],
+                                 terrain=t).steps
+        for name, ref in (("height", h), ("discharge", d), ("momentumx", mx),
+                          ("momentumy", my), ("erf_discharge", erfd)):
+            assert np.array_equal(bits(e.field(t, name)), bits(ref)), (t, name, tuning)
+        assert steps[t] == st, (t, tuning)
+    e.close()
+
+
+# zero-drop terrains first, in the middle and last; three terrains inside one
+# tile; a terrain of 4.5 tiles; drop counts that are not multiples of 64
+MIXED = [(24, 24, 0), (32, 32, 200), (16, 40, 37), (20, 20, 90), (96, 96, 0),
+         (80, 115, 700), (48, 48, 130), (30, 30, 0)]
+
+
+@pytest.mark.parametrize("tuning", [None, dict(walk_block=64, walk_grid=0, tile=0),
+                                    dict(walk_block=512, walk_grid=0, tile=0),
+                                    dict(walk_block=128, walk_grid=7, tile=0)])
+def test_partition_ensemble_shared_and_empty_terrains(shy, orc, tuning):
+    specs = [dict(nx=nx, ny=ny, drops=dr, seed=300 + i)
+             for i, (nx, ny, dr) in enumerate(MIXED)]
+    ensemble_vs_oracle(shy, orc, specs, 3, tuning)
