@@ -301,6 +301,14 @@ int shy_synchronize(shy_ctx* ctx);
 int shy_set_walk_block(shy_ctx* ctx, int threads);
 int shy_set_tuning(shy_ctx* ctx, const shy_tuning* t);
 int shy_get_tuning(const shy_ctx* ctx, shy_tuning* t);
+/* Drop order of the count walk (no reference counterpart).  on = 1: each
+   cycle assigns drops to walk threads by the position of their spawn cell,
+   so that each XCD walks drops of one region and consecutive blocks walk
+   neighbouring drops; on = 0: thread g walks drop g; on = -1 (the default):
+   ordered when a cycle has at least 262144 drops (more walk threads than
+   the GPU runs at once, where the order shortens the walk), else not.
+   Results are bit-identical either way. */
+int shy_set_drop_order(shy_ctx* ctx, int on);
 /* Autotuner (no reference counterpart): times `cycles` cycles of every
    candidate launch shape (4 walk block sizes x one-lane-per-drop or
    persistent x 3 stencil tiles) from a copy of the live state, restores the

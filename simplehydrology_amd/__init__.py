@@ -198,6 +198,7 @@ def lib():
             "shy_reset_stats": (i32, [vp]),
             "shy_synchronize": (i32, [vp]),
             "shy_set_walk_block": (i32, [vp, i32]),
+            "shy_set_drop_order": (i32, [vp, i32]),
             "shy_time_cycles": (i32, [vp, i32, i32, fp]),
             "shy_ensemble_create": (i32, [vp, i32, C.POINTER(vp)]),
             "shy_ensemble_count": (i32, [vp, C.POINTER(i32)]),
@@ -467,6 +468,11 @@ class World:
 
     def set_walk_block(self, threads: int):
         _check(self._ctx, lib().shy_set_walk_block(self._ctx, threads))
+
+    def set_drop_order(self, on):
+        """Spatial drop order of the count walk: True, False or None (automatic,
+        the default: on from 262144 drops per cycle); bit-identical results."""
+        _check(self._ctx, lib().shy_set_drop_order(self._ctx, -1 if on is None else int(bool(on))))
 
     def tuning(self) -> dict:
         t = Tuning()

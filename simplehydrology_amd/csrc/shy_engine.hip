@@ -89,6 +89,9 @@ struct shy_ctx {
   long long* tsum = nullptr;    // per-part tile sums
   int tile_split = 1;           // kTileSplit when the part buffers are small, else 1
   int4* dlog = nullptr;  // [drop] how each walk ended
+  int32_t* dperm = nullptr;  // [drop] the terrain's drop each walk thread takes (order_kernel)
+  int drop_order = -1;       // assign drops to walk threads by spawn position: 1, 0, -1 auto
+  int order_chunks = 0;      // blocks of order_kernel
   int32_t* wsteps = nullptr;  // [drop] steps of each walk
   float4* pack = nullptr;  // per cell, 4 x float4: the count walk's record
   size_t dlog_cap = 0;
@@ -124,6 +127,14 @@ constexpr int kWaveDrops = 64;  // drop ranges are padded to whole wave64s
 // drop ranges of an ensemble are padded to 256 drops: walk blocks of 64, 128
 // or 256 threads then never span terrains
 constexpr int kDropAlign = 256;
+// drops from which the automatic drop order is on: more walk threads than
+// the GPU runs at once, so the walk's blocks run in sequence and neighbouring
+// blocks share the L2 when their drops start near each other
+constexpr int kOrderMinDrops = 262144;
+
+bool ordered(const shy_ctx* c) {
+  return c->drop_order < 0 ? c->total_drops >= kOrderMinDrops : c->drop_order != 0;
+}
 constexpr int kReplayBlock = 256;
 
 std::string g_noctx_err;
@@ -242,8 +253,9 @@ constexpr int kTileSplitMaxTiles = 2048;
 // over all terrains and sets blocks (over all terrains) and window (most
 // tiles of a terrain).
 size_t part_geometry(const shy_ctx* c, int walk_block, std::vector<Terrain>* t, int& blocks,
-                     int& window) {
+                     int& window, int* chunks = nullptr) {
   size_t off = 0;
+  int nchunk = 0;
   blocks = 0;
   window = 1;
   for (size_t k = 0; k < c->ter.size(); ++k) {
@@ -266,11 +278,14 @@ size_t part_geometry(const shy_ctx* c, int walk_block, std::vector<Terrain>* t, 
       o.part_dwidth = dwidth;
       o.tile_lo = lo;
       o.tile_n = tn;
+      o.order_first = nchunk;
     }
     off += (size_t)tn * nb;
     blocks += nb;
     window = std::max(window, tn);
+    nchunk += (padded + kOrderChunk - 1) / kOrderChunk;
   }
+  if (chunks) *chunks = nchunk;
   return off;
 }
 
@@ -295,7 +310,7 @@ int ensure_wave(shy_ctx* c) {
                            part_geometry(c, bs, nullptr, nb, window) <= ((size_t)1 << 22) &&
                            window <= kFuseMaxTiles;
   std::vector<Terrain> geo = c->ter;
-  const size_t entries = part_geometry(c, walk_counts ? bs : 0, &geo, nb, window);
+  const size_t entries = part_geometry(c, walk_counts ? bs : 0, &geo, nb, window, &c->order_chunks);
   if (memcmp(geo.data(), c->ter.data(), geo.size() * sizeof(Terrain)) != 0) {
     drop_graph(c);
     c->ter = geo;
@@ -397,11 +412,14 @@ int ensure_wave(shy_ctx* c) {
   if (need_d > c->dlog_cap) {
     if (c->dlog) hipFree(c->dlog);
     if (c->wsteps) hipFree(c->wsteps);
+    if (c->dperm) hipFree(c->dperm);
     c->dlog = nullptr;
     c->wsteps = nullptr;
+    c->dperm = nullptr;
     c->dlog_cap = 0;
     HIPCHK(c, hipMalloc(&c->dlog, need_d * sizeof(int4)));
     HIPCHK(c, hipMalloc(&c->wsteps, need_d * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc(&c->dperm, need_d * sizeof(int32_t)));
     c->dlog_cap = need_d;
   }
   if (hist_need > c->hist_cap) {
@@ -432,6 +450,7 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.dlog = c->dlog;
   w.steps = c->wsteps;
   w.hist = c->hist;
+  w.perm = ordered(c) ? c->dperm : nullptr;
   w.pack = c->pack;
   w.deposit = c->deposit;
   w.visit = c->visit[p];
@@ -464,6 +483,9 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (walk) {
     pack_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, n, c->h[p],
                                                                      c->fld, c->root, c->pack);
+    if (ordered(c))
+      order_kernel<<<c->order_chunks, 1024, 0, c->stream>>>(c->d_ter, nter, c->total_drops, bs, c->state,
+                                                 c->dperm);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
@@ -736,7 +758,7 @@ int shy_destroy(shy_ctx* c) {
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
                   c->sr, c->dh, c->trk, c->deposit, c->slog, c->qb, c->ent,
-                  c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->tstart, c->dlog, c->wsteps, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
+                  c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->tstart, c->dlog, c->wsteps, c->dperm, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
   for (void* q : ptrs)
@@ -1616,6 +1638,14 @@ int shy_set_tuning(shy_ctx* c, const shy_tuning* t) {
 int shy_get_tuning(const shy_ctx* c, shy_tuning* t) {
   if (!c || !t) return SHY_ERR_ARG;
   *t = c->tune;
+  return SHY_OK;
+}
+
+int shy_set_drop_order(shy_ctx* c, int on) {
+  if (!c) return SHY_ERR_ARG;
+  if (on < -1 || on > 1) return fail(c, SHY_ERR_ARG, "drop order must be 1, 0 or -1 (automatic)");
+  if (c->drop_order != on) drop_graph(c);
+  c->drop_order = on;
   return SHY_OK;
 }
 

@@ -77,6 +77,7 @@ struct Terrain {
   long long part_off;
   int part_first, part_blocks, part_dwidth;
   int tile_lo, tile_n;
+  int order_first;  // the drop order's first chunk
   unsigned long long seed;
   unsigned int id;  // terrain number in the spawn hash
   int has_root;
@@ -105,6 +106,7 @@ struct WaveArgs {
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   int32_t* hist;      // per-terrain [tile][walk block] step counts, when the walk counts them
+  const int32_t* __restrict__ perm;  // [drop] the terrain's drop a thread walks (order_kernel), or null
   const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
   int32_t* deposit;
   uint8_t* visit;
@@ -392,7 +394,8 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
         find_terrain(w.nter, g0, [&](int k) { return (long long)w.ter[k].drop_off; }));
     const Terrain T = w.ter[t];
     int4 dl = make_int4(0, kEndPad, 0, 0);
-    if (g - T.drop_off < T.drops) {  // not a padding lane
+    const int loc = w.perm ? w.perm[g] : g - T.drop_off;  // the drop this thread walks
+    if (loc < T.drops) {  // not a padding lane
       WalkArgs a{};
       a.h = w.h + T.cell_off;
       a.fld = w.fld + T.cell_off;
@@ -406,7 +409,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.P = T.P;
       const uint32_t ncell = (uint32_t)T.nx * (uint32_t)T.ny;
       const uint32_t cid =
-          spawn_cell(cycle_key(T.seed, T.id, cycle), (uint32_t)(g - T.drop_off), ncell);
+          spawn_cell(cycle_key(T.seed, T.id, cycle), (uint32_t)loc, ncell);
       dl.y = kEndSkipped;
       if ((double)a.h[cid] >= 0.1) {  // world.h:71-72
         int end = 0, end_cell = 0;
@@ -656,7 +659,7 @@ struct PartArgs {
   int32_t* hist;   // per terrain [tile][block] counts, then exclusive offsets; [total] = entries
   int4* ent;       // [entry] {slot, start cell, qt(vol sx), qt(vol sy)}, tile order
   int rows;        // log rows (0: no walk this cycle)
-  int drops;       // log stride
+  int drops;       // log stride = all drops (padded)
   int tb;          // log2 cells per tile
 };
 
@@ -672,10 +675,14 @@ struct PartBlock {
 
 __device__ inline PartBlock part_block(const PartArgs& a) {
   const int b = blockIdx.x;
-  const Terrain& T = a.ter[find_terrain(a.nter, (long long)b,
-                                        [&](int q) { return (long long)a.ter[q].part_first; })];
+  const int t =
+      find_terrain(a.nter, (long long)b, [&](int q) { return (long long)a.ter[q].part_first; });
+  const Terrain& T = a.ter[t];
   const int j = b - T.part_first;
-  const int left = T.drops - j * T.part_dwidth;
+  // the terrain's whole padded drop range: the drop order may place real
+  // drops on any of its threads (padding threads log no steps)
+  const int end = t + 1 < a.nter ? a.ter[t + 1].drop_off : a.drops;
+  const int left = end - T.drop_off - j * T.part_dwidth;
   PartBlock p;
   p.g0 = T.drop_off + j * T.part_dwidth;
   p.width = left < T.part_dwidth ? (left > 0 ? left : 0) : T.part_dwidth;
@@ -804,6 +811,89 @@ __device__ inline void block_scan_runs(const int* v, int* out, int m, int* wsum)
     if (j < per && i < m) out[i] = run;
     run += loc[j];
   }
+}
+
+// ---------------------------------------------------------------- drop order
+// Each step of the count walk waits on its cell's 64-byte record, which
+// mostly misses the XCD's L2.  When a cycle has more drops than the GPU runs
+// at once, the walk's blocks run in sequence, and blocks that walk drops
+// starting near each other on one XCD share its L2.  Drops are independent
+// within a cycle, every log is indexed by the thread that walked it and
+// every per-cell result is an integer sum, so any assignment of drops to
+// threads gives the same results.  order_kernel assigns them spatially, per
+// chunk of kOrderChunk threads of a terrain (one block each): the chunk's
+// drops sorted by the Morton code of their spawn cell on a grid of at most
+// 64 x 64 bins, and the sorted list laid onto the chunk's walk blocks class
+// by class, class x holding the blocks b with b % 8 == x (the XCD a
+// workgroup runs on: workgroups are dispatched round-robin over the eight
+// XCDs).  Every chunk puts the same eighth of its terrain on the same XCD,
+// and the lanes of a wave start near each other.
+// perm[g] = the terrain's drop walked by thread g (padding threads receive
+// numbers >= drops).  A chunk's partial last block of bs keeps its
+// positions, so the order is a permutation for every bs; the classes match
+// the XCDs when bs divides the terrain's drop offset and the walk grid is
+// one block per bs drops.
+constexpr int kOrderChunk = 65536;
+
+// position of sorted entry s of a chunk of nbl blocks of bs whose first
+// block is global block b0: class-major, blocks of a class in order
+__device__ inline int order_position(int s, int bs, int nbl, int b0) {
+  const int k = s / bs, o = s - k * bs;
+  int cum = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int d = (x - b0) & 7;
+    const int n = d < nbl ? (nbl - 1 - d) / 8 + 1 : 0;
+    if (k < cum + n) return (d + 8 * (k - cum)) * bs + o;
+    cum += n;
+  }
+  return s;  // the partial last block keeps its positions
+}
+
+__device__ inline int morton_bin(uint32_t cid, int ny, int sx, int sy) {
+  uint32_t bx = (cid / (uint32_t)ny) >> sx, by = (cid % (uint32_t)ny) >> sy;
+  uint32_t m = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) m |= ((bx >> i) & 1u) << (2 * i) | ((by >> i) & 1u) << (2 * i + 1);
+  return (int)m;
+}
+
+// One block of 1024 threads per chunk; Terrain::order_first numbers the
+// chunks of the terrains.  The second pass recomputes each drop's bin (a
+// global round trip per drop costs more at one block per CU).
+__global__ void __launch_bounds__(1024) order_kernel(const Terrain* __restrict__ ter, int nter,
+                                                     int total_drops, int bs,
+                                                     const DeviceState* __restrict__ state,
+                                                     int32_t* perm) {
+  __shared__ int cnt[4096];
+  __shared__ int off[4096];
+  __shared__ int wsum[16];
+  const int t = find_terrain(nter, (long long)blockIdx.x,
+                             [&](int q) { return (long long)ter[q].order_first; });
+  const Terrain& T = ter[t];
+  const int end = t + 1 < nter ? ter[t + 1].drop_off : total_drops;
+  const int P = end - T.drop_off;  // the terrain's padded drops
+  const int c0 = ((int)blockIdx.x - T.order_first) * kOrderChunk;
+  const int c1 = c0 + kOrderChunk < P ? c0 + kOrderChunk : P;
+  const int r1 = c1 < T.drops ? c1 : T.drops;  // real drops of the chunk: [c0, r1)
+  const int nbl = (c1 - c0) / bs, b0 = (T.drop_off + c0) / bs;
+  const uint32_t ncell = (uint32_t)T.nx * (uint32_t)T.ny;
+  const uint64_t key = cycle_key(T.seed, T.id, state->cycle);
+  int sx = 0, sy = 0;
+  while (((T.nx - 1) >> sx) >= 64) ++sx;
+  while (((T.ny - 1) >> sy) >= 64) ++sy;
+  for (int b = threadIdx.x; b < 4096; b += 1024) cnt[b] = 0;
+  __syncthreads();
+  for (int i = c0 + (int)threadIdx.x; i < r1; i += 1024)
+    atomicAdd(&cnt[morton_bin(spawn_cell(key, (uint32_t)i, ncell), T.ny, sx, sy)], 1);
+  __syncthreads();
+  block_scan_runs(cnt, off, 4096, wsum);
+  __syncthreads();
+  for (int i = c0 + (int)threadIdx.x; i < r1; i += 1024) {
+    const int s = atomicAdd(&off[morton_bin(spawn_cell(key, (uint32_t)i, ncell), T.ny, sx, sy)], 1);
+    perm[T.drop_off + c0 + order_position(s, bs, nbl, b0)] = i;
+  }
+  for (int s = (r1 > c0 ? r1 : c0) - c0 + (int)threadIdx.x; s < c1 - c0; s += 1024)
+    perm[T.drop_off + c0 + order_position(s, bs, nbl, b0)] = c0 + s;
 }
 
 // The scatter with its writes grouped: each round of RS slots is ranked per

@@ -107,3 +107,34 @@ def test_partition_ensemble_shared_and_empty_terrains(shy, orc, tuning):
     specs = [dict(nx=nx, ny=ny, drops=dr, seed=300 + i)
              for i, (nx, ny, dr) in enumerate(MIXED)]
     ensemble_vs_oracle(shy, orc, specs, 3, tuning)
+
+
+def test_drop_order_bit_identical(shy):
+    # the spatial drop order only changes which thread walks which drop
+    out = []
+    for on in (True, False):
+        w = shy.World(512, 512, seed=5)
+        w.generate_fbm(5, 8)
+        w.set_drop_order(on)
+        print("drop order", on, flush=True)
+        w.erode_cycles(3, 8000, graph=False)
+        w.erode_cycles(2, 8000, graph=True)
+        out.append([w.field(k) for k in ("height", "discharge", "momentumx", "momentumy",
+                                         "erf_discharge")] + [w.stats()])
+        w.close()
+    for a, b in zip(out[0][:-1], out[1][:-1]):
+        assert np.array_equal(bits(a), bits(b))
+    assert out[0][-1] == out[1][-1]
+    specs = [dict(nx=nx, ny=ny, drops=dr, seed=500 + i) for i, (nx, ny, dr) in enumerate(MIXED)]
+    res = []
+    for on in (True, False):
+        e = shy.Ensemble(specs)
+        for t, sp in enumerate(specs):
+            e.generate_fbm(t, sp["seed"], 8)
+        e.set_tuning(walk_block=64, walk_grid=0, tile=0)
+        e.set_drop_order(on)
+        e.erode(3, graph=True)
+        res.append([e.field(t, "height") for t in range(len(specs))] + [e.terrain_steps()])
+        e.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
