@@ -901,37 +901,61 @@ __global__ void __launch_bounds__(1024) order_kernel(const Terrain* __restrict__
 // entries of one tile go to consecutive addresses.  LDS holds the block's
 // tile window (at most MAXT tiles, which the host checks) and RS entries.
 template <int MAXT, int RS>
-__global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
+// (two blocks of 1024 threads per CU: at most 64 VGPRs)
+__global__ void __launch_bounds__(1024, 8) part_scatter_staged_kernel(PartArgs a) {
   constexpr int PER = RS / 1024;
+  // drops whose step counts the block keeps in LDS: with 1024 tiles and 4096
+  // staged entries the block stays within 80 KB, two blocks per CU
+  constexpr int kMaxW = (MAXT <= 1024 && RS <= 4096) ? 1008 : 4096;
   __shared__ int cur[MAXT];  // next entry of each tile of the window
   __shared__ int cnt[MAXT];  // the round's entries per tile
   __shared__ int off[MAXT];  // their exclusive offsets in the stage
   __shared__ int4 stage[RS];
+  __shared__ int wst[kMaxW];  // steps of the block's drops
   __shared__ int wsum[16];
   const int tid = threadIdx.x;
   const PartBlock col = part_block(a);
   const int g0 = col.g0, width = col.width, ns = col.nslots;
   if (ns == 0) return;  // uniform over the block
   const int wlo = col.wlo, wn = col.wn;
+  const bool lds_steps = width <= kMaxW;
+  if (lds_steps)
+    for (int d = tid; d < width; d += 1024) wst[d] = a.steps[g0 + d];
   for (int t = tid; t < wn; t += 1024) cur[t] = a.hist[col.base + (long long)t * col.stride];
+  __syncthreads();
+  // log index of the block's slot x, or -1 (past the slots, or the drop took
+  // fewer steps)
+  auto slot = [&](int x) {
+    if (x >= ns) return -1;
+    const int row = x / width, d = x - row * width;
+    const int st = lds_steps ? wst[d] : a.steps[g0 + d];
+    return row < st ? row * a.drops + g0 + d : -1;
+  };
+  // The next round's log records are loaded while this round is ranked,
+  // staged and written.  The loads are unconditional (an invalid slot reads
+  // record 0, unused) so that no wait inside the round covers them.
+  int kq[PER];
+  int4 lv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    kq[u] = slot(u * 1024 + tid);
+    lv[u] = a.slog[kq[u] >= 0 ? kq[u] : 0];
+  }
   for (int r0 = 0; r0 < ns; r0 += RS) {
     for (int t = tid; t < wn; t += 1024) cnt[t] = 0;
-    __syncthreads();
     int4 ev[PER];
     int tl[PER], rk[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int x = r0 + u * 1024 + tid;
-      tl[u] = -1;
-      ev[u] = make_int4(0, 0, 0, 0);
-      const int k = x < ns ? part_slot(a, g0, width, x) : -1;
-      if (k >= 0) {
-        const int4 sl = a.slog[k];
-        const int c = sl.x;
-        ev[u] = make_int4(k, c, sl.z, sl.w);
-        tl[u] = (c >> a.tb) - wlo;
-      }
+      ev[u] = make_int4(kq[u], lv[u].x, lv[u].z, lv[u].w);
+      tl[u] = kq[u] >= 0 ? (lv[u].x >> a.tb) - wlo : -1;
     }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      kq[u] = slot(r0 + RS + u * 1024 + tid);
+      lv[u] = a.slog[kq[u] >= 0 ? kq[u] : 0];
+    }
+    __syncthreads();
 #pragma unroll
     for (int u = 0; u < PER; ++u) rk[u] = tl[u] >= 0 ? atomicAdd(&cnt[tl[u]], 1) : 0;
     __syncthreads();
@@ -949,7 +973,6 @@ __global__ void __launch_bounds__(1024) part_scatter_staged_kernel(PartArgs a) {
     }
     __syncthreads();
     for (int t = tid; t < wn; t += 1024) cur[t] += cnt[t];
-    __syncthreads();
   }
 }
 
