@@ -10,8 +10,8 @@ particle-steps/s over exactly --steps timed cycles, inputs resident in HBM.
 Prints one JSON line.  Roofline: the kernel with the largest measured time
 per cycle, against the HBM peak, with its algorithmic bytes per
 particle-step:
-  count walk (walk_kernel)  B_COUNT = 53: reads 5 heights and
-      {mx, my, d, erf(0.4 d)} of the step's cell = 36 B; writes the 16-B step
+  count walk (walk_kernel)  B_COUNT = 45: reads {h, normal x, normal z,
+      erf(0.4 d), mx, my, d} of the step's cell = 28 B; writes the 16-B step
       record {start cell, c_eq, tracks}; cascade-site byte 1 B
   commit replay (replay_kernel)  B_REPLAY = 20: reads the start cell and c_eq
       of the step record 8 B and the cell's {burst scale, root density} 8 B;
@@ -39,7 +39,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-B_COUNT = 53          # algorithmic bytes per particle-step, count walk
+B_COUNT = 45          # algorithmic bytes per particle-step, count walk
 B_REPLAY = 20         # algorithmic bytes per particle-step, commit replay
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNELS = (("walk_kernel", "ms_count_walk", B_COUNT), ("replay_kernel", "ms_replay", B_REPLAY))

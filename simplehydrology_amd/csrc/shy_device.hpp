@@ -98,6 +98,7 @@ struct CellView {
   float h, xm, xp, ym, yp;  // h(x,y), h(x-1,y), h(x+1,y), h(x,y-1), h(x,y+1)
   float mx, my, d, erfd;    // momentum, discharge, erf(0.4 d)
   float root;               // rootdensity
+  float n0, n2;             // the cell's normal (x, z), when read from a packed record
 };
 
 // quad::_normal (cellpool.h:181-204) from a loaded stencil.  Plane k is

@@ -24,8 +24,8 @@
 //   hist        int32   per terrain a [tile][block] count matrix of the
 //                       partition, then exclusive offsets over all of them
 //   tstart      int32   [tile] first partition entry of every cell tile
-//   pack        float4x4 per cell: the count walk's one-line record of the
-//                       cycle's start (5 heights, flow, root), by pack_kernel
+//   pack        float4x2 per cell: the count walk's 32-byte record of the
+//                       cycle's start (height, normal, flow), by pack_kernel
 //   sr          float2  {burst scale, root density} per cell (tile_count_kernel)
 //   tcount, tsum int32, int64x4  per-part tile counts and sums when a tile is
 //                       split over kTileSplit blocks
@@ -93,7 +93,7 @@ struct shy_ctx {
   int drop_order = -1;       // assign drops to walk threads by spawn position: 1, 0, -1 auto
   int order_chunks = 0;      // blocks of order_kernel
   int32_t* wsteps = nullptr;  // [drop] steps of each walk
-  float4* pack = nullptr;  // per cell, 4 x float4: the count walk's record
+  float4* pack = nullptr;  // per cell, 2 x float4: the count walk's record
   size_t dlog_cap = 0;
   long long* tsteps = nullptr;
   DeviceState* state = nullptr;
@@ -482,7 +482,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   const bool walk = c->total_drops > 0;
   if (walk) {
     pack_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, n, c->h[p],
-                                                                     c->fld, c->root, c->pack);
+                                                                     c->fld, c->pack);
     if (ordered(c))
       order_kernel<<<c->order_chunks, 1024, 0, c->stream>>>(c->d_ter, nter, c->total_drops, bs, c->state,
                                                  c->dperm);
@@ -821,7 +821,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
   if ((e = hipMalloc(&c->sr, n * sizeof(float2))) != hipSuccess) return bail(e, "hipMalloc sr");
-  if ((e = hipMalloc(&c->pack, 4 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
+  if ((e = hipMalloc(&c->pack, 2 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");

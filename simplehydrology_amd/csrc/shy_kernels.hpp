@@ -107,7 +107,7 @@ struct WaveArgs {
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   int32_t* hist;      // per-terrain [tile][walk block] step counts, when the walk counts them
   const int32_t* __restrict__ perm;  // [drop] the terrain's drop a thread walks (order_kernel), or null
-  const float4* __restrict__ pack;  // per cell 64 B: what one walk step reads (pack_kernel)
+  const float4* __restrict__ pack;  // per cell 32 B: what one walk step reads (pack_kernel)
   int32_t* deposit;
   uint8_t* visit;
   DeviceState* state;
@@ -190,11 +190,12 @@ __device__ inline void block_add_stats(long long (&v)[NC], long long* out) {
 // {erf(0.4 d), root, 0, 0}; neighbours off the map repeat the cell's own
 // height, as load_unpacked's clamped reads do (the normal never uses them).
 __device__ inline CellView load_packed(const float4* __restrict__ pk, int i) {
-  const float4 a = pk[4 * (size_t)i], b = pk[4 * (size_t)i + 1], c = pk[4 * (size_t)i + 2];
+  const float4 a = pk[2 * (size_t)i], b = pk[2 * (size_t)i + 1];
   CellView v;
-  v.h = a.x; v.xm = a.y; v.xp = a.z; v.ym = a.w;
-  v.yp = b.x; v.mx = b.y; v.my = b.z; v.d = b.w;
-  v.erfd = c.x; v.root = c.y;
+  v.h = a.x; v.n0 = a.y; v.n2 = a.z; v.erfd = a.w;
+  v.mx = b.x; v.my = b.y; v.d = b.z;
+  v.xm = v.xp = v.ym = v.yp = 0.0f;  // not read: the record holds the normal
+  v.root = 0.0f;                      // not read by the count walk (the replay's sr has it)
   return v;
 }
 
@@ -271,7 +272,12 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   CellView c = load_cell(ix, iy, i0);
   for (;;) {
     float n0, n1, n2;
-    normal_from(c, ix, iy, nx, ny, n0, n1, n2);  // water.h:70
+    if constexpr (COUNT) {  // the packed record holds the cell's normal (water.h:70)
+      n0 = c.n0;
+      n2 = c.n2;
+    } else {
+      normal_from(c, ix, iy, nx, ny, n0, n1, n2);  // water.h:70
+    }
     (void)n1;
     if ((float)age > P.maxAge || vol < P.minVol) {  // water.h:74-82
       *end = (float)age > P.maxAge ? kEndAge : kEndVol;
@@ -334,11 +340,15 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   return steps;
 }
 
-// The walk's per-cell record for this cycle (load_packed), every cell of
-// every terrain: 5 heights of the normal's stencil, the flow float4, root.
+// The count walk's 32-byte per-cell record for this cycle (load_packed),
+// every cell of every terrain: {h, normal x, normal z, erf(0.4 d)},
+// {momentum x, momentum y, discharge, 0}.  The normal (quad::_normal,
+// cellpool.h:181-204) depends only on the cell's 5-height stencil of the
+// snapshot, so it is computed here once per cell, with the walk's own
+// arithmetic, instead of at every step: a shorter chain per step.
 __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
                             const float* __restrict__ h, const float4* __restrict__ fld,
-                            const float* __restrict__ root, float4* pack) {
+                            float4* pack) {
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const int t = find_terrain(nter, (long long)g, [&](int q) { return ter[q].cell_off; });
@@ -350,10 +360,12 @@ __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
   const float xm = hh[x > 0 ? i - ny : i], xp = hh[x + 1 < T.nx ? i + ny : i];
   const float ym = hh[y > 0 ? i - 1 : i], yp = hh[y + 1 < ny ? i + 1 : i];
   const float4 f = fld[g];
-  const float r = T.has_root ? root[g] : 0.0f;
-  pack[4 * g] = make_float4(hc, xm, xp, ym);
-  pack[4 * g + 1] = make_float4(yp, f.x, f.y, f.z);
-  pack[4 * g + 2] = make_float4(f.w, r, 0.0f, 0.0f);
+  CellView c;
+  c.h = hc; c.xm = xm; c.xp = xp; c.ym = ym; c.yp = yp;
+  float n0, n1, n2;
+  normal_from(c, x, y, T.nx, ny, n0, n1, n2);  // water.h:70, once per cell instead of per step
+  pack[2 * g] = make_float4(hc, n0, n2, f.w);
+  pack[2 * g + 1] = make_float4(f.x, f.y, f.z, 0.0f);
 }
 
 // Count pass over every drop of every terrain; grid-stride, so the grid may
@@ -402,7 +414,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.root = T.has_root ? w.root + T.cell_off : nullptr;
       a.cell_off = (int)T.cell_off;
       a.slog = w.slog;
-      a.pack = w.pack ? w.pack + 4 * T.cell_off : nullptr;
+      a.pack = w.pack ? w.pack + 2 * T.cell_off : nullptr;
       a.nx = T.nx;
       a.ny = T.ny;
       a.drops = w.total_drops;  // log stride
@@ -814,7 +826,7 @@ __device__ inline void block_scan_runs(const int* v, int* out, int m, int* wsum)
 }
 
 // ---------------------------------------------------------------- drop order
-// Each step of the count walk waits on its cell's 64-byte record, which
+// Each step of the count walk waits on its cell's 32-byte record, which
 // mostly misses the XCD's L2.  When a cycle has more drops than the GPU runs
 // at once, the walk's blocks run in sequence, and blocks that walk drops
 // starting near each other on one XCD share its L2.  Drops are independent
