@@ -15,8 +15,8 @@
 //                       *_track fields do)
 //   deposit     int32   terminal deposits, 2^-20; cleared by tile_reduce_kernel
 // and per wave (sized by total drops x (max maxAge + 1) log slots):
-//   slog        int4    [step][drop] step log of the count walk: start cell,
-//                       c_eq and the momentum tracks of each step
+//   slog, strk  int2x2  [step][drop] step log of the count walk: {start cell,
+//                       c_eq} (what the replay reads) and {momentum tracks}
 //   qb          int32   [drop][step] height commit of each step (replay)
 //   ent         int4    the valid slots grouped by cell tile (partition)
 //   vq          int32   [terrain][step] quantised volume track of a step
@@ -26,7 +26,7 @@
 //   tstart      int32   [tile] first partition entry of every cell tile
 //   pack        float4x2 per cell: the count walk's 32-byte record of the
 //                       cycle's start (height, normal, flow), by pack_kernel
-//   sr          float2  {burst scale, root density} per cell (tile_count_kernel)
+//   sr          float   burst scale per cell (tile_count_kernel)
 //   tcount, tsum int32, int64x4  per-part tile counts and sums when a tile is
 //                       split over kTileSplit blocks
 // A device table of Terrain entries maps drops, cells and cascade tiles to
@@ -60,11 +60,12 @@ struct shy_ctx {
   float* root = nullptr;
   uint8_t* visit[2] = {nullptr, nullptr};
   int32_t* first = nullptr;
-  float2* sr = nullptr;
+  float* sr = nullptr;
   int32_t* dh = nullptr;
   long long* trk = nullptr;  // 3*n: d, mx, my
   int32_t* deposit = nullptr;
-  int4* slog = nullptr;      // [step][drop] step log of the count walk
+  int2* slog = nullptr;      // [step][drop] step log of the count walk: {start cell, c_eq}
+  int2* strk = nullptr;      // [step][drop] the steps' momentum tracks
   int32_t* qb = nullptr;     // [drop][step] height commits of the replay
   size_t qb_cap = 0;         // ints qb holds
   int qstride = 0;           // qb's stride per drop (log rows rounded up to kReplayDepth)
@@ -393,12 +394,14 @@ int ensure_wave(shy_ctx* c) {
   drop_graph(c);  // a captured graph holds the old buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (need > c->wave_cap) {
-    void* old[] = {c->slog, c->ent};
+    void* old[] = {c->slog, c->strk, c->ent};
     for (void* q : old)
       if (q) hipFree(q);
-    c->slog = c->ent = nullptr;
+    c->slog = c->strk = nullptr;
+    c->ent = nullptr;
     c->wave_cap = 0;
-    HIPCHK(c, hipMalloc(&c->slog, need * sizeof(int4)));
+    HIPCHK(c, hipMalloc(&c->slog, need * sizeof(int2)));
+    HIPCHK(c, hipMalloc(&c->strk, need * sizeof(int2)));
     HIPCHK(c, hipMalloc(&c->ent, need * sizeof(int4)));
     c->wave_cap = need;
   }
@@ -444,6 +447,7 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.root = c->root;
   w.first = c->first;
   w.slog = c->slog;
+  w.strk = c->strk;
   w.qb = c->qb;
   w.sr = c->sr;
   w.qstride = c->qstride;
@@ -500,6 +504,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   // partition of the valid log slots by cell tile (empty without drops)
   PartArgs pa;
   pa.slog = c->slog;
+  pa.strk = c->strk;
   pa.steps = c->wsteps;
   pa.ter = c->d_ter;
   pa.nter = nter;
@@ -544,8 +549,13 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (walk) {
-    replay_kernel<<<(c->total_drops + kReplayBlock - 1) / kReplayBlock, kReplayBlock, 0,
-                    c->stream>>>(w);
+    bool root = false;
+    for (const Terrain& T : c->ter) root = root || T.has_root;
+    const unsigned nb = (unsigned)((c->total_drops + kReplayBlock - 1) / kReplayBlock);
+    if (root)
+      replay_kernel<true><<<nb, kReplayBlock, 0, c->stream>>>(w);
+    else
+      replay_kernel<false><<<nb, kReplayBlock, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
@@ -757,7 +767,7 @@ int shy_destroy(shy_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
-                  c->sr, c->dh, c->trk, c->deposit, c->slog, c->qb, c->ent,
+                  c->sr, c->dh, c->trk, c->deposit, c->slog, c->strk, c->qb, c->ent,
                   c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->tstart, c->dlog, c->wsteps, c->dperm, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
@@ -820,7 +830,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->fld, n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc fld");
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
-  if ((e = hipMalloc(&c->sr, n * sizeof(float2))) != hipSuccess) return bail(e, "hipMalloc sr");
+  if ((e = hipMalloc(&c->sr, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc sr");
   if ((e = hipMalloc(&c->pack, 2 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
@@ -839,7 +849,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   hipMemsetAsync(c->fld, 0, n * sizeof(float4), c->stream);
   hipMemsetAsync(c->root, 0, n * sizeof(float), c->stream);
   hipMemsetAsync(c->first, 0, n * sizeof(int32_t), c->stream);
-  hipMemsetAsync(c->sr, 0, n * sizeof(float2), c->stream);
+  hipMemsetAsync(c->sr, 0, n * sizeof(float), c->stream);
   hipMemsetAsync(c->dh, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
   hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);

@@ -4,8 +4,8 @@
 // drops that all read the map as it stood when the cycle began:
 //
 //  walk_kernel           one drop per lane walks its trajectory; for every
-//                        step it logs, coalesced as [step][drop], one 16-byte
-//                        record {start cell, c_eq, tracks} of what the
+//                        step it logs, coalesced as [step][drop], 16 bytes
+//                        {start cell, c_eq} + {tracks} of what the
 //                        snapshot fixes (and, where its blocks are the
 //                        partition blocks, counts the step's cell tile in
 //                        LDS); per drop it logs how the walk ended, counts
@@ -99,10 +99,11 @@ struct WaveArgs {
   const float4* __restrict__ fld;
   const float* __restrict__ root;
   int32_t* first;     // drops whose first step starts at the cell
-  int4* slog;         // [step][drop] {start cell over all terrains, c_eq bits, qt(vol sx), qt(vol sy)}
+  int2* slog;         // [step][drop] {start cell over all terrains, c_eq bits}
+  int2* strk;         // [step][drop] {qt(vol sx), qt(vol sy)}: the step's momentum tracks
   int32_t* qb;        // [drop][step] height commit of the step, 2^-20 (replay)
   int qstride;        // qb's stride per drop: log rows rounded up to kReplayDepth
-  const float2* __restrict__ sr;  // {burst scale, root density} per cell (tile_count_kernel)
+  const float* __restrict__ sr;   // burst scale per cell (tile_count_kernel)
   int4* dlog;         // [drop] {steps, end kind, end cell, 0}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   int32_t* hist;      // per-terrain [tile][walk block] step counts, when the walk counts them
@@ -133,7 +134,8 @@ struct WalkArgs {
   const float* __restrict__ h;
   const float4* __restrict__ fld;  // {momentumx, momentumy, discharge, erf(0.4*discharge)}
   const float* __restrict__ root;  // nullptr when the map has no root density
-  int4* slog;                      // [step][drop] step log (count pass)
+  int2* slog;                      // [step][drop] step log (count pass): {cell, c_eq}
+  int2* strk;                      // [step][drop] its tracks
   int cell_off;                    // the terrain's first cell over all terrains
   const float4* __restrict__ pack; // packed cells, or nullptr (read h, fld, root)
   uint8_t* visit;                  // cascade sites
@@ -195,7 +197,7 @@ __device__ inline CellView load_packed(const float4* __restrict__ pk, int i) {
   v.h = a.x; v.n0 = a.y; v.n2 = a.z; v.erfd = a.w;
   v.mx = b.x; v.my = b.y; v.d = b.z;
   v.xm = v.xp = v.ym = v.yp = 0.0f;  // not read: the record holds the normal
-  v.root = 0.0f;                      // not read by the count walk (the replay's sr has it)
+  v.root = 0.0f;                      // not read by the count walk (the replay reads it)
   return v;
 }
 
@@ -263,7 +265,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   // real log overwrites one step later or, for a drop that takes no step,
   // which no reader looks at (its step count is 0)
   size_t pk = (size_t)lane_drop;
-  int4 ps = make_int4(a.cell_off + i0, 0, 0, 0);
+  int2 ps = make_int2(a.cell_off + i0, 0), pt = make_int2(0, 0);
   // the count walk reads the cycle's packed records, the trace the map arrays
   auto load_cell = [&](int x, int y, int i) {
     if constexpr (COUNT) return load_packed(a.pack, i);
@@ -295,7 +297,10 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     // off the map the record is not used; loading the current cell's keeps
     // the load unconditional, so its wait is counted exactly
     const CellView cn = ex ? (COUNT ? load_cell(ix, iy, i0) : c) : load_cell(jx, jy, j);
-    if (COUNT) a.slog[pk] = ps;
+    if (COUNT) {
+      a.slog[pk] = ps;
+      a.strk[pk] = pt;
+    }
     const float h2 = ex ? (float)((double)c.h - 0.002) : cn.h;  // water.h:120-124
     float c_eq = (1.0f + P.entrainment * c.erfd) * (c.h - h2);  // water.h:127
     if (c_eq < 0) c_eq = 0;
@@ -303,7 +308,8 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       // everything of the step that the snapshot fixes; the tracks
       // (water.h:115-117) use the volume before this step's evaporation
       pk = klog;
-      ps = make_int4(a.cell_off + i0, __float_as_int(c_eq), quant_t(vol * sx), quant_t(vol * sy));
+      ps = make_int2(a.cell_off + i0, __float_as_int(c_eq));
+      pt = make_int2(quant_t(vol * sx), quant_t(vol * sy));
       step_hook(a.cell_off + i0);
     }
     if (TRACE) {
@@ -335,7 +341,10 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     c = cn;
     ix = jx; iy = jy; i0 = j;
   }
-  if (COUNT) a.slog[pk] = ps;
+  if (COUNT) {
+    a.slog[pk] = ps;
+    a.strk[pk] = pt;
+  }
   if (TRACE) *tr_n = nrec;
   return steps;
 }
@@ -414,6 +423,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.root = T.has_root ? w.root + T.cell_off : nullptr;
       a.cell_off = (int)T.cell_off;
       a.slog = w.slog;
+      a.strk = w.strk;
       a.pack = w.pack ? w.pack + 2 * T.cell_off : nullptr;
       a.nx = T.nx;
       a.ny = T.ny;
@@ -460,6 +470,10 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
 // per SIMD is all a 65536-drop wave fills, so the ring is the latency hiding.
 constexpr int kReplayDepth = 32;
 
+// ROOT: some terrain has a root density (a terrain without one holds zeros,
+// so the same gather serves every terrain); without, each step gathers only
+// the 4-byte burst scale.
+template <bool ROOT>
 __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
   constexpr int D = kReplayDepth;
   LaneStats S;
@@ -476,7 +490,8 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
     if (dl.y == kEndSkipped) skipped += 1;
     if (dl.y <= kEndExit) {
       walked += 1;
-      const float2* __restrict__ sr = w.sr;
+      const float* __restrict__ sr = w.sr;
+      const float* __restrict__ rmap = w.root;
       const float dep = T.P.depositionRate, e = T.P.evapRate;
       float sed = 0.0f, vol = 1.0f;
       double gain = 0.0;
@@ -491,15 +506,14 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
         int cl[D];
 #pragma unroll
         for (int u = 0; u < D; ++u) {
-          const int4 l = w.slog[slot(u)];
+          const int2 l = w.slog[slot(u)];
           cl[u] = l.x;
           cq[u] = __int_as_float(l.y);
         }
 #pragma unroll
         for (int u = 0; u + 1 < D; ++u) {
-          const float2 r = sr[cl[u]];  // {burst scale, root density}
-          sc[u] = r.x;
-          rt[u] = r.y;
+          sc[u] = sr[cl[u]];  // burst scale and root density of the start cell
+          rt[u] = ROOT ? rmap[cl[u]] : 0.0f;
         }
         rt[D - 1] = 0.0f;
         sc[D - 1] = 0.0f;
@@ -526,13 +540,12 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
             }
             // slot u takes step k + D; the gather of step k + D - 1 (slot v,
             // whose row arrived one step ago) goes out now
-            const int4 l = w.slog[slot(k + D)];
+            const int2 l = w.slog[slot(k + D)];
             cl[u] = l.x;
             cq[u] = __int_as_float(l.y);
             const int v = (u + D - 1) % D;  // constant once unrolled
-            const float2 r = sr[cl[v]];
-            sc[v] = r.x;
-            rt[v] = r.y;
+            sc[v] = sr[cl[v]];
+            rt[v] = ROOT ? rmap[cl[v]] : 0.0f;
           }
 #pragma unroll
           for (int u = 0; u < D; u += 4)
@@ -664,7 +677,8 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(int32_t* v,
 // entries stay contiguous.  Order inside a tile does not matter: every
 // consumer sums integers.
 struct PartArgs {
-  const int4* __restrict__ slog;      // [slot] {start cell, c_eq, qt(vol sx), qt(vol sy)}
+  const int2* __restrict__ slog;      // [slot] {start cell, c_eq}
+  const int2* __restrict__ strk;      // [slot] {qt(vol sx), qt(vol sy)}
   const int32_t* __restrict__ steps;  // [drop] valid rows of the log
   const Terrain* __restrict__ ter;
   int nter;
@@ -750,10 +764,10 @@ __global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
   for (int s = (int)threadIdx.x; s < ns; s += blockDim.x) {
     const int k = part_slot(a, g0, width, s);
     if (k >= 0) {
-      const int4 sl = a.slog[k];
-      const int c = sl.x;
+      const int c = a.slog[k].x;
+      const int2 tk = a.strk[k];
       const int e = atomicAdd(&cur[(c >> a.tb) - col.wlo], 1);
-      a.ent[e] = make_int4(k, c, sl.z, sl.w);
+      a.ent[e] = make_int4(k, c, tk.x, tk.y);
     }
   }
 }
@@ -946,12 +960,14 @@ __global__ void __launch_bounds__(1024, 8) part_scatter_staged_kernel(PartArgs a
   // The next round's log records are loaded while this round is ranked,
   // staged and written.  The loads are unconditional (an invalid slot reads
   // record 0, unused) so that no wait inside the round covers them.
-  int kq[PER];
-  int4 lv[PER];
+  int kq[PER], lc[PER];
+  int2 lt[PER];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     kq[u] = slot(u * 1024 + tid);
-    lv[u] = a.slog[kq[u] >= 0 ? kq[u] : 0];
+    const int k = kq[u] >= 0 ? kq[u] : 0;
+    lc[u] = a.slog[k].x;
+    lt[u] = a.strk[k];
   }
   for (int r0 = 0; r0 < ns; r0 += RS) {
     for (int t = tid; t < wn; t += 1024) cnt[t] = 0;
@@ -959,13 +975,15 @@ __global__ void __launch_bounds__(1024, 8) part_scatter_staged_kernel(PartArgs a
     int tl[PER], rk[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      ev[u] = make_int4(kq[u], lv[u].x, lv[u].z, lv[u].w);
-      tl[u] = kq[u] >= 0 ? (lv[u].x >> a.tb) - wlo : -1;
+      ev[u] = make_int4(kq[u], lc[u], lt[u].x, lt[u].y);
+      tl[u] = kq[u] >= 0 ? (lc[u] >> a.tb) - wlo : -1;
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       kq[u] = slot(r0 + RS + u * 1024 + tid);
-      lv[u] = a.slog[kq[u] >= 0 ? kq[u] : 0];
+      const int k = kq[u] >= 0 ? kq[u] : 0;
+      lc[u] = a.slog[k].x;
+      lt[u] = a.strk[k];
     }
     __syncthreads();
 #pragma unroll
@@ -990,8 +1008,8 @@ __global__ void __launch_bounds__(1024, 8) part_scatter_staged_kernel(PartArgs a
 
 
 // Per tile: the walk steps starting at each cell (LDS histogram of the
-// tile's entries) -> burst scale, stored with the root density the walk
-// used as one 8-byte gather for the replay; cascade sites where more steps
+// tile's entries) -> burst scale, the replay's 4-byte gather per step;
+// cascade sites where more steps
 // start than first steps (an arrival, water.h:151); clears the first-step
 // counts.  With S > 1 each tile's entries are split over S blocks (block
 // t*S + s takes the s-th part), which write their counts to part[block] and
@@ -1012,8 +1030,8 @@ __device__ inline void tile_part(const int32_t* __restrict__ start, int t, int s
   }
 }
 
-// burst scale, root density and cascade site of cell i, k steps starting there
-__device__ inline void cell_count_finish(size_t i, int k, float2* sr, int32_t* first,
+// burst scale and cascade site of cell i, k steps starting there
+__device__ inline void cell_count_finish(size_t i, int k, float* sr, int32_t* first,
                                          uint8_t* visit, const float4* __restrict__ fld,
                                          const float* __restrict__ root,
                                          const Terrain* __restrict__ ter, int nter) {
@@ -1022,7 +1040,7 @@ __device__ inline void cell_count_finish(size_t i, int k, float2* sr, int32_t* f
   const float rd = ter[t].has_root ? root[i] : 0.0f;
   float effD = P.depositionRate * (1.0f - rd);
   if (effD < 0) effD = 0;
-  sr[i] = make_float2(burst_scale(k, effD, P.entrainment, fld[i].w), rd);
+  sr[i] = burst_scale(k, effD, P.entrainment, fld[i].w);
   const int f = first[i];
   if (k > f) visit[i] = 1;
   if (f) first[i] = 0;
@@ -1031,7 +1049,7 @@ __device__ inline void cell_count_finish(size_t i, int k, float2* sr, int32_t* f
 template <int S>
 __global__ void __launch_bounds__(1024) tile_count_kernel(
     const int32_t* __restrict__ start, const int4* __restrict__ ent, size_t n,
-    float2* sr, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
+    float* sr, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
     const float* __restrict__ root, const Terrain* __restrict__ ter, int nter, int32_t* part) {
   __shared__ int cnt[kTileCells];
   for (int j = threadIdx.x; j < kTileCells; j += blockDim.x) cnt[j] = 0;
@@ -1066,7 +1084,7 @@ __global__ void __launch_bounds__(1024) tile_count_kernel(
 
 // the S parts' counts of every cell (fixed order), then as tile_count_kernel
 template <int S>
-__global__ void tile_count_finish_kernel(const int32_t* __restrict__ part, size_t n, float2* sr,
+__global__ void tile_count_finish_kernel(const int32_t* __restrict__ part, size_t n, float* sr,
                                          int32_t* first, uint8_t* visit,
                                          const float4* __restrict__ fld,
                                          const float* __restrict__ root,
