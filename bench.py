@@ -13,9 +13,9 @@ particle-step:
   count walk (walk_kernel)  B_COUNT = 45: reads {h, normal x, normal z,
       erf(0.4 d), mx, my, d} of the step's cell = 28 B; writes the 16-B step
       record {start cell, c_eq, tracks}; cascade-site byte 1 B
-  commit replay (replay_kernel)  B_REPLAY = 20: reads the start cell and c_eq
-      of the step record 8 B and the cell's {burst scale, root density} 8 B;
-      writes the height commit 4 B
+  commit replay (replay_kernel)  B_REPLAY = 16: reads the start cell and c_eq
+      of the step 8 B and the cell's burst scale 4 B (and its root density,
+      4 B more, when a terrain has one); writes the height commit 4 B
 ``traffic`` is the HBM bytes per launch of that kernel measured by rocprofv3
 PMC runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy-kernel calibration) on the
 same command, read from profiles/hbm_traffic.json when it holds an entry for
@@ -40,7 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 B_COUNT = 45          # algorithmic bytes per particle-step, count walk
-B_REPLAY = 20         # algorithmic bytes per particle-step, commit replay
+B_REPLAY = 16         # algorithmic bytes per particle-step, commit replay
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNELS = (("walk_kernel", "ms_count_walk", B_COUNT), ("replay_kernel", "ms_replay", B_REPLAY))
 

@@ -246,6 +246,9 @@ size_t wave_steps(const shy_ctx* c) {
 constexpr int kPartMaxBlocks = 512;
 constexpr int kTileSplit = 4;             // blocks per tile in the tile kernels
 constexpr int kTileSplitMaxTiles = 2048;
+// partition blocks up to which each takes a CU's whole LDS (the scatter's
+// rounds of 8192 entries): no more blocks than CUs
+constexpr int kBigRoundBlocks = 256;
 
 // The partition geometry: per terrain its tiles, its blocks of dwidth drops
 // (dwidth = walk_block when the walk counts, else enough drops for at most
@@ -526,7 +529,9 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     scan_sums_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, nh, c->hbsum);
     scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
     scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);
-    if (c->part_window <= 1024)
+    if (c->part_window <= 1024 && nb <= kBigRoundBlocks)
+      part_scatter_staged_kernel<1024, 8192><<<nb, 1024, 0, c->stream>>>(pa);
+    else if (c->part_window <= 1024)
       part_scatter_staged_kernel<1024, 4096><<<nb, 1024, 0, c->stream>>>(pa);
     else if (c->part_window <= 4096)
       part_scatter_staged_kernel<4096, 4096><<<nb, 1024, 0, c->stream>>>(pa);

@@ -927,8 +927,9 @@ __global__ void __launch_bounds__(1024) order_kernel(const Terrain* __restrict__
 // entries of one tile go to consecutive addresses.  LDS holds the block's
 // tile window (at most MAXT tiles, which the host checks) and RS entries.
 template <int MAXT, int RS>
-// (two blocks of 1024 threads per CU: at most 64 VGPRs)
-__global__ void __launch_bounds__(1024, 8) part_scatter_staged_kernel(PartArgs a) {
+// (rounds of 4096: two blocks of 1024 threads per CU, at most 64 VGPRs;
+// rounds of 8192 fill the LDS of a CU with one block)
+__global__ void __launch_bounds__(1024, RS <= 4096 ? 8 : 4) part_scatter_staged_kernel(PartArgs a) {
   constexpr int PER = RS / 1024;
   // drops whose step counts the block keeps in LDS: with 1024 tiles and 4096
   // staged entries the block stays within 80 KB, two blocks per CU
