@@ -20,10 +20,12 @@ def bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-def run_pair(shy, orc, n, seed, plan, params=None):
+def run_pair(shy, orc, n, seed, plan, params=None, walk_block=None):
     """plan: list of (drops, oracle params or None to keep) per cycle."""
     P = params if params is not None else orc.Params.default()
     w = shy.World(n, n, seed=seed)
+    if walk_block:
+        w.set_walk_block(walk_block)
     w.generate_fbm(seed, 8)
     h = orc.fbm(seed, n, n)
     assert np.array_equal(bits(w.field("height")), bits(h))
@@ -138,3 +140,9 @@ def test_drop_order_bit_identical(shy):
         e.close()
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_partition_more_blocks_than_cus(shy, orc):
+    # walk blocks of 64: 20480 drops = 320 partition blocks, more than the CUs,
+    # so the scatter runs rounds of 4096 (two blocks per CU) instead of 8192
+    run_pair(shy, orc, 512, 6, [(20480, None), (20480, None)], walk_block=64)
