@@ -99,6 +99,7 @@ struct CellView {
   float mx, my, d, erfd;    // momentum, discharge, erf(0.4 d)
   float root;               // rootdensity
   float n0, n2;             // the cell's normal (x, z), when read from a packed record
+  float rf;                 // 1/|(mx, my)| when |(mx, my)| > 0, from a packed record
 };
 
 // quad::_normal (cellpool.h:181-204) from a loaded stencil.  Plane k is
@@ -136,15 +137,21 @@ __device__ inline void normal_from(const CellView& c, int x, int y, int nx, int 
   }
 }
 
+// 1/|f| of the momentum field, as the coupling below normalises it
+__device__ inline float inv_len(float fx, float fy) {
+  return 1.0f / __builtin_sqrtf(fx * fx + fy * fy);
+}
+
 // water.h:95-111: gravity, momentum coupling, fixed step of length sqrt(2).
-__device__ inline void step_motion(const shy_params& P, float n0, float n2, float vol, float d,
-                                   float fx, float fy, float& sx, float& sy, float& px,
-                                   float& py) {
+// rf = inv_len(fx, fy), used only when |f| > 0 (the count walk reads it from
+// the cell's packed record; the other callers compute it).
+__device__ inline void step_motion_rf(const shy_params& P, float n0, float n2, float vol,
+                                      float d, float fx, float fy, float rf, float& sx,
+                                      float& sy, float& px, float& py) {
   const float k = kLod * P.gravity;
   sx = sx + (k * n0) / vol;
   sy = sy + (k * n2) / vol;
   if (__builtin_sqrtf(fx * fx + fy * fy) > 0.0f && __builtin_sqrtf(sx * sx + sy * sy) > 0.0f) {
-    float rf = 1.0f / __builtin_sqrtf(fx * fx + fy * fy);
     float rs = 1.0f / __builtin_sqrtf(sx * sx + sy * sy);
     float dt = (fx * rf) * (sx * rs) + (fy * rf) * (sy * rs);
     float c = ((kLod * P.momentumTransfer) * dt) / (vol + d);
@@ -159,6 +166,12 @@ __device__ inline void step_motion(const shy_params& P, float n0, float n2, floa
   }
   px = px + sx;
   py = py + sy;
+}
+
+__device__ inline void step_motion(const shy_params& P, float n0, float n2, float vol, float d,
+                                   float fx, float fy, float& sx, float& sy, float& px,
+                                   float& py) {
+  step_motion_rf(P, n0, n2, vol, d, fx, fy, inv_len(fx, fy), sx, sy, px, py);
 }
 
 // water.h:135-136, evaluated in double as the reference's 1.0-evapRate is.

@@ -195,7 +195,7 @@ __device__ inline CellView load_packed(const float4* __restrict__ pk, int i) {
   const float4 a = pk[2 * (size_t)i], b = pk[2 * (size_t)i + 1];
   CellView v;
   v.h = a.x; v.n0 = a.y; v.n2 = a.z; v.erfd = a.w;
-  v.mx = b.x; v.my = b.y; v.d = b.z;
+  v.mx = b.x; v.my = b.y; v.d = b.z; v.rf = b.w;
   v.xm = v.xp = v.ym = v.yp = 0.0f;  // not read: the record holds the normal
   v.root = 0.0f;                      // not read by the count walk (the replay reads it)
   return v;
@@ -290,7 +290,10 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     ++steps;
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
-    step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);  // water.h:95-111
+    if constexpr (COUNT)  // water.h:95-111
+      step_motion_rf(P, n0, n2, vol, c.d, c.mx, c.my, c.rf, sx, sy, px, py);
+    else
+      step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);
     const int jx = (int)px, jy = (int)py;
     const bool ex = off_map(jx, jy, nx, ny);
     const int j = jx * ny + jy;
@@ -351,10 +354,11 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
 
 // The count walk's 32-byte per-cell record for this cycle (load_packed),
 // every cell of every terrain: {h, normal x, normal z, erf(0.4 d)},
-// {momentum x, momentum y, discharge, 0}.  The normal (quad::_normal,
-// cellpool.h:181-204) depends only on the cell's 5-height stencil of the
-// snapshot, so it is computed here once per cell, with the walk's own
-// arithmetic, instead of at every step: a shorter chain per step.
+// {momentum x, momentum y, discharge, 1/|momentum|}.  The normal
+// (quad::_normal, cellpool.h:181-204) and the momentum's inverse length
+// (water.h:101-104) depend only on the cell's snapshot, so they are computed
+// here once per cell, with the walk's own arithmetic, instead of at every
+// step: a shorter chain per step.
 __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
                             const float* __restrict__ h, const float4* __restrict__ fld,
                             float4* pack) {
@@ -373,8 +377,10 @@ __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
   c.h = hc; c.xm = xm; c.xp = xp; c.ym = ym; c.yp = yp;
   float n0, n1, n2;
   normal_from(c, x, y, T.nx, ny, n0, n1, n2);  // water.h:70, once per cell instead of per step
+  // 1/|momentum| for the coupling term (read only where |momentum| > 0)
+  const float rf = f.x * f.x + f.y * f.y > 0.0f ? inv_len(f.x, f.y) : 0.0f;
   pack[2 * g] = make_float4(hc, n0, n2, f.w);
-  pack[2 * g + 1] = make_float4(f.x, f.y, f.z, 0.0f);
+  pack[2 * g + 1] = make_float4(f.x, f.y, f.z, rf);
 }
 
 // Count pass over every drop of every terrain; grid-stride, so the grid may
