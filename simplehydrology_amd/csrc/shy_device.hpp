@@ -178,6 +178,19 @@ __device__ inline void step_motion(const shy_params& P, float n0, float n2, floa
 __device__ inline float evap_sed(float sed, float e) {
   return (float)((double)sed / (1.0 - (double)e));
 }
+
+// The same quotient with the divisor's reciprocal given: c = 1 - e and
+// y = 1.0 / c (correctly rounded).  Markstein's theorem: with y within half
+// an ulp of 1/c and q = RN(a y) within one ulp of a/c, the remainder
+// r = a - c q is exact in one FMA and RN(q + r y) is the correctly rounded
+// a/c (no overflow or underflow: a is a float, c is in (0, 1]).  Three
+// dependent f64 operations instead of the division's sequence.
+__device__ inline float evap_sed_rcp(float sed, double c, double y) {
+  const double a = (double)sed;
+  const double q = a * y;
+  const double r = __builtin_fma(-q, c, a);
+  return (float)__builtin_fma(r, y, q);
+}
 __device__ inline float evap_vol(float vol, float e) {
   return (float)((double)vol * (1.0 - (double)e));
 }

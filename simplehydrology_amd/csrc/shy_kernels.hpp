@@ -499,6 +499,7 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
       const float* __restrict__ sr = w.sr;
       const float* __restrict__ rmap = w.root;
       const float dep = T.P.depositionRate, e = T.P.evapRate;
+      const double ce = 1.0 - (double)e, ye = 1.0 / ce;  // evaporation divisor, its reciprocal
       float sed = 0.0f, vol = 1.0f;
       double gain = 0.0;
       long long wdh = 0;
@@ -539,7 +540,7 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
               qv[u] = -q;
               wdh -= q;
               sed += (float)q * kHQInv;
-              const float s1 = evap_sed(sed, e);
+              const float s1 = evap_sed_rcp(sed, ce, ye);  // = evap_sed(sed, e)
               gain += (double)s1 - (double)sed;
               sed = s1;
               vol = evap_vol(vol, e);
