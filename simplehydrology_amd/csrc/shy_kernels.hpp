@@ -471,17 +471,21 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
 // drop (water.h:74-82, 139-142).  -q is written drop-major, four steps per
 // 16-byte store: a drop's consecutive steps often start in one cell tile, so
 // tile_reduce_kernel's gathers of them share lines.
-// Steps of the log in flight per lane: the log rows of step k+D and the
-// per-cell gathers of step k+D-1 are issued while step k computes.  One wave
-// per SIMD is all a 65536-drop wave fills, so the ring is the latency hiding.
+// Steps of the log in flight per lane: while step k computes, the log row
+// of step k+D is loaded and the per-cell gather of step k+D-L is issued with
+// the cell of a row loaded L steps ago, so no step waits on a load issued
+// shortly before it.  One wave per SIMD is all a 65536-drop wave fills, so
+// the ring is the latency hiding.
 constexpr int kReplayDepth = 32;
+constexpr int kReplayLag = 16;
 
 // ROOT: some terrain has a root density (a terrain without one holds zeros,
 // so the same gather serves every terrain); without, each step gathers only
 // the 4-byte burst scale.
 template <bool ROOT>
 __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
-  constexpr int D = kReplayDepth;
+  constexpr int D = kReplayDepth, L = kReplayLag;
+  static_assert(L >= 1 && L < D, "gathers run D - L steps ahead of use");
   LaneStats S;
   long long walked = 0, skipped = 0;
   const size_t stride = (size_t)w.total_drops;
@@ -518,12 +522,15 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
           cq[u] = __int_as_float(l.y);
         }
 #pragma unroll
-        for (int u = 0; u + 1 < D; ++u) {
-          sc[u] = sr[cl[u]];  // burst scale and root density of the start cell
-          rt[u] = ROOT ? rmap[cl[u]] : 0.0f;
+        for (int u = 0; u < D; ++u) {
+          if (u < D - L) {
+            sc[u] = sr[cl[u]];  // burst scale and root density of the start cell
+            rt[u] = ROOT ? rmap[cl[u]] : 0.0f;
+          } else {
+            sc[u] = 0.0f;  // gathered in the loop's first L steps
+            rt[u] = 0.0f;
+          }
         }
-        rt[D - 1] = 0.0f;
-        sc[D - 1] = 0.0f;
         static_assert(D % 4 == 0, "the ring stores its commits as int4s");
         int4* qdst = reinterpret_cast<int4*>(w.qb + (size_t)g * w.qstride);
         for (int k0 = 0; k0 < n; k0 += D) {
@@ -545,12 +552,12 @@ __global__ void __launch_bounds__(256) replay_kernel(WaveArgs w) {
               sed = s1;
               vol = evap_vol(vol, e);
             }
-            // slot u takes step k + D; the gather of step k + D - 1 (slot v,
-            // whose row arrived one step ago) goes out now
+            // slot u takes step k + D; the gather of step k + D - L (slot v,
+            // whose row was loaded L steps ago) goes out now
             const int2 l = w.slog[slot(k + D)];
             cl[u] = l.x;
             cq[u] = __int_as_float(l.y);
-            const int v = (u + D - 1) % D;  // constant once unrolled
+            const int v = (u + D - L) % D;  // constant once unrolled
             sc[v] = sr[cl[v]];
             rt[v] = ROOT ? rmap[cl[v]] : 0.0f;
           }
