@@ -3,6 +3,9 @@
 // One cycle (= one World::erode call, source/world.h:54-88) is one wave of
 // drops that all read the map as it stood when the cycle began:
 //
+//  pack_kernel           one 32-byte record per cell of what a walk step reads
+//                        there, the cell's normal precomputed
+//  order_kernel          (many drops) which drop each walk thread takes
 //  walk_kernel           one drop per lane walks its trajectory; for every
 //                        step it logs, coalesced as [step][drop], 16 bytes
 //                        {start cell, c_eq} + {tracks} of what the
@@ -14,11 +17,14 @@
 //                        per cell: a cell is a site iff more steps start there
 //                        than first steps, or a walk ended there.
 //  part_*_kernel, scan   group the valid log slots by cell tile (2048 cells):
-//                        per-block LDS histograms over tiles, an exclusive
-//                        scan of the [tile][block] matrix, then an LDS-cursor
-//                        scatter of {slot, cell, tracks} into tile order.  No
-//                        global atomic per step: the device's atomic rate, not
-//                        its bandwidth, bounded a walk that took one.
+//                        per-block LDS histograms over the tiles of the
+//                        block's terrain (built by the walk itself where its
+//                        blocks are the partition blocks), one exclusive scan
+//                        of every terrain's [tile][block] matrix, then an
+//                        LDS-staged scatter of {slot, cell, tracks} into tile
+//                        order.  No global atomic per step: the device's
+//                        atomic rate, not its bandwidth, bounded a walk that
+//                        took one.
 //  tile_count_kernel     per tile, the steps starting at each cell (LDS) ->
 //                        cascade sites and the burst scale
 //                        s = (1-(1-gamma)^k)/(gamma k), gamma = min(1, effD (1 +
@@ -29,9 +35,9 @@
 //                        overshoot.
 //  replay_kernel         replays each drop's logged steps: the only chain left
 //                        is the sediment recurrence (arithmetic); the log loads
-//                        stream and the per-cell gathers (burst scale, root) do
-//                        not depend on the previous step, so they are issued
-//                        ahead.  Each step's erosion/deposition effD*cdiff*s is
+//                        stream and the per-cell gathers (burst scale, root
+//                        density where a terrain has one) do not depend on the
+//                        previous step, so they are issued ahead.  Each step's erosion/deposition effD*cdiff*s is
 //                        quantised once to int32 (2^-20, moved between map and
 //                        drop as that integer) and stored drop-major.
 //  tile_reduce_kernel    per tile, sums in LDS the height commits (gathered by
@@ -89,7 +95,7 @@ struct Terrain {
 // terrains.
 constexpr int kTileBits = 11;             // cells per tile = 2048
 constexpr int kTileCells = 1 << kTileBits;
-constexpr int kFuseMaxTiles = 4096;       // tile window of a walk block that counts in LDS
+constexpr int kFuseMaxTiles = 4096;       // most tiles of a terrain whose walk blocks count in LDS
 
 struct WaveArgs {
   const Terrain* __restrict__ ter;
@@ -765,7 +771,7 @@ __global__ void __launch_bounds__(1024) part_hist_kernel(PartArgs a) {
     a.hist[col.base + (long long)r * col.stride] = h[r];
 }
 
-// Scatter without staging (any tile window): one 16-byte entry per step at
+// Scatter without staging (terrains of any size): one 16-byte entry per step at
 // its tile's LDS cursor.
 template <int MAXT>
 __global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
@@ -939,7 +945,7 @@ __global__ void __launch_bounds__(1024) order_kernel(const Terrain* __restrict__
 // The scatter with its writes grouped: each round of RS slots is ranked per
 // tile in LDS, staged in tile order and written out so that the round's
 // entries of one tile go to consecutive addresses.  LDS holds the block's
-// tile window (at most MAXT tiles, which the host checks) and RS entries.
+// terrain's tiles (at most MAXT, which the host checks) and RS entries.
 template <int MAXT, int RS>
 // (rounds of 4096: two blocks of 1024 threads per CU, at most 64 VGPRs;
 // rounds of 8192 fill the LDS of a CU with one block)
@@ -948,7 +954,7 @@ __global__ void __launch_bounds__(1024, RS <= 4096 ? 8 : 4) part_scatter_staged_
   // drops whose step counts the block keeps in LDS: with 1024 tiles and 4096
   // staged entries the block stays within 80 KB, two blocks per CU
   constexpr int kMaxW = (MAXT <= 1024 && RS <= 4096) ? 1008 : 4096;
-  __shared__ int cur[MAXT];  // next entry of each tile of the window
+  __shared__ int cur[MAXT];  // next entry of each tile of the block's terrain
   __shared__ int cnt[MAXT];  // the round's entries per tile
   __shared__ int off[MAXT];  // their exclusive offsets in the stage
   __shared__ int4 stage[RS];
