@@ -497,10 +497,13 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   if (walk) {
+    // the record prefetch pays where every drop runs at once (the walk is
+    // bound by its chain's latency): the same waves the drop order leaves off
+    const bool pf = !ordered(c);
     if (c->walk_counts)
-      walk_kernel<true><<<grid, bs, 0, c->stream>>>(w);
+      (pf ? walk_kernel<true, true> : walk_kernel<true, false>)<<<grid, bs, 0, c->stream>>>(w);
     else
-      walk_kernel<false><<<grid, bs, 0, c->stream>>>(w);
+      (pf ? walk_kernel<false, true> : walk_kernel<false, false>)<<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));

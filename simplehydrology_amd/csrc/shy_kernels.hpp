@@ -110,7 +110,7 @@ struct WaveArgs {
   int32_t* qb;        // [drop][step] height commit of the step, 2^-20 (replay)
   int qstride;        // qb's stride per drop: log rows rounded up to kReplayDepth
   const float* __restrict__ sr;   // burst scale per cell (tile_count_kernel)
-  int4* dlog;         // [drop] {steps, end kind, end cell, 0}
+  int4* dlog;         // [drop] {steps, end kind, end cell, unused}
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   int32_t* hist;      // per-terrain [tile][walk block] step counts, when the walk counts them
   const int32_t* __restrict__ perm;  // [drop] the terrain's drop a thread walks (order_kernel), or null
@@ -250,10 +250,10 @@ struct NoStepHook {
   __device__ void operator()(int) const {}
 };
 
-template <int MODE, class Hook = NoStepHook>
+template <int MODE, class Hook = NoStepHook, bool PREFETCH = false>
 __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, int* end,
                          int* end_cell, shy_drop_step* tr, int max_tr, int* tr_n,
-                         Hook step_hook = Hook()) {
+                         Hook step_hook = Hook(), uint32_t* pf_sink = nullptr) {
   constexpr bool TRACE = MODE == kTrace;
   constexpr bool COUNT = MODE == kCount;
   const shy_params& P = a.P;
@@ -278,7 +278,25 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     else return load_unpacked(a, x, y, i);
   };
   CellView c = load_cell(ix, iy, i0);
+  // PREFETCH (count walk): at the top of each step, a load of the record of
+  // the cell the drop would reach at its current speed.  When the new speed
+  // keeps the drop on that cell, the step's real record load, issued after
+  // the motion update, finds its line already on the way: a shorter step
+  // where the walk is bound by its chain's latency, extra traffic where it is
+  // bound by bandwidth (waves of many more drops than the GPU runs at once).
+  // The prefetched word is folded into pf_sink one step later, after the
+  // wait for a load issued after it, so it never adds a wait, and it keeps
+  // the load alive.
+  uint32_t pf_acc = 0;
+  float pf = 0.0f;
   for (;;) {
+    if constexpr (COUNT && PREFETCH) {
+      pf_acc ^= __float_as_uint(pf);
+      int qx = (int)(px + sx), qy = (int)(py + sy);
+      qx = qx < 0 ? 0 : (qx >= nx ? nx - 1 : qx);
+      qy = qy < 0 ? 0 : (qy >= ny ? ny - 1 : qy);
+      pf = a.pack[2 * (size_t)(qx * ny + qy)].x;
+    }
     float n0, n1, n2;
     if constexpr (COUNT) {  // the packed record holds the cell's normal (water.h:70)
       n0 = c.n0;
@@ -353,6 +371,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   if (COUNT) {
     a.slog[pk] = ps;
     a.strk[pk] = pt;
+    if (pf_sink) *pf_sink = pf_acc;
   }
   if (TRACE) *tr_n = nrec;
   return steps;
@@ -395,7 +414,7 @@ __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
 // thread, no grid stride): the block counts its steps per tile of its
 // terrain in LDS and writes its column of the terrain's [tile][block]
 // matrix, which part_hist_kernel would otherwise build from the log.
-template <bool COUNT_TILES>
+template <bool COUNT_TILES, bool PREFETCH>
 __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
   __shared__ int th[COUNT_TILES ? kFuseMaxTiles : 1];
   int wlo = 0, wn = 0;
@@ -447,9 +466,11 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       dl.y = kEndSkipped;
       if ((double)a.h[cid] >= 0.1) {  // world.h:71-72
         int end = 0, end_cell = 0;
-        dl.x = walk_drop<kCount>(a, g, (float)(cid / (uint32_t)T.ny),
+        uint32_t sink = 0;
+        dl.x = walk_drop<kCount, decltype(count_step), PREFETCH>(a, g, (float)(cid / (uint32_t)T.ny),
                                  (float)(cid % (uint32_t)T.ny), &end, &end_cell, nullptr, 0,
-                                 nullptr, count_step);
+                                 nullptr, count_step, &sink);
+        dl.w = (int)sink;  // not read: it keeps the record prefetches
         dl.y = end;
         dl.z = end_cell;
         if (dl.x > 0) {
