@@ -284,14 +284,17 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   // the motion update, finds its line already on the way: a shorter step
   // where the walk is bound by its chain's latency, extra traffic where it is
   // bound by bandwidth (waves of many more drops than the GPU runs at once).
-  // The prefetched word is folded into pf_sink one step later, after the
-  // wait for a load issued after it, so it never adds a wait, and it keeps
-  // the load alive.
+  // The prefetched word is folded into pf_sink at the end of the step, after
+  // the wait for the step's record load (issued after it), so it never adds
+  // a wait, and it keeps the load alive.
   uint32_t pf_acc = 0;
   float pf = 0.0f;
+  // The first record is waited for here, before the loop: left pending, it
+  // makes the wait at the top of the loop count the previous step's log
+  // stores as well, on every step.
+  if constexpr (COUNT) __builtin_amdgcn_s_waitcnt(0);
   for (;;) {
     if constexpr (COUNT && PREFETCH) {
-      pf_acc ^= __float_as_uint(pf);
       int qx = (int)(px + sx), qy = (int)(py + sy);
       qx = qx < 0 ? 0 : (qx >= nx ? nx - 1 : qx);
       qy = qy < 0 ? 0 : (qy >= ny ? ny - 1 : qy);
@@ -367,6 +370,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     ++age;
     c = cn;
     ix = jx; iy = jy; i0 = j;
+    if constexpr (COUNT && PREFETCH) pf_acc ^= __float_as_uint(pf);
   }
   if (COUNT) {
     a.slog[pk] = ps;
