@@ -4,7 +4,7 @@ No reference counterpart: the reference converges the exponentially averaged
 discharge/momentum fields (source/world.h:80-86) only by running the particle
 method.  This module predicts the converged fields from a terrain so the
 particle method can start near its fixed point.  The particle method stays
-the fallback and the judge (``train_surrogate.warm_start``).
+the fallback and the arbiter (``train_surrogate.warm_start``).
 
 The network is a 4-level residual U-Net, 3x3 convolutions,
 32-64-96-128 channels, BF16 weights and activations with FP32 accumulation.

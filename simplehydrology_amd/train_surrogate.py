@@ -20,7 +20,7 @@ with a 32-cell halo, import of the predicted fields, then ``K`` plain
 verification cycles.  The gate compares the mean of erf(0.4 d) over the last
 ``W`` verification cycles with the prediction and rejects the warm start
 (keeping the plain run's state) when their relative L1 distance exceeds
-``gate_l1``.  The particle method stays the judge and the fallback.
+``gate_l1``.  The particle method stays the arbiter and the fallback.
 
 Evaluation (:func:`evaluate`): relative L1 error of trailing ``W``-cycle
 mean fields against the mean over ``[ref0, ref1)`` of a long cold run, for
