@@ -142,15 +142,26 @@ __device__ inline float inv_len(float fx, float fy) {
   return 1.0f / __builtin_sqrtf(fx * fx + fy * fy);
 }
 
+// a / b, correctly rounded, given y = 1.0f / b (correctly rounded): q = a y is
+// within an ulp of a/b, the remainder r = a - b q is exact in one FMA, and
+// RN(q + r y) is the correctly rounded quotient (Markstein's theorem; no
+// overflow or underflow for the operands here: |a| is 0 or above 1e-30 and
+// b in (0, 1]).  r = 0 means q is exact, which keeps the sign of a zero.
+__device__ inline float div_rcp(float a, float b, float y) {
+  const float q = a * y;
+  const float r = __builtin_fmaf(-q, b, a);
+  return r == 0.0f ? q : __builtin_fmaf(r, y, q);
+}
+
 // water.h:95-111: gravity, momentum coupling, fixed step of length sqrt(2).
 // rf = inv_len(fx, fy), used only when |f| > 0 (the count walk reads it from
-// the cell's packed record; the other callers compute it).
+// the cell's packed record; the other callers compute it); yv = 1.0f / vol.
 __device__ inline void step_motion_rf(const shy_params& P, float n0, float n2, float vol,
-                                      float d, float fx, float fy, float rf, float& sx,
-                                      float& sy, float& px, float& py) {
+                                      float yv, float d, float fx, float fy, float rf,
+                                      float& sx, float& sy, float& px, float& py) {
   const float k = kLod * P.gravity;
-  sx = sx + (k * n0) / vol;
-  sy = sy + (k * n2) / vol;
+  sx = sx + div_rcp(k * n0, vol, yv);  // (k n0) / vol
+  sy = sy + div_rcp(k * n2, vol, yv);
   if (__builtin_sqrtf(fx * fx + fy * fy) > 0.0f && __builtin_sqrtf(sx * sx + sy * sy) > 0.0f) {
     float rs = 1.0f / __builtin_sqrtf(sx * sx + sy * sy);
     float dt = (fx * rf) * (sx * rs) + (fy * rf) * (sy * rs);
@@ -171,7 +182,7 @@ __device__ inline void step_motion_rf(const shy_params& P, float n0, float n2, f
 __device__ inline void step_motion(const shy_params& P, float n0, float n2, float vol, float d,
                                    float fx, float fy, float& sx, float& sy, float& px,
                                    float& py) {
-  step_motion_rf(P, n0, n2, vol, d, fx, fy, inv_len(fx, fy), sx, sy, px, py);
+  step_motion_rf(P, n0, n2, vol, 1.0f / vol, d, fx, fy, inv_len(fx, fy), sx, sy, px, py);
 }
 
 // water.h:135-136, evaluated in double as the reference's 1.0-evapRate is.

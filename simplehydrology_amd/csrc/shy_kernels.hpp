@@ -289,6 +289,9 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   // a wait, and it keeps the load alive.
   uint32_t pf_acc = 0;
   float pf = 0.0f;
+  // 1 / vol for the gravity term, computed a step ahead, off the chain that
+  // leads from a record to the next one
+  float yv = 1.0f / vol;
   // The first record is waited for here, before the loop: left pending, it
   // makes the wait at the top of the loop count the previous step's log
   // stores as well, on every step.
@@ -318,7 +321,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
     if constexpr (COUNT)  // water.h:95-111
-      step_motion_rf(P, n0, n2, vol, c.d, c.mx, c.my, c.rf, sx, sy, px, py);
+      step_motion_rf(P, n0, n2, vol, yv, c.d, c.mx, c.my, c.rf, sx, sy, px, py);
     else
       step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);
     const int jx = (int)px, jy = (int)py;
@@ -362,6 +365,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       }
     }
     vol = evap_vol(vol, P.evapRate);  // water.h:136
+    if (COUNT) yv = 1.0f / vol;
     if (ex) {  // water.h:139-142
       *end = kEndExit;
       *end_cell = i0;
