@@ -221,7 +221,7 @@ int layout(shy_ctx* c) {
       return fail(c, SHY_ERR_ARG, "ensemble too large: drops and cascade tiles must stay below 2^31");
   }
   if ((cells + kTileCells - 1) / kTileCells > kPartMaxTiles)
-    return fail(c, SHY_ERR_ARG, "context too large: at most 2^26 cells over all terrains");
+    return fail(c, SHY_ERR_ARG, "context too large: at most 2^27 cells over all terrains");
   c->total_drops = (int)drops;
   c->total_tiles = (int)tiles;
   c->ntiles = (int)((cells + kTileCells - 1) / kTileCells);
@@ -364,7 +364,7 @@ int ensure_wave(shy_ctx* c) {
   }
   {
     // split each tile's entries over kTileSplit blocks while the per-part
-    // buffers stay modest (up to 2048 tiles: 512 MiB of partial sums)
+    // buffers stay modest (up to 2048 tiles: 1 GiB of partial sums)
     const int split = c->ntiles <= kTileSplitMaxTiles ? kTileSplit : 1;
     if (split != c->tile_split || (split > 1 && !c->tcount)) {
       drop_graph(c);

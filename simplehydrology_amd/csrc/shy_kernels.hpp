@@ -16,7 +16,7 @@
 //                        cascade sites (arrival cells, water.h:151) follow
 //                        per cell: a cell is a site iff more steps start there
 //                        than first steps, or a walk ended there.
-//  part_*_kernel, scan   group the valid log slots by cell tile (2048 cells):
+//  part_*_kernel, scan   group the valid log slots by cell tile (4096 cells):
 //                        per-block LDS histograms over the tiles of the
 //                        block's terrain (built by the walk itself where its
 //                        blocks are the partition blocks), one exclusive scan
@@ -93,7 +93,7 @@ struct Terrain {
 // Whole-ensemble arguments of the walk kernels (concatenated arrays).
 // The partition's cell tiles: 2^kTileBits consecutive cell indices over all
 // terrains.
-constexpr int kTileBits = 11;             // cells per tile = 2048
+constexpr int kTileBits = 12;             // cells per tile = 4096
 constexpr int kTileCells = 1 << kTileBits;
 constexpr int kFuseMaxTiles = 4096;       // most tiles of a terrain whose walk blocks count in LDS
 

@@ -1,9 +1,9 @@
 """The step-log partition's code paths, bit-exact against the oracle.
 
-The cycle groups its step log by 2048-cell tile.  Which scatter runs depends
-on the most tiles of one terrain: up to 1024 tiles (maps up to 2^21 cells;
-every other GPU test), up to 4096 tiles (2048x2048 here) and the unstaged
-scatter beyond that (4096x4096 here).  Also: a cycle without drops (empty
+The cycle groups its step log by 4096-cell tile.  Which scatter runs depends
+on the most tiles of one terrain: up to 1024 tiles (maps up to 2^22 cells;
+every other GPU test), up to 4096 tiles (4096x4096 here) and the unstaged
+scatter beyond that (4096x8192 here).  Also: a cycle without drops (empty
 partition, the tile reduce still writes zero commits and tracks), a
 parameter change between cycles (the per-terrain volume-track table is
 rebuilt), and ensembles whose terrains share tiles, fit inside one tile or
@@ -20,14 +20,16 @@ def bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-def run_pair(shy, orc, n, seed, plan, params=None, walk_block=None):
-    """plan: list of (drops, oracle params or None to keep) per cycle."""
+def run_pair(shy, orc, n, seed, plan, params=None, walk_block=None, ny=None):
+    """plan: list of (drops, oracle params or None to keep) per cycle; an
+    n x ny terrain (ny defaults to n)."""
     P = params if params is not None else orc.Params.default()
-    w = shy.World(n, n, seed=seed)
+    ny = n if ny is None else ny
+    w = shy.World(n, ny, seed=seed)
     if walk_block:
         w.set_walk_block(walk_block)
     w.generate_fbm(seed, 8)
-    h = orc.fbm(seed, n, n)
+    h = orc.fbm(seed, n, ny)
     assert np.array_equal(bits(w.field("height")), bits(h))
     z = np.zeros_like(h)
     d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
@@ -49,13 +51,13 @@ def run_pair(shy, orc, n, seed, plan, params=None, walk_block=None):
 
 
 def test_partition_window_4096_tiles(shy, orc):
-    # 2048x2048 = 2048 tiles in one block's window: the 4096-tile staged scatter
-    run_pair(shy, orc, 2048, 3, [(16384, None), (16384, None)])
+    # 4096x4096 = 4096 tiles in one terrain: the 4096-tile staged scatter
+    run_pair(shy, orc, 4096, 3, [(8192, None), (8192, None)])
 
 
 def test_partition_unstaged_scatter(shy, orc):
-    # 4096x4096 = 8192 tiles: beyond LDS staging, the plain LDS-cursor scatter
-    run_pair(shy, orc, 4096, 4, [(8192, None), (8192, None)])
+    # 4096x8192 = 8192 tiles: beyond LDS staging, the plain LDS-cursor scatter
+    run_pair(shy, orc, 4096, 4, [(8192, None), (8192, None)], ny=8192)
 
 
 def test_partition_empty_and_parameter_change(shy, orc):
