@@ -141,6 +141,11 @@ def lib():
         L.orc_wave_cycle.argtypes = [fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, P, C.c_uint64,
                                      C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Stats),
                                      C.c_void_p, C.c_void_p]
+        L.orc_wave_cycle_ex.argtypes = [fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, P, C.c_uint64,
+                                        C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Stats),
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_wave_cascade.argtypes = [fp, u8p, C.c_int, C.c_int, P, i32p]
+        L.orc_wave_cascade.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -220,9 +225,16 @@ _REF_FNL = os.path.join(_HERE, "_ref", "libfnl_ref.so")
 _ref_fnl = None
 
 
+def ref_fnl_built() -> bool:
+    """Whether ``make -C oracle ref`` has built the reference FastNoiseLite
+    library (checks the file only; nothing is loaded)."""
+    return os.path.exists(_REF_FNL)
+
+
 def ref_fnl():
     """The reference's own FastNoiseLite compiled by ``make -C oracle ref``, or
-    None when it has not been built (it needs the reference sources)."""
+    None when it has not been built (it needs the reference sources).  Loaded
+    on first call only, so a process that never runs its test never maps it."""
     global _ref_fnl
     if _ref_fnl is None and os.path.exists(_REF_FNL):
         L = C.CDLL(_REF_FNL)
@@ -305,19 +317,41 @@ def trace_frozen(h, d, mx, my, root, erfd, params, x0, y0, max_steps=600, use_re
 
 
 def wave_cycle(h, d, mx, my, root, erfd, params, seed, cycle, drops, terrain=0,
-               want_dh=False):
-    """One wave-synchronous cycle in place on SoA float32 (nx, ny) arrays."""
+               want_dh=False, want_tracks=False):
+    """One wave-synchronous cycle in place on SoA float32 (nx, ny) arrays.
+
+    ``want_dh`` adds the walk's summed height commits (int32, 2^-20 units) and
+    the cascade sites; ``want_tracks`` adds the cycle's summed discharge and
+    momentum tracks (int64 (3, nx, ny), 2^-24 units, before the blend)."""
     nx, ny = h.shape
     for a in (h, d, mx, my, root, erfd):
         assert a.dtype == np.float32 and a.flags.c_contiguous and a.shape == (nx, ny)
     st = Stats()
     dh = np.zeros((nx, ny), np.int32) if want_dh else None
     vis = np.zeros((nx, ny), np.uint8) if want_dh else None
-    lib().orc_wave_cycle(h.reshape(-1), d.reshape(-1), mx.reshape(-1), my.reshape(-1),
-                         root.reshape(-1), erfd.reshape(-1), nx, ny, C.byref(params), seed,
-                         terrain, cycle, drops, C.byref(st),
-                         dh.ctypes.data if want_dh else None,
-                         vis.ctypes.data if want_dh else None)
+    trk = np.zeros((3, nx, ny), np.int64) if want_tracks else None
+    lib().orc_wave_cycle_ex(h.reshape(-1), d.reshape(-1), mx.reshape(-1), my.reshape(-1),
+                            root.reshape(-1), erfd.reshape(-1), nx, ny, C.byref(params), seed,
+                            terrain, cycle, drops, C.byref(st),
+                            dh.ctypes.data if want_dh else None,
+                            vis.ctypes.data if want_dh else None,
+                            trk.ctypes.data if want_tracks else None)
+    out = (st,)
     if want_dh:
-        return st, dh, vis
-    return st
+        out += (dh, vis)
+    if want_tracks:
+        out += (trk,)
+    return out if len(out) > 1 else st
+
+
+def wave_cascade(H, visit, params) -> tuple:
+    """The wave's cascade stage alone on heights ``H`` with sites ``visit``:
+    returns (int32 deltas in 2^-20 units, number of non-zero transfers)."""
+    H = np.ascontiguousarray(H, np.float32)
+    nx, ny = H.shape
+    vis = np.ascontiguousarray(visit, np.uint8)
+    assert vis.shape == (nx, ny)
+    Cd = np.zeros((nx, ny), np.int32)
+    moves = lib().orc_wave_cascade(H.reshape(-1), vis.reshape(-1), nx, ny, C.byref(params),
+                                   Cd.reshape(-1))
+    return Cd, int(moves)

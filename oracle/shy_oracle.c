@@ -914,10 +914,41 @@ static int64_t wave_cascade_site(const float* H, int nx, int ny, const orc_param
   return moves;
 }
 
-void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
+/* The cascade stage of a wave (world.h:90-168 at every site of the cycle):
+   the window sequence of wave_cascade_site at each site, on the post-walk
+   heights H, each window weighted by m = the largest number of site windows
+   covering any cell of the window.  Adds the integer deltas to C (2^-20
+   units); returns the number of non-zero transfers. */
+int64_t orc_wave_cascade(const float* H, const uint8_t* visit, int nx, int ny,
+                         const orc_params* P, int32_t* C) {
+  size_t n = (size_t)nx * ny;
+  int32_t* wc = (int32_t*)calloc(n, sizeof(int32_t));
+  int64_t moves = 0;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      if (!visit[(size_t)x * ny + y]) continue;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          if (!oob(nx, ny, x + a, y + b)) wc[(size_t)(x + a) * ny + (y + b)] += 1;
+    }
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y) {
+      if (!visit[(size_t)x * ny + y]) continue;
+      int m = 1;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          if (!oob(nx, ny, x + a, y + b) && wc[(size_t)(x + a) * ny + (y + b)] > m)
+            m = wc[(size_t)(x + a) * ny + (y + b)];
+      moves += wave_cascade_site(H, nx, ny, P, x, y, m, C);
+    }
+  free(wc);
+  return moves;
+}
+
+void orc_wave_cycle_ex(float* h, float* d, float* mx, float* my, const float* root,
                     float* erfd, int nx, int ny, const orc_params* P, uint64_t seed,
                     uint32_t terrain, uint32_t cycle, int drops, orc_stats* st,
-                    int32_t* dh_out, uint8_t* visit_out) {
+                    int32_t* dh_out, uint8_t* visit_out, int64_t* trk_out) {
   size_t n = (size_t)nx * ny;
   wacc A;
   A.dh = (int32_t*)calloc(n, sizeof(int32_t));
@@ -928,7 +959,6 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
   A.count = (int32_t*)calloc(n, sizeof(int32_t));
   float* scale = (float*)malloc(n * sizeof(float));
   int32_t* C = (int32_t*)calloc(n, sizeof(int32_t));
-  int32_t* wc = (int32_t*)calloc(n, sizeof(int32_t));
   float* H = (float*)malloc(n * sizeof(float));
   orc_stats local;
   memset(&local, 0, sizeof(local));
@@ -961,28 +991,17 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
   }
   if (dh_out) memcpy(dh_out, A.dh, n * sizeof(int32_t));
   if (visit_out) memcpy(visit_out, A.visit, n);
+  if (trk_out) { /* the cycle's summed tracks, 2^-24 units, before the blend */
+    memcpy(trk_out, A.td, n * sizeof(int64_t));
+    memcpy(trk_out + n, A.tmx, n * sizeof(int64_t));
+    memcpy(trk_out + 2 * n, A.tmy, n * sizeof(int64_t));
+  }
 
   /* apply the walk's height commits */
   for (size_t i = 0; i < n; ++i) H[i] = h[i] + (float)A.dh[i] * HQ_INV;
 
   /* cascade: weighted Jacobi sweep over the sites */
-  for (int x = 0; x < nx; ++x)
-    for (int y = 0; y < ny; ++y) {
-      if (!A.visit[(size_t)x * ny + y]) continue;
-      for (int a = -1; a <= 1; ++a)
-        for (int b = -1; b <= 1; ++b)
-          if (!oob(nx, ny, x + a, y + b)) wc[(size_t)(x + a) * ny + (y + b)] += 1;
-    }
-  for (int x = 0; x < nx; ++x)
-    for (int y = 0; y < ny; ++y) {
-      if (!A.visit[(size_t)x * ny + y]) continue;
-      int m = 1;
-      for (int a = -1; a <= 1; ++a)
-        for (int b = -1; b <= 1; ++b)
-          if (!oob(nx, ny, x + a, y + b) && wc[(size_t)(x + a) * ny + (y + b)] > m)
-            m = wc[(size_t)(x + a) * ny + (y + b)];
-      local.casc_moves += wave_cascade_site(H, nx, ny, P, x, y, m, C);
-    }
+  local.casc_moves += orc_wave_cascade(H, A.visit, nx, ny, P, C);
 
   /* apply the cascade, blend (world.h:81-86), refresh erf(0.4 d) */
   for (size_t i = 0; i < n; ++i) {
@@ -1002,5 +1021,13 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
     st->walk_dh += local.walk_dh; st->casc_moves += local.casc_moves;
   }
   free(A.dh); free(A.td); free(A.tmx); free(A.tmy); free(A.visit); free(A.count);
-  free(scale); free(C); free(wc); free(H);
+  free(scale); free(C); free(H);
+}
+
+void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,
+                    float* erfd, int nx, int ny, const orc_params* P, uint64_t seed,
+                    uint32_t terrain, uint32_t cycle, int drops, orc_stats* st,
+                    int32_t* dh_out, uint8_t* visit_out) {
+  orc_wave_cycle_ex(h, d, mx, my, root, erfd, nx, ny, P, seed, terrain, cycle, drops, st,
+                    dh_out, visit_out, NULL);
 }

@@ -137,6 +137,18 @@ void orc_wave_cycle(float* h, float* d, float* mx, float* my,
                     const orc_params* p, uint64_t seed, uint32_t terrain,
                     uint32_t cycle, int drops, orc_stats* st,
                     int32_t* dh_out, uint8_t* visit_out);
+/* the same cycle; trk_out (3n int64: discharge, momentumx, momentumy
+   tracks in 2^-24 units, the cycle's sums before the blend) may be NULL */
+void orc_wave_cycle_ex(float* h, float* d, float* mx, float* my,
+                       const float* root, float* erfd, int nx, int ny,
+                       const orc_params* p, uint64_t seed, uint32_t terrain,
+                       uint32_t cycle, int drops, orc_stats* st,
+                       int32_t* dh_out, uint8_t* visit_out, int64_t* trk_out);
+/* the wave's cascade stage alone: windows at every site (visit != 0) on the
+   heights H, weighted by the largest site-window cover count m of their
+   cells; adds 2^-20 deltas to C; returns the non-zero transfers */
+int64_t orc_wave_cascade(const float* H, const uint8_t* visit, int nx, int ny,
+                         const orc_params* p, int32_t* C);
 
 #ifdef __cplusplus
 }

@@ -136,6 +136,45 @@ def test_wave_cycles_bit_exact(shy, orc, n, drops, cycles):
     w.close()
 
 
+def test_tracks_bit_exact(shy, orc):
+    """The cycle's discharge/momentum tracks (water.h:115-117), read back
+    before the next cycle, equal the oracle's integer track sums; on an
+    inclined plane with one drop they hold the closed form: each visited
+    cell the pre-evaporation volume and volume x the downhill step."""
+    import math
+    cases = [("fbm", 128, 2048, 5, None), ("plane", 64, 1, 41, (2 / 1024, -3 / 1024))]
+    for name, n, drops, seed, ab in cases:
+        say("tracks", name, n, drops)
+        if ab is None:
+            h = orc.fbm(seed, n, n)
+            P = orc.Params.default()
+        else:
+            x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+            h = (0.5 + ab[0] * x + ab[1] * y).astype(np.float32)
+            P = orc.Params.default(depositionRate=0.0)
+        z = np.zeros((n, n), np.float32)
+        d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+        w = shy.World(n, n, seed=seed)
+        w.params = shy.Params(depositionRate=P.depositionRate)
+        w.upload("height", h)
+        for c in range(2):
+            w.erode(drops)
+            _, trk = orc.wave_cycle(h, d, mx, my, root, erfd, P, seed, c, drops,
+                                    want_tracks=True)
+            for k, f in enumerate(("discharge_track", "momentumx_track", "momentumy_track")):
+                want = (trk[k].astype(np.float64) * 2.0 ** -24).astype(np.float32)
+                assert np.array_equal(bits(w.field(f)), bits(want)), (name, c, f)
+        if ab is not None:
+            u = -np.array(ab) / math.hypot(*ab)
+            t = w.field("discharge_track")
+            m = t > 0
+            tx, ty = w.field("momentumx_track")[m], w.field("momentumy_track")[m]
+            assert m.sum() > 10
+            assert np.all(tx * u[0] + ty * u[1] > 0)
+            assert np.allclose(np.hypot(tx, ty), math.sqrt(2) * t[m], rtol=1e-6)
+        w.close()
+
+
 def test_wave_stats_equal(shy, orc):
     n, drops, seed = 128, 2048, 7
     h = orc.fbm(seed, n, n)

@@ -19,9 +19,18 @@ sign or index, or a transposed operand fails one of them:
 * blend        -- the EMA decay closed form (B3).
 * ledger       -- sum(h) change = evaporation gains - exported sediment, for
                   the sequential and the wave oracle (B5).
+* tracks       -- on an inclined plane (B2) each visited cell holds the
+                  volume the drop carried into the step (before evaporation)
+                  and that volume times the downhill step sqrt(2) u, in the
+                  sequential and the wave restatement; the blended momentum
+                  points downhill.
 * wave         -- burst-scale closed forms; a one-drop wave equals the frozen
-                  trace's commits; sequential vs wave terrains agree in
-                  hypsometry and drainage statistics.
+                  trace's commits; the wave cascade at an isolated site equals
+                  the sequential World::cascade to the 2^-20 quantisation, and
+                  overlapping site windows are weighted by the largest cover
+                  count of their cells (hand values for m = 1, 2, 3);
+                  sequential vs wave terrains agree in hypsometry and drainage
+                  statistics.
 """
 import math
 
@@ -306,6 +315,174 @@ def test_one_drop_wave_matches_frozen_trace():
         assert abs(int(dh[c]) - q) <= 1, (k, c)
 
 
+def _plane(n, a, b):
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    return (0.5 + a * x + b * y).astype(np.float32)
+
+
+def _volumes(k):
+    """vol_0 = 1, vol_{j+1} = f32(f64(vol_j) (1 - e)): water.h:136's rounding."""
+    e = np.float64(np.float32(0.001))
+    v = [F32(1.0)]
+    for _ in range(k - 1):
+        v.append(F32(np.float64(v[-1]) * (1.0 - e)))
+    return np.array(v, np.float32)
+
+
+# Tracks on an inclined plane (water.h:115-117): with no erosion (deposition
+# rate 0) and slopes below maxdiff (no cascade transfer), every step is the
+# straight step sqrt(2) u down u = -(a, b)/|(a, b)| and starts on a new cell,
+# so each visited cell holds exactly one step's tracks: the volume the drop
+# carried INTO the step (before evaporation, :115 precedes :136) and that
+# volume times the step's speed (downhill, so its sign is fixed).  Slopes are
+# multiples of 2^-10, so the float32 heights are the plane exactly and the
+# normal's direction is exact to a few ulp.
+PLANES = [(4 / 1024, 0.0), (2 / 1024, -3 / 1024), (-3 / 1024, 1 / 1024), (0.0, -5 / 1024)]
+
+
+@pytest.mark.parametrize("a,b", PLANES)
+def test_wave_tracks_on_plane_closed_form(a, b):
+    n = 64
+    Pz = O.Params.default(depositionRate=0.0)
+    u = -np.array([a, b]) / math.hypot(a, b)
+    for seed in range(3):
+        h = _plane(n, a, b)
+        z = np.zeros_like(h)
+        d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+        st, dh, vis, trk = O.wave_cycle(h, d, mx, my, root, erfd, Pz, 40 + seed, 0, 1,
+                                        want_dh=True, want_tracks=True)
+        cells = np.argwhere(trk[0] != 0)
+        assert st.walked == 1 and st.exits == 1 and len(cells) == st.steps
+        cid = int(O.spawn_cells(40 + seed, 0, 0, 1, n * n)[0])
+        x0, y0 = divmod(cid, n)
+        rel = cells - np.array([x0, y0])
+        along = rel @ u
+        perp = rel @ np.array([-u[1], u[0]])
+        order = np.argsort(along)
+        cells, along, perp = cells[order], along[order], perp[order]
+        assert tuple(cells[0]) == (x0, y0)
+        assert np.abs(perp).max() < 1.5  # on the straight line of B2
+        assert np.all(np.abs(np.diff(along) - math.sqrt(2)) < 1.0)
+        vol = _volumes(len(cells))
+        td = trk[0][cells[:, 0], cells[:, 1]]
+        tx = trk[1][cells[:, 0], cells[:, 1]].astype(np.float64)
+        ty = trk[2][cells[:, 0], cells[:, 1]].astype(np.float64)
+        # discharge track: the pre-evaporation volume, exactly (2^-24 units)
+        assert np.array_equal(td, np.rint(vol.astype(np.float64) * 2.0 ** 24).astype(np.int64))
+        # momentum tracks: volume times the downhill step sqrt(2) u, to 1e-6
+        for t, uc in ((tx, u[0]), (ty, u[1])):
+            want = vol.astype(np.float64) * math.sqrt(2) * uc * 2.0 ** 24
+            assert np.all(np.abs(t - want) <= 1e-6 * 2.0 ** 24 + 1), (t[:3], want[:3])
+        assert np.array_equal(dh, np.zeros_like(dh))
+        # after the blend (world.h:81-86, from zero): 0.1 x the tracks, pointing downhill
+        c = (cells[:, 0], cells[:, 1])
+        assert np.all(mx[c] * u[0] + my[c] * u[1] > 0)
+        assert np.allclose(d[c], 0.1 * vol, rtol=1e-6)
+
+
+@pytest.mark.parametrize("a,b", PLANES)
+def test_sequential_tracks_on_plane_closed_form(a, b):
+    n = 64
+    Pz = O.Params.default(depositionRate=0.0)
+    u = -np.array([a, b]) / math.hypot(a, b)
+    h = _plane(n, a, b)
+    cells = O.to_aos(h)
+    drop = dict(age=0, pos=(30.0, 33.0), speed=(0.0, 0.0), volume=1.0, sediment=0.0)
+    k = 0
+    while True:
+        x, y = int(drop["pos"][0]), int(drop["pos"][1])
+        v = F32(drop["volume"])
+        before = cells[x, y, 4:7].copy()
+        alive = O.seq_descend(cells, Pz, drop)
+        assert np.array_equal(before, np.zeros(3, np.float32))  # a new cell every step
+        assert cells[x, y, 4] == v == _volumes(k + 1)[k]  # :115, before :136
+        sx, sy = float(drop["speed"][0]), float(drop["speed"][1])
+        assert abs(sx - math.sqrt(2) * u[0]) < 1e-6 and abs(sy - math.sqrt(2) * u[1]) < 1e-6
+        for j, uc in ((5, u[0]), (6, u[1])):
+            assert abs(float(cells[x, y, j]) - float(v) * math.sqrt(2) * uc) <= 2e-6
+        k += 1
+        if not alive:
+            break
+    assert drop["volume"] == 0.0 and k > 10  # left the map (water.h:139-142)
+    assert np.array_equal(cells[..., 0], h)  # no erosion, no cascade
+    # after a whole World::erode cycle the blended momentum points downhill
+    cells = O.to_aos(h)
+    O.seq_erode(cells, Pz, 3, 0, 20)
+    m = cells[..., 4] > 0
+    assert m.sum() > 20
+    assert np.all(cells[..., 2][m] * u[0] + cells[..., 3][m] * u[1] > 0)
+
+
+def _spike_transfers(hc, nb, dist, settling=0.8, maxdiff=0.01):
+    """World::cascade's window sequence (world.h:116-166) evaluated by hand in
+    double for a centre hc over neighbours nb (already in visiting order):
+    transfer = settling (|diff| - dist maxdiff) / 2, the centre re-read."""
+    out = []
+    for hn, dd in zip(nb, dist):
+        diff = hc - hn
+        ex = abs(diff) - dd * maxdiff if hn > 0.1 else abs(diff)
+        t = settling * ex / 2 if ex > 0 else 0.0
+        hc = hc - t if diff > 0 else hc + t
+        out.append(t)
+    return out, hc
+
+
+SPIKE_ORDER = [(-1, -1), (-1, 0), (-1, 1), (0, -1), (0, 1), (1, -1), (1, 0), (1, 1)]
+
+
+def test_wave_cascade_isolated_site_equals_sequential_cascade():
+    """One site (window weight m = 1): the wave's integer commits equal the
+    sequential World::cascade on the same heights, within the 2^-20
+    quantisation of each transfer."""
+    rng = np.random.default_rng(11)
+    for trial in range(20):
+        H = (0.3 + 0.2 * rng.random((7, 9))).astype(np.float32)
+        H[rng.integers(0, 7), rng.integers(0, 9)] = 0.05  # one cell below the 0.1 threshold
+        cx, cy = int(rng.integers(0, 7)), int(rng.integers(0, 9))  # edges included
+        H[cx, cy] += F32(rng.choice([0.4, -0.25]))
+        vis = np.zeros(H.shape, np.uint8)
+        vis[cx, cy] = 1
+        Cd, moves = O.wave_cascade(H, vis, P)
+        cells = O.to_aos(H)
+        O.seq_cascade(cells, P, float(cx), float(cy))
+        seq = cells[..., 0].astype(np.float64)
+        wave = H.astype(np.float64) + Cd * 2.0 ** -20
+        err = np.abs(seq - wave) * 2.0 ** 20
+        nb = np.ones(H.shape, bool)
+        nb[cx, cy] = False
+        assert err[nb].max() <= 1.0, trial
+        assert err[cx, cy] <= 0.5 * moves + 1.0, trial
+        assert moves == int((seq != H).sum() - (seq[cx, cy] != H[cx, cy])), trial
+        assert Cd.sum() == 0  # mass conserved exactly
+
+
+@pytest.mark.parametrize("others,m", [([(5, 7)], 2), ([(5, 7), (7, 5)], 3),
+                                      ([(5, 8)], 1)])
+def test_wave_cascade_window_weight_overlapping_sites(others, m):
+    """Site A at (5, 5) is a spike (1.0 over 0.5); the other sites sit on the
+    flat 0.5 and move nothing.  A's window is weighted by the LARGEST number
+    of site windows covering any of its cells: with a site at (5, 7) the
+    cells (4..6, 6) are covered twice (m = 2); with (5, 7) and (7, 5) the
+    cell (6, 6) three times (m = 3); a site at (5, 8) shares no cell (m = 1).
+    A's own centre is covered once in every case."""
+    H = np.full((11, 11), 0.5, np.float32)
+    H[5, 5] = 1.0
+    vis = np.zeros(H.shape, np.uint8)
+    vis[5, 5] = 1
+    for (x, y) in others:
+        vis[x, y] = 1
+    Cd, moves = O.wave_cascade(H, vis, P)
+    dist = [math.sqrt(2) if (i and j) else 1.0 for (i, j) in SPIKE_ORDER]
+    t, _ = _spike_transfers(1.0, [0.5] * 8, dist)
+    assert moves == 8
+    for (i, j), tk in zip(SPIKE_ORDER, t):
+        want = tk / m * 2.0 ** 20
+        assert abs(int(Cd[5 + i, 5 + j]) - want) <= 1.0, ((i, j), Cd[5 + i, 5 + j], want)
+    assert abs(int(Cd[5, 5]) + sum(t) / m * 2.0 ** 20) <= 8 * 0.5 + 1
+    assert Cd.sum() == 0
+    assert np.count_nonzero(Cd) == 9
+
+
 def test_spawn_stream_uniform_and_skip_rule():
     n = 64
     cells = O.spawn_cells(4, 0, 0, 20000, n * n)
@@ -339,7 +516,7 @@ def test_fnl_gradient_set_structure():
     assert edges[:60] == list(range(12)) * 5 and edges[60:] == [8, 1, 9, 3]
 
 
-@pytest.mark.skipif(O.ref_fnl() is None, reason="oracle/_ref not built (needs the reference "
+@pytest.mark.skipif(not O.ref_fnl_built(), reason="oracle/_ref not built (needs the reference "
                                                  "sources: make -C oracle ref)")
 def test_fnl_restatement_matches_reference_build():
     """The C restatement of quad::map::init's noise against the reference's own
