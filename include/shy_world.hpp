@@ -16,6 +16,7 @@
 #ifndef SHY_WORLD_HPP
 #define SHY_WORLD_HPP
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -24,17 +25,27 @@
 
 namespace shy {
 
+struct ivec2;
 struct vec2 {
   float x = 0.0f, y = 0.0f;
   vec2() = default;
   vec2(float a, float b) : x(a), y(b) {}
+  inline vec2(ivec2 v);  // implicit, as glm's conversion constructors
 };
 struct ivec2 {
   int x = 0, y = 0;
   ivec2() = default;
+  explicit ivec2(int a) : x(a), y(a) {}
   ivec2(int a, int b) : x(a), y(b) {}
-  explicit ivec2(vec2 v) : x((int)v.x), y((int)v.y) {}  // truncation, as glm
+  ivec2(vec2 v) : x((int)v.x), y((int)v.y) {}  // implicit truncation toward zero, as glm
 };
+inline vec2::vec2(ivec2 v) : x((float)v.x), y((float)v.y) {}
+inline vec2 operator+(vec2 a, vec2 b) { return vec2(a.x + b.x, a.y + b.y); }
+inline vec2 operator-(vec2 a, vec2 b) { return vec2(a.x - b.x, a.y - b.y); }
+inline ivec2 operator+(ivec2 a, ivec2 b) { return ivec2(a.x + b.x, a.y + b.y); }
+inline ivec2 operator-(ivec2 a, ivec2 b) { return ivec2(a.x - b.x, a.y - b.y); }
+inline ivec2 operator/(ivec2 a, int d) { return ivec2(a.x / d, a.y / d); }
+inline bool operator==(ivec2 a, ivec2 b) { return a.x == b.x && a.y == b.y; }
 struct vec3 {
   float x = 0.0f, y = 0.0f, z = 0.0f;
 };
@@ -56,16 +67,88 @@ struct cell {
   float rootdensity;
 };
 
+/* mappool::sliceval (source/cellpool.h:44-47): a cell reference and its
+   position in the slice, so `for (auto [cell, pos] : node.s)` binds `cell`
+   to the cell itself, as the reference's loops do (world.h:56-61, 81-86). */
+template <typename T>
+struct sliceval {
+  T& start;
+  ivec2 pos;
+};
+
+/* mappool::slice (source/cellpool.h:73-99) over the host snapshot: the
+   res.x by res.y cells from `org` of a grid with row stride `stride`,
+   iterated x-major (pos.y fastest), as slice_iterator does (:49-71). */
+template <typename T>
+struct slice {
+  T* root = nullptr;
+  int stride = 0;
+  ivec2 org = ivec2(0, 0);
+  ivec2 res = ivec2(0, 0);
+
+  size_t size() const { return (size_t)res.x * res.y; }
+  bool oob(ivec2 p) const { return p.x < 0 || p.y < 0 || p.x >= res.x || p.y >= res.y; }
+  T* get(ivec2 p) const {
+    if (root == nullptr || oob(p)) return nullptr;
+    return root + (size_t)(org.x + p.x) * stride + (org.y + p.y);
+  }
+
+  struct iterator {
+    const slice* s;
+    ivec2 pos;
+    sliceval<T> operator*() const { return {*s->get(pos), pos}; }
+    iterator& operator++() {
+      if (++pos.y == s->res.y) {
+        pos.y = 0;
+        ++pos.x;
+      }
+      return *this;
+    }
+    bool operator!=(const iterator& o) const { return !(pos == o.pos); }
+  };
+  iterator begin() const { return iterator{this, res.x > 0 && res.y > 0 ? ivec2(0, 0) : ivec2(res.x, 0)}; }
+  iterator end() const { return iterator{this, ivec2(res.x, 0)}; }
+};
+
+struct map;
+
+/* quad::node (source/cellpool.h:222-250): one tile of the map, a view of the
+   host snapshot.  The engine holds one seamless grid, so node::normal is the
+   map's normal; the reference's own node::normal reads zero heights beyond
+   its node (cellpool.h:237-240 through _normal), which equals the map's with
+   the reference's single node (mapsize = 1, cellpool.h:171). */
+struct node {
+  ivec2 pos = ivec2(0, 0);  // absolute position of the node's first cell
+  slice<cell> s;
+  const map* owner = nullptr;
+
+  cell* get(ivec2 p) { return s.get(p - pos); }
+  bool oob(ivec2 p) const { return s.oob(p - pos); }
+  float height(ivec2 p) {
+    cell* c = get(p);
+    return c == nullptr ? 0.0f : c->height;
+  }
+  inline float discharge(ivec2 p) const;  // erf(0.4 discharge), cellpool.h:242-244
+  inline vec3 normal(ivec2 p) const;
+};
+
 /* quad::map (source/cellpool.h:319-449): the device terrain plus a host
-   snapshot for the read accessors; sync() refreshes the snapshot. */
+   snapshot for the read accessors; sync() refreshes the snapshot and the
+   node grid, upload() writes the snapshot back.  The nodes tile the map in
+   tileres x tileres blocks (quad::tileres, cellpool.h:167-169), node (i, j)
+   at index i*mapsize_y + j as map::get computes it (:421-426). */
 struct map {
   shy_ctx* ctx = nullptr;
   int nx = 0, ny = 0;
+  int tileres = 512;
   std::vector<cell> cells;
   std::vector<float> erfd, normals;
+  std::vector<node> nodes;
+  int mapsize_x = 0, mapsize_y = 0;
 
-  // quad::map::init (cellpool.h:323-411): the OpenSimplex2 terrain at
-  // tileres 512 (cellpool.h:167-169), seed passed as the reference's int SEED
+  // quad::map::init (cellpool.h:323-411): the OpenSimplex2 terrain sampled at
+  // x / tileres (cellpool.h:360-404; the reference's tileres is 512,
+  // :167-169), seed passed as the reference's int SEED
   void init(int sx, int sy, unsigned int seed);
   // the seeded Perlin fBm of the benchmark workloads instead
   void init_fbm(int sx, int sy, unsigned int seed);
@@ -80,11 +163,32 @@ struct map {
     check(ctx, shy_download_cells(ctx, &cells[0].height));
     check(ctx, shy_download_field(ctx, SHY_ERF_DISCHARGE, erfd.data()));
     check(ctx, shy_download_normals(ctx, normals.data()));
+    build_nodes();
   }
   void upload() { check(ctx, shy_upload_cells(ctx, &cells[0].height)); }
+  void build_nodes() {
+    if (tileres < 1) throw Error(SHY_ERR_ARG, "map: tileres must be >= 1");
+    mapsize_x = (nx + tileres - 1) / tileres;
+    mapsize_y = (ny + tileres - 1) / tileres;
+    nodes.assign((size_t)mapsize_x * mapsize_y, node());
+    for (int i = 0; i < mapsize_x; ++i)
+      for (int j = 0; j < mapsize_y; ++j) {
+        node& t = nodes[(size_t)i * mapsize_y + j];
+        t.pos = ivec2(i * tileres, j * tileres);
+        t.owner = this;
+        t.s.root = cells.data();
+        t.s.stride = ny;
+        t.s.org = t.pos;
+        t.s.res = ivec2(std::min(tileres, nx - t.pos.x), std::min(tileres, ny - t.pos.y));
+      }
+  }
 
   bool oob(ivec2 p) const { return p.x < 0 || p.y < 0 || p.x >= nx || p.y >= ny; }
-  cell* getCell(ivec2 p) { return oob(p) ? nullptr : &cells[(size_t)p.x * ny + p.y]; }
+  node* get(ivec2 p) {
+    if (oob(p)) return nullptr;
+    return &nodes[(size_t)(p.x / tileres) * mapsize_y + (p.y / tileres)];
+  }
+  cell* getCell(ivec2 p) { return oob(p) ? nullptr : get(p)->get(p); }
   float height(ivec2 p) const { return oob(p) ? 0.0f : cells[(size_t)p.x * ny + p.y].height; }
   /* node::discharge: erf(0.4 * discharge) (cellpool.h:242-244) */
   float discharge(ivec2 p) const { return oob(p) ? 0.0f : erfd[(size_t)p.x * ny + p.y]; }
@@ -93,6 +197,9 @@ struct map {
     return vec3{n[0], n[1], n[2]};
   }
 };
+
+inline float node::discharge(ivec2 p) const { return owner->discharge(p); }
+inline vec3 node::normal(ivec2 p) const { return owner->normal(p); }
 
 struct Drop;
 
@@ -167,7 +274,7 @@ inline void map::init(int sx, int sy, unsigned int seed) {
   ny = sy;
   const shy_params p = World::params();
   check(nullptr, shy_create(&p, nx, ny, seed, &ctx));
-  check(ctx, shy_generate_reference(ctx, (int)seed, 512));
+  check(ctx, shy_generate_reference(ctx, (int)seed, tileres));
   sync();
 }
 

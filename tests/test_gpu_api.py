@@ -104,8 +104,8 @@ def test_map_view_accessors(shy, orc):
     w.close()
 
 
-def fnv_hash(h, d):
-    v = np.stack([h.ravel(), d.ravel()], 1).astype(np.float32).view(np.uint8).ravel()
+def fnv_hash(*fields):
+    v = np.stack([f.ravel() for f in fields], 1).astype(np.float32).view(np.uint8).ravel()
     x = 1469598103934665603
     for b in v.tolist():
         x ^= b
@@ -113,28 +113,46 @@ def fnv_hash(h, d):
     return f"{x:016x}"
 
 
+def node_major(a, tileres):
+    """Cells in the order map.nodes then node.s visit them: node (i, j) at
+    index i*mapsize + j (cellpool.h:421-426), x-major inside a node."""
+    nx, ny = a.shape
+    parts = [a[i:i + tileres, j:j + tileres].ravel()
+             for i in range(0, nx, tileres) for j in range(0, ny, tileres)]
+    return np.concatenate(parts)
+
+
 def test_cpp_facade_demo_matches_python_path(shy):
-    """include/shy_world.hpp (World::erode, Drop::descend) through the C ABI
-    gives the same bits as the Python host for the same loop."""
+    """include/shy_world.hpp (World::erode, Drop::descend, the node-level map
+    chain map.get(p)->get(p), map.nodes / node.s iteration, getCell with
+    implicit vec2 -> ivec2) through the C ABI gives the same bits as the
+    Python host for the same loop."""
     import json
     import os
     import subprocess
     demo = os.path.join(os.path.dirname(shy.LIB_PATH), "world_demo")
     n, cycles, drops, seed = 64, 6, 512, 3
-    for terrain in ("reference", "fbm"):
-        print("demo", terrain, flush=True)
-        r = subprocess.run([demo, str(n), str(cycles), str(drops), str(seed), terrain],
-                           capture_output=True, text=True, timeout=120)
+    inb = sum(1 for x in range(n) for y in range(n) for dx in (-1, 0, 1) for dy in (-1, 0, 1)
+              if (dx or dy) and 0 <= x + dx < n and 0 <= y + dy < n)
+    for terrain, tileres in (("reference", 512), ("fbm", 32), ("reference", 24)):
+        print("demo", terrain, tileres, flush=True)
+        r = subprocess.run([demo, str(n), str(cycles), str(drops), str(seed), terrain,
+                            str(tileres)], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         out = json.loads(r.stdout)
         w = shy.World(n, n, seed=seed)
         if terrain == "fbm":
             w.generate_fbm(seed, 8)
         else:
-            w.generate_reference(seed, 512)
+            w.generate_reference(seed, tileres)
         for _ in range(cycles):
             w.erode(drops)
-        assert out["hash"] == fnv_hash(w.field("height"), w.field("discharge"))
+        h, d = w.field("height"), w.field("discharge")
+        assert out["nodes"] == (-(-n // tileres)) ** 2
+        assert out["hash"] == fnv_hash(h, d)
+        assert out["hash_nodes"] == fnv_hash(node_major(h, tileres), node_major(d, tileres))
+        assert out["hash_momentum"] == fnv_hash(w.field("momentumx"), w.field("momentumy"))
+        assert out["neighbours"] == inb
         assert out["particle_steps"] == w.stats()["steps"]
         drop = w.drop((n // 2 + 0.5, n // 3 + 0.5))
         steps = 0
