@@ -7,21 +7,36 @@ parameters, 65536 drops per cycle.  A "step" of this script is one cycle =
 one World::erode(65536) call (walk + cascade/blend).  The reported value is
 particle-steps/s over exactly --steps timed cycles, inputs resident in HBM.
 
-Prints one JSON line.  Roofline: the kernel with the largest measured time
-per cycle, against the HBM peak, with its algorithmic bytes per
-particle-step:
-  count walk (walk_kernel)  B_COUNT = 45: reads {h, normal x, normal z,
-      erf(0.4 d), mx, my, d} of the step's cell = 28 B; writes the 16-B step
-      record {start cell, c_eq, tracks}; cascade-site byte 1 B
-  commit replay (replay_kernel)  B_REPLAY = 16: reads the start cell and c_eq
-      of the step 8 B and the cell's burst scale 4 B (and its root density,
-      4 B more, when a terrain has one); writes the height commit 4 B
-``traffic`` is the HBM bytes per launch of that kernel measured by rocprofv3
-PMC runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy-kernel calibration) on the
-same command, read from profiles/hbm_traffic.json when it holds an entry for
-this workload, else null.  CPU baseline: the oracle's sequential restatement
-of World::erode on the same terrain and spawn stream, one core, on a bounded
+Prints one JSON line.  Roofline ("roofline"): the dominant kernel (the
+larger measured time per cycle of the count walk and the commit replay)
+against the HBM peak, with the algorithmic bytes of a particle-step fixed by
+the survey of the reference's hot path (water.h:58-156 with the cascade of
+world.h:90-168 moved to a per-cycle stencil pass):
+  B_STEP = 60 B per particle-step (10 map reads, 4 commits, 1 visit, 4 B each)
+  B_CELL = 56 B per cell per cycle (cascade pass 8 B, track reset + blend 48 B)
+achieved = particle-steps per launch x B_STEP / the kernel's average launch
+time.  "cycle" gives the same for the whole cycle (steps x B_STEP + cells x
+B_CELL over the cycle's time).  Each kernel's own algorithmic bytes are
+reported beside it as a secondary figure:
+  count walk (walk_kernel)  45 B: reads {h, normal x, normal z, erf(0.4 d),
+      mx, my, d} of the step's cell = 28 B; writes the 16-B step record
+      {start cell, c_eq, tracks}; cascade-site byte 1 B
+  commit replay (replay_kernel)  16 B: reads the start cell and c_eq of the
+      step 8 B and the cell's burst scale 4 B (and its root density, 4 B
+      more, when a terrain has one); writes the height commit 4 B
+``traffic`` is the bytes per launch of that kernel between L2 and the memory
+fabric (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+calibration; part of it may be served by the 256 MB last-level cache rather
+than HBM) and ``l2_hit`` its L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)),
+both read from profiles/hbm_traffic.json when it holds an entry for this
+workload, else null.  CPU baseline: the oracle's sequential restatement of
+World::erode on the same terrain and spawn stream, one core, on a bounded
 sample of cycles.
+
+The default line also carries "ensemble": the batched ensemble of 64 uneven
+terrains (sides 128..1024, per-terrain parameters, 1 drop per 16 cells) in
+one launch per kernel, autotuned and timed the same way (--no-ensemble skips
+it; --workload ensemble prints it as its own line).
 
 The second half of the metric, time-to-steady discharge with the surrogate
 warm start vs the plain GPU method, is measured on a 2048x2048 fixed fBm
@@ -39,23 +54,30 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-B_COUNT = 45          # algorithmic bytes per particle-step, count walk
-B_REPLAY = 16         # algorithmic bytes per particle-step, commit replay
+B_STEP = 60           # algorithmic bytes per particle-step (cascade in the stencil pass)
+B_CELL = 56           # algorithmic bytes per cell per cycle: cascade pass 8 + blend 48
+B_COUNT = 45          # the count walk's own bytes per particle-step
+B_REPLAY = 16         # the commit replay's own bytes per particle-step
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNELS = (("walk_kernel", "ms_count_walk", B_COUNT), ("replay_kernel", "ms_replay", B_REPLAY))
 
 
-def roofline(tm: dict, cycles: int, steps: int, workload: str) -> dict:
-    """Roofline of the dominant kernel (largest summed time) and the
-    achieved bandwidth of each walk kernel."""
+def roofline(tm: dict, cycles: int, steps: int, cells: int, workload: str) -> dict:
+    """Roofline of the dominant kernel (largest summed time) at B_STEP bytes
+    per particle-step, each walk kernel at its own bytes, and the whole
+    cycle."""
     per = {}
     for name, key, b in KERNELS:
         avg_s = tm[key] / 1e3 / cycles
         per[name] = {"ms_avg": avg_s * 1e3, "bytes_per_step": b,
-                     "achieved": steps / cycles * b / avg_s / 1e9}
+                     "achieved": steps / cycles * b / avg_s / 1e9,
+                     "frac": steps / cycles * b / avg_s / 1e9 / HBM_PEAK_GBS}
     dom = max(per, key=lambda k: per[k]["ms_avg"])
     d = per[dom]
-    traffic, source = None, None
+    achieved = steps / cycles * B_STEP / (d["ms_avg"] / 1e3) / 1e9
+    cyc_s = tm["ms_total"] / 1e3 / cycles
+    cyc_bytes = steps / cycles * B_STEP + cells * B_CELL
+    traffic, l2_hit, source = None, None, None
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
@@ -63,8 +85,9 @@ def roofline(tm: dict, cycles: int, steps: int, workload: str) -> dict:
         entry = wl.get(dom)
         if entry:
             live = steps / cycles
-            source = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated: "
-                      "profiles/hbm_traffic.json")
+            l2_hit = entry.get("l2_hit")
+            source = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (copy-calibrated) and "
+                      "TCC_HIT / TCC_MISS: profiles/hbm_traffic.json")
             if abs(live / wl["steps_per_launch"] - 1.0) < 1e-9:
                 traffic = entry["bytes_per_launch"]
                 source += " (same command, same particle-steps per launch)"
@@ -74,9 +97,16 @@ def roofline(tm: dict, cycles: int, steps: int, workload: str) -> dict:
                 traffic = entry["bytes_per_step"] * live
                 source += (f" (measured {entry['bytes_per_step']:.1f} B per particle-step at "
                            f"{wl['steps_per_launch']:.4g} steps/launch, scaled to this run)")
-    out = {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
-           "unit": "GB/s", "frac": d["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
-           "bytes_per_step": d["bytes_per_step"], "kernels": per}
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "l2_hit": l2_hit, "bytes_per_step": B_STEP,
+           "kernel_own": {"bytes_per_step": d["bytes_per_step"], "achieved": d["achieved"],
+                          "frac": d["frac"]},
+           "cycle": {"ms": cyc_s * 1e3, "algorithmic_bytes": cyc_bytes,
+                     "achieved": cyc_bytes / cyc_s / 1e9,
+                     "frac": cyc_bytes / cyc_s / 1e9 / HBM_PEAK_GBS,
+                     "bytes": f"{B_STEP} B/particle-step + {B_CELL} B/cell"},
+           "kernels": per}
     if source:
         out["traffic_source"] = source
     return out
@@ -120,33 +150,31 @@ def cpu_baseline(n: int, drops: int, seed: int, cycles: int) -> dict:
 def ensemble_specs(shy, n_terrains: int, seed: int):
     """64 terrains, sides uniform over {128,...,1024} from the seed, each with
     its own fBm seed, evapRate in [0.0005, 0.002], depositionRate in
-    [0.05, 0.2] and 1 drop per 16 cells."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    sides = (128, 256, 384, 512, 768, 1024)
+    [0.05, 0.2] and 1 drop per 16 cells (shy.ensemble_spec)."""
     specs = []
-    for t in range(n_terrains):
-        s = int(rng.choice(sides))
-        p = shy.Params(evapRate=float(rng.uniform(0.0005, 0.002)),
-                       depositionRate=float(rng.uniform(0.05, 0.2)))
-        specs.append(dict(nx=s, ny=s, drops=s * s // 16, seed=int(rng.integers(1, 2**62)),
-                          params=p))
+    for t in shy.ensemble_spec(n_terrains, seed):
+        specs.append(dict(nx=t["nx"], ny=t["ny"], drops=t["drops"], seed=t["seed"],
+                          params=shy.Params(evapRate=t["evapRate"],
+                                            depositionRate=t["depositionRate"])))
     return specs
 
 
-def run_ensemble(shy, args) -> dict:
-    specs = ensemble_specs(shy, args.terrains, args.seed)
+def run_ensemble(shy, terrains: int, seed: int, cycles: int, warmup: int, autotune: bool) -> dict:
+    """The batched ensemble: ``warmup`` graph-replayed cycles, then ``cycles``
+    timed cycles with HIP events between kernels (after the autotuner picked
+    the launch shape, when ``autotune``)."""
+    specs = ensemble_specs(shy, terrains, seed)
     e = shy.Ensemble(specs)
     for t, s in enumerate(specs):
         e.generate_fbm(t, s["seed"], 8)
     tuned, tuned_ms = None, None
-    if not args.no_autotune:
+    if autotune:
         tuned, tuned_ms = e.autotune(2)
-    if args.warmup:
-        e.erode(args.warmup, graph=True)
+    if warmup:
+        e.erode(warmup, graph=True)
     e.synchronize()
     s0 = e.stats()["steps"]
-    tm = e.time_cycles(args.steps)
+    tm = e.time_cycles(cycles)
     steps = e.stats()["steps"] - s0
     sec = tm["ms_total"] / 1e3
     cells = sum(s["nx"] * s["ny"] for s in specs)
@@ -154,17 +182,17 @@ def run_ensemble(shy, args) -> dict:
     e.close()
     return {
         "metric": "particle-steps/s, ensemble of uneven terrains",
-        "value": steps / sec, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": tm["ms_total"] / args.steps,
+        "value": steps / sec, "unit": "particle-steps/s", "n_gpus": 1, "steps": cycles,
+        "warmup": warmup, "ms_per_step": tm["ms_total"] / cycles,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"batched ensemble: {len(specs)} fBm terrains, sides in "
                                "{128..1024}, per-terrain params, 1 drop per 16 cells",
                    "cells": cells, "drops_per_cycle": drops, "tuning": tuned,
                    "tuned_ms_per_cycle": tuned_ms},
-        "cycles_per_s": args.steps / sec, "particle_steps": steps,
-        "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
-        "roofline": roofline(tm, args.steps, steps, "ensemble64"),
+        "cycles_per_s": cycles / sec, "particle_steps": steps,
+        "kernel_ms_per_cycle": {k[3:]: v / cycles for k, v in tm.items() if k != "ms_total"},
+        "roofline": roofline(tm, cycles, steps, cells, "ensemble64"),
         "cpu_baseline": None,
     }
 
@@ -215,6 +243,10 @@ def main():
                          "launch per kernel")
     ap.add_argument("--terrains", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="leave the ensemble block out of the default line")
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="timed cycles of the ensemble block")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the time-to-steady (surrogate vs plain) leg")
     ap.add_argument("--surrogate-size", type=int, default=2048)
@@ -242,7 +274,8 @@ def main():
         del src, dst
 
     if args.workload == "ensemble":
-        print(json.dumps(run_ensemble(shy, args)))
+        print(json.dumps(run_ensemble(shy, args.terrains, args.seed, args.steps, args.warmup,
+                                      not args.no_autotune)))
         return
 
     n, drops = args.size, args.drops
@@ -282,9 +315,15 @@ def main():
         "particle_steps": steps,
         "host_wall_s": wall,
         "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
-        "roofline": roofline(tm, args.steps, steps, f"single{n}_drops{drops}"),
+        "roofline": roofline(tm, args.steps, steps, n * n, f"single{n}_drops{drops}"),
     }
     w.close()
+    if not args.no_ensemble:
+        ens = run_ensemble(shy, args.terrains, args.seed, args.ensemble_steps, 5,
+                           not args.no_autotune)
+        out["ensemble"] = {k: ens[k] for k in (
+            "value", "unit", "steps", "warmup", "ms_per_step", "cycles_per_s", "particle_steps",
+            "config", "kernel_ms_per_cycle", "roofline")}
     if not args.no_surrogate:
         out.update(surrogate_leg(7, args.surrogate_size))
     if not args.no_cpu:

@@ -264,12 +264,14 @@ int shy_vegetation_upload(shy_ctx* ctx, const float* xys, int n);
 
 /* World::erode(cycles) (source/world.h:54-88): one cycle of `drops` spawns.
    Wave-synchronous: every drop reads the map as it stood at the start of the
-   cycle.  A counting pass finds how many walk steps start at each cell; the
-   commit pass then applies each drop's erosion scaled so that the k drops
-   through a cell commit what k drops in sequence would, through fixed-point
-   atomics (the result does not depend on drop order); the commits are
-   applied, the cascade runs as a weighted stencil over the cells drops
-   arrived at, and the discharge/momentum fields are blended. */
+   cycle.  A counting pass walks every drop and logs each step's start cell;
+   the log is partitioned by cell tile and counted per cell, so the commit
+   pass can scale each step's erosion such that the k drops through a cell
+   commit what k drops in sequence would.  Every commit is quantised to an
+   integer (heights 2^-20, tracks 2^-24) and summed per cell in integers, with
+   no per-step atomic on the map, so the result does not depend on drop order
+   or launch shape; the cascade then runs as a weighted stencil over the cells
+   drops arrived at, and the discharge/momentum fields are blended. */
 int shy_erode(shy_ctx* ctx, int drops);
 /* `cycles` consecutive World::erode(drops) calls; use_graph=1 replays them
    from a captured hipGraph. */

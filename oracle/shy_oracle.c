@@ -170,6 +170,31 @@ void orc_fbm(uint64_t seed, int nx, int ny, int octaves, float base_freq,
 }
 
 /* --------------------------------------------- the reference's terrain */
+/* The OpenSimplex2 restatement below follows FastNoiseLite, which carries
+   this notice:
+
+     MIT License
+
+     Copyright(c) 2020 Jordan Peck (jordan.me2@gmail.com)
+     Copyright(c) 2020 Contributors
+
+     Permission is hereby granted, free of charge, to any person obtaining a
+     copy of this software and associated documentation files(the "Software"),
+     to deal in the Software without restriction, including without limitation
+     the rights to use, copy, modify, merge, publish, distribute, sublicense,
+     and / or sell copies of the Software, and to permit persons to whom the
+     Software is furnished to do so, subject to the following conditions :
+
+     The above copyright notice and this permission notice shall be included in
+     all copies or substantial portions of the Software.
+
+     THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR
+     IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF MERCHANTABILITY,
+     FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT.IN NO EVENT SHALL THE
+     AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER
+     LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING
+     FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE USE OR OTHER
+     DEALINGS IN THE SOFTWARE. */
 /* quad::map::init (cellpool.h:349-409) drives FastNoiseLite, a third-party
    header vendored in the reference (source/include/FastNoiseLite.h, MIT, no
    version number in the file).  Restated here: GetNoise(x, y, z)
@@ -426,6 +451,12 @@ int orc_veg_grow(const float* h, const float* d, float* root, int nx, int ny, in
 }
 
 /* ------------------------------------------------------------ accessors */
+/* ivec2(pos) of one coordinate: truncation toward zero (glm's conversion).
+   A NaN coordinate (a speed made NaN by a volume that underflowed to 0 with
+   minVol <= 0) is -1, off the map: (int)NaN is undefined in C, and the
+   reference's x86 build gets INT_MIN, also off the map. */
+static inline int cell_coord(float p) { return p == p ? (int)p : -1; }
+
 static inline int oob(int nx, int ny, int x, int y) {
   /* quad::map::oob, cellpool.h:413-419 */
   return x < 0 || y < 0 || x >= nx || y >= ny;
@@ -518,7 +549,7 @@ void orc_seq_cascade(float* cells, int nx, int ny, const orc_params* P, float fx
   int px[8], py[8];
   float sh[8], sd[8];
   int num = 0;
-  int cx = (int)fx, cy = (int)fy;
+  int cx = cell_coord(fx), cy = cell_coord(fy);
   for (int k = 0; k < 8; ++k) {
     int x = cx + 1 * off[k][0], y = cy + 1 * off[k][1];
     if (oob(nx, ny, x, y)) continue;
@@ -568,7 +599,7 @@ void orc_seq_cascade(float* cells, int nx, int ny, const orc_params* P, float fx
 static int seq_descend_impl(float* cells, int nx, int ny, const orc_params* P, int32_t* age,
                             float* pos, float* speed, float* volume, float* sediment,
                             double* gain) {
-  int ix = (int)pos[0], iy = (int)pos[1];
+  int ix = cell_coord(pos[0]), iy = cell_coord(pos[1]);
   if (oob(nx, ny, ix, iy)) return 0; /* :62-68 */
   float* c = &cells[((size_t)ix * ny + iy) * C_N];
   float n[3];
@@ -588,7 +619,7 @@ static int seq_descend_impl(float* cells, int nx, int ny, const orc_params* P, i
   c[C_DT] += *volume; /* :115-117 */
   c[C_MXT] += *volume * sx;
   c[C_MYT] += *volume * sy;
-  int jx = (int)px, jy = (int)py;
+  int jx = cell_coord(px), jy = cell_coord(py);
   float h2;
   if (oob(nx, ny, jx, jy)) /* :120-124 */
     h2 = (float)((double)c[C_H] - 0.002);
@@ -698,7 +729,7 @@ int orc_trace_frozen(const float* h, const float* d, const float* mx, const floa
   int nst = 0;
   *end_reason = 0;
   for (;;) {
-    int ix = (int)px, iy = (int)py;
+    int ix = cell_coord(px), iy = cell_coord(py);
     if (oob(nx, ny, ix, iy)) { *end_reason = 1; break; }
     size_t i0 = (size_t)ix * ny + iy;
     float n[3];
@@ -708,7 +739,7 @@ int orc_trace_frozen(const float* h, const float* d, const float* mx, const floa
     float effD = P->depositionRate * (1.0f - root[i0]);
     if (effD < 0) effD = 0;
     step_motion(P, n, vol, d[i0], mx[i0], my[i0], &sx, &sy, &px, &py);
-    int jx = (int)px, jy = (int)py;
+    int jx = cell_coord(px), jy = cell_coord(py);
     float hc = h[i0];
     float h2 = oob(nx, ny, jx, jy) ? (float)((double)hc - 0.002) : h[(size_t)jx * ny + jy];
     float E = use_ref_erf ? erff(0.4f * d[i0]) : erfd[i0];
@@ -778,7 +809,7 @@ static void wave_walk(const float* h, const float* d, const float* mx, const flo
   double gain = 0.0;
   int64_t steps = 0;
   for (;;) {
-    int ix = (int)px, iy = (int)py;
+    int ix = cell_coord(px), iy = cell_coord(py);
     size_t i0 = (size_t)ix * ny + iy; /* in bounds: spawn or checked below */
     float n[3];
     normal_st(h, 1, nx, ny, ix, iy, n);
@@ -801,7 +832,7 @@ static void wave_walk(const float* h, const float* d, const float* mx, const flo
       A->tmx[i0] += qt(vol * sx);
       A->tmy[i0] += qt(vol * sy);
     }
-    int jx = (int)px, jy = (int)py;
+    int jx = cell_coord(px), jy = cell_coord(py);
     int ex = oob(nx, ny, jx, jy);
     float hc = h[i0];
     float h2 = ex ? (float)((double)hc - 0.002) : h[(size_t)jx * ny + jy];
@@ -849,11 +880,17 @@ static float powi_f32(float r, int32_t k) {
   return res;
 }
 
+/* Burst scale of a cell that k count-pass steps start at: with g = effD (1 +
+   entrainment erf(0.4 d)) the fraction of its distance to equilibrium one
+   step moves it (water.h:127-132), k steps in sequence move it
+   R = 1 - (1-g)^k; the k commits are scaled by R / (k g).  For g > 1, R is
+   capped at 1: a wave updates a cell and its downstream neighbour from one
+   snapshot, a Jacobi step that is stable only for R <= 1. */
 float orc_burst_scale(int32_t k, float effD, float entrainment, float erfd) {
-  if (k <= 1) return 1.0f;
   float g = effD * (1.0f + entrainment * erfd);
   if (!(g > 0.0f)) return 1.0f;
-  if (g > 1.0f) g = 1.0f;
+  if (g > 1.0f) return 1.0f / ((float)(k < 1 ? 1 : k) * g);
+  if (k <= 1) return 1.0f;
   return (1.0f - powi_f32(1.0f - g, k)) / (g * (float)k);
 }
 
@@ -945,10 +982,66 @@ int64_t orc_wave_cascade(const float* H, const uint8_t* visit, int nx, int ny,
   return moves;
 }
 
+/* One wave of the drops [drop0, drop1) of a cycle on the current heights h:
+   count walk, burst scale, scaled commits, the cascade over the wave's sites,
+   and the heights updated in place.  Tracks are added to A's track sums (the
+   caller blends once per cycle); dh/visit, when given, are accumulated. */
+static void wave_pass(float* h, const float* d, const float* mx, const float* my,
+                      const float* root, const float* erfd, int nx, int ny, const orc_params* P,
+                      uint64_t key, int drop0, int drop1, wacc* A, orc_stats* local,
+                      int32_t* dh_out, uint8_t* visit_out) {
+  size_t n = (size_t)nx * ny;
+  memset(A->dh, 0, n * sizeof(int32_t));
+  memset(A->visit, 0, n);
+  memset(A->count, 0, n * sizeof(int32_t));
+  float* scale = (float*)malloc(n * sizeof(float));
+  int32_t* C = (int32_t*)calloc(n, sizeof(int32_t));
+  float* H = (float*)malloc(n * sizeof(float));
+
+  /* pass A: visit counts (no commits) */
+  A->scale = NULL;
+  for (int i = drop0; i < drop1; ++i) {
+    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
+    if ((double)h[cid] < 0.1) continue;
+    wave_walk(h, d, mx, my, root, erfd, nx, ny, P, (float)(cid / (uint32_t)ny),
+              (float)(cid % (uint32_t)ny), A, local);
+  }
+  for (size_t c = 0; c < n; ++c) {
+    float effD = P->depositionRate * (1.0f - root[c]);
+    if (effD < 0) effD = 0;
+    scale[c] = orc_burst_scale(A->count[c], effD, P->entrainment, erfd[c]);
+  }
+  /* pass B: scaled commits */
+  A->scale = scale;
+  for (int i = drop0; i < drop1; ++i) {
+    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
+    if ((double)h[cid] < 0.1) {
+      local->skipped++;
+      continue;
+    }
+    local->walked++;
+    wave_walk(h, d, mx, my, root, erfd, nx, ny, P, (float)(cid / (uint32_t)ny),
+              (float)(cid % (uint32_t)ny), A, local);
+  }
+  if (dh_out)
+    for (size_t i = 0; i < n; ++i) dh_out[i] += A->dh[i];
+  if (visit_out)
+    for (size_t i = 0; i < n; ++i) visit_out[i] |= A->visit[i];
+
+  /* apply the walk's height commits */
+  for (size_t i = 0; i < n; ++i) H[i] = h[i] + (float)A->dh[i] * HQ_INV;
+
+  /* cascade: weighted Jacobi sweep over the sites */
+  local->casc_moves += orc_wave_cascade(H, A->visit, nx, ny, P, C);
+
+  for (size_t i = 0; i < n; ++i) h[i] = H[i] + (float)C[i] * HQ_INV;
+  free(scale); free(C); free(H);
+}
+
 void orc_wave_cycle_ex(float* h, float* d, float* mx, float* my, const float* root,
-                    float* erfd, int nx, int ny, const orc_params* P, uint64_t seed,
-                    uint32_t terrain, uint32_t cycle, int drops, orc_stats* st,
-                    int32_t* dh_out, uint8_t* visit_out, int64_t* trk_out) {
+                       float* erfd, int nx, int ny, const orc_params* P, uint64_t seed,
+                       uint32_t terrain, uint32_t cycle, int drops, orc_stats* st,
+                       int32_t* dh_out, uint8_t* visit_out, int64_t* trk_out) {
   size_t n = (size_t)nx * ny;
   wacc A;
   A.dh = (int32_t*)calloc(n, sizeof(int32_t));
@@ -957,55 +1050,21 @@ void orc_wave_cycle_ex(float* h, float* d, float* mx, float* my, const float* ro
   A.tmy = (int64_t*)calloc(n, sizeof(int64_t));
   A.visit = (uint8_t*)calloc(n, 1);
   A.count = (int32_t*)calloc(n, sizeof(int32_t));
-  float* scale = (float*)malloc(n * sizeof(float));
-  int32_t* C = (int32_t*)calloc(n, sizeof(int32_t));
-  float* H = (float*)malloc(n * sizeof(float));
   orc_stats local;
   memset(&local, 0, sizeof(local));
   uint64_t key = orc_cycle_key(seed, terrain, cycle);
+  if (dh_out) memset(dh_out, 0, n * sizeof(int32_t));
+  if (visit_out) memset(visit_out, 0, n);
 
-  /* pass A: visit counts (no commits) */
-  A.scale = NULL;
-  for (int i = 0; i < drops; ++i) {
-    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
-    if ((double)h[cid] < 0.1) continue;
-    wave_walk(h, d, mx, my, root, erfd, nx, ny, P, (float)(cid / (uint32_t)ny),
-              (float)(cid % (uint32_t)ny), &A, &local);
-  }
-  for (size_t c = 0; c < n; ++c) {
-    float effD = P->depositionRate * (1.0f - root[c]);
-    if (effD < 0) effD = 0;
-    scale[c] = orc_burst_scale(A.count[c], effD, P->entrainment, erfd[c]);
-  }
-  /* pass B: scaled commits */
-  A.scale = scale;
-  for (int i = 0; i < drops; ++i) {
-    uint32_t cid = orc_spawn_cell(key, (uint32_t)i, (uint32_t)n);
-    if ((double)h[cid] < 0.1) {
-      local.skipped++;
-      continue;
-    }
-    local.walked++;
-    wave_walk(h, d, mx, my, root, erfd, nx, ny, P, (float)(cid / (uint32_t)ny),
-              (float)(cid % (uint32_t)ny), &A, &local);
-  }
-  if (dh_out) memcpy(dh_out, A.dh, n * sizeof(int32_t));
-  if (visit_out) memcpy(visit_out, A.visit, n);
+  wave_pass(h, d, mx, my, root, erfd, nx, ny, P, key, 0, drops, &A, &local, dh_out, visit_out);
   if (trk_out) { /* the cycle's summed tracks, 2^-24 units, before the blend */
     memcpy(trk_out, A.td, n * sizeof(int64_t));
     memcpy(trk_out + n, A.tmx, n * sizeof(int64_t));
     memcpy(trk_out + 2 * n, A.tmy, n * sizeof(int64_t));
   }
 
-  /* apply the walk's height commits */
-  for (size_t i = 0; i < n; ++i) H[i] = h[i] + (float)A.dh[i] * HQ_INV;
-
-  /* cascade: weighted Jacobi sweep over the sites */
-  local.casc_moves += orc_wave_cascade(H, A.visit, nx, ny, P, C);
-
-  /* apply the cascade, blend (world.h:81-86), refresh erf(0.4 d) */
+  /* blend (world.h:81-86), refresh erf(0.4 d) */
   for (size_t i = 0; i < n; ++i) {
-    h[i] = H[i] + (float)C[i] * HQ_INV;
     float t_d = (float)((double)A.td[i] * TQ_INV);
     float t_x = (float)((double)A.tmx[i] * TQ_INV);
     float t_y = (float)((double)A.tmy[i] * TQ_INV);
@@ -1021,7 +1080,6 @@ void orc_wave_cycle_ex(float* h, float* d, float* mx, float* my, const float* ro
     st->walk_dh += local.walk_dh; st->casc_moves += local.casc_moves;
   }
   free(A.dh); free(A.td); free(A.tmx); free(A.tmy); free(A.visit); free(A.count);
-  free(scale); free(C); free(H);
 }
 
 void orc_wave_cycle(float* h, float* d, float* mx, float* my, const float* root,

@@ -55,6 +55,27 @@ class Params(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+ENSEMBLE_SIDES = (128, 256, 384, 512, 768, 1024)
+
+
+def ensemble_spec(n_terrains: int = 64, seed: int = 1) -> list:
+    """The batched-ensemble workload: ``n_terrains`` terrains with sides drawn
+    uniformly from ENSEMBLE_SIDES, each with its own fBm seed (also its
+    World::SEED), evapRate uniform in [0.0005, 0.002], depositionRate uniform
+    in [0.05, 0.2] and one drop per 16 cells.  Plain dicts {nx, ny, drops,
+    seed, evapRate, depositionRate}; the draws come from numpy's PCG64 stream
+    of ``seed`` in that order, terrain by terrain."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_terrains):
+        s = int(rng.choice(ENSEMBLE_SIDES))
+        e = float(rng.uniform(0.0005, 0.002))
+        dr = float(rng.uniform(0.05, 0.2))
+        out.append(dict(nx=s, ny=s, drops=s * s // 16, seed=int(rng.integers(1, 2**62)),
+                        evapRate=e, depositionRate=dr))
+    return out
+
+
 class MapHeader(C.Structure):
     """shy_map_header of include/shy.h (72 bytes)."""
     _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("nx", C.c_int32),

@@ -144,10 +144,17 @@ __device__ inline float inv_len(float fx, float fy) {
 
 // a / b, correctly rounded, given y = 1.0f / b (correctly rounded): q = a y is
 // within an ulp of a/b, the remainder r = a - b q is exact in one FMA, and
-// RN(q + r y) is the correctly rounded quotient (Markstein's theorem; no
-// overflow or underflow for the operands here: |a| is 0 or above 1e-30 and
-// b in (0, 1]).  r = 0 means q is exact, which keeps the sign of a zero.
-__device__ inline float div_rcp(float a, float b, float y) {
+// RN(q + r y) is the correctly rounded quotient (Markstein's theorem).  The
+// theorem needs q, r and y in the normal range: with 2^-60 <= |a| <= 2^60 and
+// 2^-60 <= b <= 2^60 they are (|q| <= 2^120, |r| >= 2^-84).  Outside that
+// range (a tiny slope, or a volume driven to the extremes by minVol <= 0 or
+// a negative evapRate) the plain division runs instead.  r = 0 means q is
+// exact, which keeps the sign of a zero.
+__host__ __device__ inline float div_rcp(float a, float b, float y) {
+  const float aa = __builtin_fabsf(a);
+  const bool in_range = (aa >= 0x1p-60f || aa == 0.0f) && aa <= 0x1p60f && b >= 0x1p-60f &&
+                        b <= 0x1p60f;
+  if (!in_range) return a / b;
   const float q = a * y;
   const float r = __builtin_fmaf(-q, b, a);
   return r == 0.0f ? q : __builtin_fmaf(r, y, q);
@@ -186,7 +193,7 @@ __device__ inline void step_motion(const shy_params& P, float n0, float n2, floa
 }
 
 // water.h:135-136, evaluated in double as the reference's 1.0-evapRate is.
-__device__ inline float evap_sed(float sed, float e) {
+__host__ __device__ inline float evap_sed(float sed, float e) {
   return (float)((double)sed / (1.0 - (double)e));
 }
 
@@ -194,17 +201,26 @@ __device__ inline float evap_sed(float sed, float e) {
 // y = 1.0 / c (correctly rounded).  Markstein's theorem: with y within half
 // an ulp of 1/c and q = RN(a y) within one ulp of a/c, the remainder
 // r = a - c q is exact in one FMA and RN(q + r y) is the correctly rounded
-// a/c (no overflow or underflow: a is a float, c is in (0, 1]).  Three
-// dependent f64 operations instead of the division's sequence.
-__device__ inline float evap_sed_rcp(float sed, double c, double y) {
+// a/c (no overflow or underflow in double: a is a float and c = 1 - e with e
+// a float below 1, so 2^-24 <= c <= 2^129).  Three dependent f64 operations
+// instead of the division's sequence.
+__host__ __device__ inline float evap_sed_rcp(float sed, double c, double y) {
   const double a = (double)sed;
   const double q = a * y;
   const double r = __builtin_fma(-q, c, a);
   return (float)__builtin_fma(r, y, q);
 }
-__device__ inline float evap_vol(float vol, float e) {
+__host__ __device__ inline float evap_vol(float vol, float e) {
   return (float)((double)vol * (1.0 - (double)e));
 }
+
+// One coordinate of ivec2(pos): truncation toward zero, as glm's vec2 ->
+// ivec2 conversion.  A position is finite and within maxAge * sqrt(2) of its
+// spawn unless the speed became NaN (a volume that underflowed to 0 with
+// minVol <= 0 makes the gravity term infinite); a NaN coordinate names no
+// cell (-1, off the map), where the reference's x86 conversion (INT_MIN)
+// sends it too.
+__host__ __device__ inline int cell_coord(float p) { return p == p ? (int)p : -1; }
 
 __device__ inline bool off_map(int x, int y, int nx, int ny) {
   return x < 0 || y < 0 || x >= nx || y >= ny;

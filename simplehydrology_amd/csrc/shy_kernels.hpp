@@ -324,7 +324,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       step_motion_rf(P, n0, n2, vol, yv, c.d, c.mx, c.my, c.rf, sx, sy, px, py);
     else
       step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);
-    const int jx = (int)px, jy = (int)py;
+    const int jx = cell_coord(px), jy = cell_coord(py);
     const bool ex = off_map(jx, jy, nx, ny);
     const int j = jx * ny + jy;
     // off the map the record is not used; loading the current cell's keeps
@@ -640,12 +640,18 @@ __device__ inline float powi_f32(float r, int k) {
   return res;
 }
 
-// Burst scale of a cell whose count pass saw k walk steps start there.
+// Burst scale of a cell whose count pass saw k walk steps start there.  One
+// step moves the cell a fraction g = effD (1 + entrainment erf(0.4 d)) of its
+// distance to equilibrium (water.h:127-132), k steps in sequence
+// R = 1 - (1-g)^k of it; the k wave commits are scaled to move it R in total.
+// A wave updates a cell and its downstream neighbour from one snapshot (a
+// Jacobi step), which is stable only while R <= 1, so for g > 1 the k commits
+// together move the cell exactly to the snapshot's equilibrium (R = 1).
 __device__ inline float burst_scale(int k, float effD, float entrainment, float erfd) {
-  if (k <= 1) return 1.0f;
   float g = effD * (1.0f + entrainment * erfd);
   if (!(g > 0.0f)) return 1.0f;
-  if (g > 1.0f) g = 1.0f;
+  if (g > 1.0f) return 1.0f / ((float)(k < 1 ? 1 : k) * g);
+  if (k <= 1) return 1.0f;
   return (1.0f - powi_f32(1.0f - g, k)) / (g * (float)k);
 }
 
@@ -1823,6 +1829,32 @@ __global__ void fbm_kernel(float* out, int nx, int ny, const uint8_t* __restrict
 }
 
 // ------------------------------------------- the reference's own terrain
+// The OpenSimplex2 restatement below follows FastNoiseLite, which carries
+// this notice:
+//
+//   MIT License
+//
+//   Copyright(c) 2020 Jordan Peck (jordan.me2@gmail.com)
+//   Copyright(c) 2020 Contributors
+//
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files(the "Software"),
+//   to deal in the Software without restriction, including without limitation
+//   the rights to use, copy, modify, merge, publish, distribute, sublicense,
+//   and / or sell copies of the Software, and to permit persons to whom the
+//   Software is furnished to do so, subject to the following conditions :
+//
+//   The above copyright notice and this permission notice shall be included in
+//   all copies or substantial portions of the Software.
+//
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR
+//   IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF MERCHANTABILITY,
+//   FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT.IN NO EVENT SHALL THE
+//   AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER
+//   LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING
+//   FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE USE OR OTHER
+//   DEALINGS IN THE SOFTWARE.
+//
 // quad::map::init (cellpool.h:349-409): eight layers of FastNoiseLite noise
 // (OpenSimplex2, 3D, FBm of 3 octaves, seed 1337, lacunarity 2, gain 0.5,
 // bounding 1/1.75) at frequencies 1, 2, 4, ... and weights 0.6, 0.36, ...,
@@ -2009,7 +2041,7 @@ __global__ void normalize_kernel(float* v, size_t n, const float2* part) {
 // ------------------------------------------ single-site reference kernels
 // World::cascade(pos) on the live map (world.h:90-168).
 __device__ void cascade_live(float* h, int nx, int ny, const shy_params& P, float fx, float fy) {
-  const int cx = (int)fx, cy = (int)fy;
+  const int cx = cell_coord(fx), cy = cell_coord(fy);
   if (off_map(cx, cy, nx, ny)) return;
   const int ci = cx * ny + cy;
   // neighbour heights are captured before the loop (world.h:116-125); the
@@ -2051,13 +2083,14 @@ struct LiveArgs {
 
 // One Drop::descend() call on the live map (water.h:58-156), one thread:
 // writes go straight to the map and the cascade runs inline, as in the
-// reference.  Tracks accumulate in fixed point (2^-32).
+// reference.  Tracks accumulate in fixed point (int32 steps of 2^-24 summed
+// into int64).
 __global__ void descend_live_kernel(LiveArgs a, shy_drop* drop, int* alive) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   shy_drop D = *drop;
   const shy_params& P = a.P;
   const int nx = a.nx, ny = a.ny;
-  const int ix = (int)D.pos[0], iy = (int)D.pos[1];
+  const int ix = cell_coord(D.pos[0]), iy = cell_coord(D.pos[1]);
   *alive = 0;
   if (off_map(ix, iy, nx, ny)) return;  // water.h:62-68
   const int i0 = ix * ny + iy;
@@ -2083,7 +2116,7 @@ __global__ void descend_live_kernel(LiveArgs a, shy_drop* drop, int* alive) {
   a.td[i0] += quant_t(D.volume);
   a.tmx[i0] += quant_t(D.volume * sx);
   a.tmy[i0] += quant_t(D.volume * sy);
-  const int jx = (int)px, jy = (int)py;
+  const int jx = cell_coord(px), jy = cell_coord(py);
   const bool ex = off_map(jx, jy, nx, ny);
   const float h2 = ex ? (float)((double)a.h[i0] - 0.002) : a.h[jx * ny + jy];
   const float E = erf_f32(a.erf_a, 0.4f * f.z);

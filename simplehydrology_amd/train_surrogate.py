@@ -348,6 +348,85 @@ def evaluate(net, n=2048, seed=7, c0=10, K=20, W=10, ref=(200, 400), warm_cycles
     return res
 
 
+def steady_profile(n=2048, seed=7, cycles=1000, windows=(10, 50, 100, 200), ref_window=200,
+                   deposition=None, net=None, c0=10, K=20, gate_l1=0.02, log=print):
+    """How close the plain method's discharge comes to steady, measured on
+    one terrain run cold for ``cycles`` cycles (erosion on unless
+    ``deposition`` is given: default parameters, n*n/16 drops per cycle, the
+    cascade in every cycle).
+
+    D(c) = erf(0.4 d) after cycle c (the value node::discharge returns,
+    source/cellpool.h:242-244).  Q = the mean of D over the last
+    ``ref_window`` cycles.  For each trailing window W, M_W(c) = the mean of
+    D over cycles (c-W, c]; its error is L1(M_W(c), Q) / L1(0, Q), and its
+    change per cycle is L1(M_W(c), M_W(c-1)) / L1(0, M_W(c)).  Only cycles
+    c <= cycles - ref_window count for the floor (later windows overlap Q).
+
+    With ``net``, the warm start runs too: ``c0`` plain cycles from the same
+    state, window-wise inference, import, ``K`` verification cycles, and the
+    gate at ``gate_l1``; its error is measured against the same Q."""
+    import torch
+    drops = n * n // 16
+    p = Params()
+    if deposition is not None:
+        p.depositionRate = deposition
+    w = World(n, n, seed=seed, params=p)
+    w.generate_fbm(seed, 8)
+    rec = Recorder(w)
+    Ds = torch.empty(cycles, n, n, dtype=torch.float16, device="cuda:0")
+    state_c0 = None
+    h0 = rec.grab()[..., 0].clone()
+    t = Timer()
+    for c in range(cycles):
+        w.erode_cycles(1, drops, graph=True)
+        Ds[c] = erf_discharge(rec.grab()[..., 1]).half()
+        if c + 1 == c0:
+            state_c0 = w.cells()
+    run_ms = t.ms()
+    h1 = rec.grab()[..., 0]
+    height_change = l1_rel(h1, h0)
+    w.close()
+    Q = Ds[cycles - ref_window:].float().mean(0)
+    last = cycles - ref_window
+    prof = {}
+    for W in windows:
+        errs, chg = {}, {}
+        for c in range(max(W, 10), cycles + 1, 10):
+            m = Ds[c - W:c].float().mean(0)
+            errs[c] = l1_rel(m, Q)
+            mp = Ds[c - 1 - W:c - 1].float().mean(0) if c - 1 - W >= 0 else None
+            if mp is not None:
+                chg[c] = float((m - mp).abs().sum() / m.abs().sum())
+        floor_c = min((c for c in errs if c <= last), key=lambda c: errs[c])
+        prof[W] = {"error": errs, "change_per_cycle": chg,
+                   "reachable_min_l1": errs[floor_c], "at_cycle": floor_c,
+                   "first_below_2pct": next((c for c in sorted(errs) if c <= last
+                                             and errs[c] <= 0.02), None)}
+    per_cycle = [float((Ds[c].float() - Ds[c - 1].float()).abs().sum() / Ds[c].float().abs().sum())
+                 for c in range(1, cycles, max(1, cycles // 100))]
+    res = {"n": n, "seed": seed, "drops_per_cycle": drops, "cycles": cycles,
+           "deposition": p.depositionRate, "ref_window": [last, cycles],
+           "ms_per_cycle": run_ms / cycles, "height_l1_change": height_change,
+           "per_cycle_change": {"min": min(per_cycle), "last": per_cycle[-1]},
+           "windows": prof}
+    if net is not None:
+        w2 = World(n, n, seed=seed, params=p)
+        w2.upload_cells(state_c0)
+        w2.set_seed(seed, c0)
+        st = Recorder(w2).grab()
+        sg.predict_tiled(net, st[..., 0], st[..., 1], st[..., 2], st[..., 3])
+        torch.cuda.synchronize()
+        ws = warm_start(net, w2, drops, seed, K=K, W=10, gate_l1=gate_l1)
+        wm = ws.pop("window_mean")
+        res["warm_start"] = {"gate": ws, "error_after_verification": l1_rel(wm, Q),
+                             "cold_error_same_cycle": prof[10]["error"].get(c0 + K)}
+        w2.close()
+    del Ds
+    torch.cuda.empty_cache()
+    log(json.dumps({k: v for k, v in res.items() if k != "windows"}))
+    return res
+
+
 def save_weights(net, path=WEIGHTS):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.savez_compressed(path, **net.state_dict())

@@ -132,8 +132,11 @@ def test_cpp_facade_demo_matches_python_path(shy):
     import subprocess
     demo = os.path.join(os.path.dirname(shy.LIB_PATH), "world_demo")
     n, cycles, drops, seed = 64, 6, 512, 3
+    # the demo reads getCell(p + vec2(dx, dy)) at p = (x + 0.5, y + 0.25): the
+    # vec2 -> ivec2 conversion truncates toward zero as glm's does, so
+    # x + 0.5 - 1 = -0.5 at x = 0 names cell 0 (a floor would leave the map)
     inb = sum(1 for x in range(n) for y in range(n) for dx in (-1, 0, 1) for dy in (-1, 0, 1)
-              if (dx or dy) and 0 <= x + dx < n and 0 <= y + dy < n)
+              if (dx or dy) and 0 <= int(x + 0.5 + dx) < n and 0 <= int(y + 0.25 + dy) < n)
     for terrain, tileres in (("reference", 512), ("fbm", 32), ("reference", 24)):
         print("demo", terrain, tileres, flush=True)
         r = subprocess.run([demo, str(n), str(cycles), str(drops), str(seed), terrain,

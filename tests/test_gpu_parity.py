@@ -136,6 +136,32 @@ def test_wave_cycles_bit_exact(shy, orc, n, drops, cycles):
     w.close()
 
 
+@pytest.mark.parametrize("kw", [
+    dict(minVol=1e-37, evapRate=0.2),  # volumes run down to 1e-37
+    dict(gravity=1e-25),               # gravity terms below 2^-60
+    dict(evapRate=-0.1),               # volumes grow past 2^60
+])
+def test_wave_cycles_extreme_parameters(shy, orc, kw):
+    """The walk divides the gravity term by the volume through the volume's
+    reciprocal; where the operands leave the range in which that is exact,
+    it divides plainly.  Parameters that drive them there still give the
+    oracle's bits (the oracle divides plainly everywhere)."""
+    n, drops, seed = 64, 256, 3
+    h = orc.fbm(seed, n, n)
+    z = np.zeros((n, n), np.float32)
+    d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+    P = orc.Params.default(**kw)
+    w = shy.World(n, n, seed=seed)
+    w.params = shy.Params(**kw)
+    w.upload("height", h)
+    for c in range(2):
+        say("extreme parameters", kw, "cycle", c)
+        w.erode(drops)
+        orc.wave_cycle(h, d, mx, my, root, erfd, P, seed, c, drops)
+        assert_world_equals(w, h, d, mx, my, erfd, f"{kw} cycle={c}")
+    w.close()
+
+
 def test_tracks_bit_exact(shy, orc):
     """The cycle's discharge/momentum tracks (water.h:115-117), read back
     before the next cycle, equal the oracle's integer track sums; on an

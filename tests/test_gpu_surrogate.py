@@ -1,7 +1,7 @@
 """GPU: what the trained surrogate gains, and the warm-start gate.
 
-The network (simplehydrology_amd/weights, trained by train_surrogate on 40
-simulator-made 2048x2048 flow maps) maps the state after 10 cold cycles to
+The network (simplehydrology_amd/weights, trained by train_surrogate on 80
+simulator-made 2048x2048 flow maps, 2 more held out) maps the state after 10 cold cycles to
 the long-run mean discharge/momentum.  These tests run on a held-out terrain
 of the training statistics (fBm base frequency 1/2048) at 1024x1024.
 """

@@ -279,15 +279,43 @@ def test_wave_mass_ledger_B5():
 
 # ------------------------------------------------------------------ wave
 def test_burst_scale_closed_forms():
+    """k steps through a cell with feedback g move it R = 1 - (1-g)^k of the
+    way to equilibrium in sequence; the scaled wave commits move it k s g.
+    For g > 1 the wave's total is capped at R = 1."""
     f = O.lib().orc_burst_scale
-    assert f(0, 0.1, 10.0, 0.5) == 1.0 and f(1, 0.1, 10.0, 1.0) == 1.0
+    assert f(0, 0.1, 10.0, 0.5) == 1.0 and f(1, 0.1, 10.0, 0.5) == 1.0  # g = 0.6
+    for k in (1, 2, 7, 100):
+        g = 0.1 * (1 + 10 * 1.0)  # 1.1
+        assert math.isclose(k * f(k, 0.1, 10.0, 1.0) * g, 1.0, rel_tol=1e-6)
+        g = 0.2 * (1 + 10 * 0.8)  # 1.8
+        assert math.isclose(k * f(k, 0.2, 10.0, 0.8) * g, 1.0, rel_tol=1e-6)
     for k in (2, 7, 100):
-        assert math.isclose(f(k, 0.1, 10.0, 1.0), 1.0 / k, rel_tol=1e-6)  # g >= 1
         g = 0.1 * (1 + 10 * 0.05)
         s = f(k, 0.1, 10.0, 0.05)
         assert math.isclose(k * s * g, 1 - (1 - g) ** k, rel_tol=1e-5)
         assert f(k, 0.0, 10.0, 0.5) == 1.0
     assert f(5, 0.1, 10.0, 0.0) > f(50, 0.1, 10.0, 0.0)
+
+
+def test_wave_stable_at_high_deposition_rate():
+    """With depositionRate 0.197 one step's feedback g reaches 2.2 in wet
+    cells.  The sequential reference (a Gauss-Seidel sweep) stays bounded;
+    a wave commits every cell from one snapshot (a Jacobi step), which
+    diverged to heights of -190..100 within 20 cycles until its relaxation
+    per cell was capped at R = 1.  Now both stay in the terrain's range."""
+    n, drops, seed, t = 128, 1024, 2345, 6
+    P2 = O.Params.default(evapRate=0.0012277864616474527, depositionRate=0.19711057997018583)
+    h0 = O.fbm(seed, n, n)
+    h = h0.copy()
+    z = np.zeros_like(h)
+    d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
+    cells = O.to_aos(h0)
+    for c in range(20):
+        O.wave_cycle(h, d, mx, my, root, erfd, P2, seed, c, drops, terrain=t)
+        O.seq_erode(cells, P2, seed, c, drops)
+    hs = cells[..., 0]
+    assert -0.1 < hs.min() and hs.max() <= 1.0
+    assert -0.1 < h.min() and h.max() <= 1.0
 
 
 def test_one_drop_wave_matches_frozen_trace():
@@ -616,20 +644,40 @@ def _drain(d):
 def test_sequential_vs_wave_terrain_statistics():
     """Sequential reference vs wave-synchronous terrains, 128x128, 1 drop per
     16 cells, 40 cycles: hypsometric curve, drainage-area histogram (of the
-    discharge field), height correlation; both stay bounded."""
+    discharge field) and height correlation, each against the sequential
+    method's own spread (the mean over pairs of three spawn streams).  The wave's gap measures
+    9x that spread in hypsometry, 4x in drainage and 2.5x in (1 - height
+    correlation); the bounds leave a margin of a third.  Both stay bounded."""
     n, cyc = 128, 40
     drops = n * n // 16
     h0 = O.fbm(3, n, n)
-    cells = O.to_aos(h0)
-    for k in range(cyc):
-        O.seq_erode(cells, P, 1, k, drops)
-    hs, ds = cells[..., 0], cells[..., 1]
+    seq = []
+    for seed in (1, 2, 3):
+        cells = O.to_aos(h0)
+        for k in range(cyc):
+            O.seq_erode(cells, P, seed, k, drops)
+        seq.append((cells[..., 0].copy(), cells[..., 1].copy()))
     h = h0.copy()
     z = np.zeros_like(h)
     d, mx, my, root, erfd = z.copy(), z.copy(), z.copy(), z.copy(), z.copy()
     for k in range(cyc):
         O.wave_cycle(h, d, mx, my, root, erfd, P, 1, k, drops)
-    assert -0.5 < h.min() and h.max() < 1.5 and -0.5 < hs.min() and hs.max() < 1.5
-    assert np.abs(_hyps(h) - _hyps(hs)).mean() < 0.05
-    assert np.abs(_drain(d) - _drain(ds)).sum() < 0.3
-    assert np.corrcoef(h.ravel(), hs.ravel())[0, 1] > 0.9
+    assert -0.5 < h.min() and h.max() < 1.5
+    assert all(-0.5 < a.min() and a.max() < 1.5 for a, _ in seq)
+
+    def mean_pairs(f):
+        return np.mean([f(seq[i], seq[j]) for i, j in ((0, 1), (0, 2), (1, 2))])
+
+    def mean_vs_wave(f):
+        return np.mean([f((h, d), s) for s in seq])
+
+    hyp = lambda a, b: np.abs(_hyps(a[0]) - _hyps(b[0])).mean()  # noqa: E731
+    drn = lambda a, b: np.abs(_drain(a[1]) - _drain(b[1])).sum()  # noqa: E731
+    cor = lambda a, b: 1.0 - np.corrcoef(a[0].ravel(), b[0].ravel())[0, 1]  # noqa: E731
+    spread_h, spread_d, spread_c = mean_pairs(hyp), mean_pairs(drn), mean_pairs(cor)
+    gap_h, gap_d, gap_c = mean_vs_wave(hyp), mean_vs_wave(drn), mean_vs_wave(cor)
+    print(f"hypsometry {gap_h:.4f} vs spread {spread_h:.4f}; drainage {gap_d:.3f} vs "
+          f"{spread_d:.3f}; 1-corr {gap_c:.4f} vs {spread_c:.4f}")
+    assert gap_h <= 12.0 * spread_h
+    assert gap_d <= 6.0 * spread_d
+    assert gap_c <= 4.0 * spread_c
