@@ -159,16 +159,19 @@ def ensemble_specs(shy, n_terrains: int, seed: int):
     return specs
 
 
-def run_ensemble(shy, terrains: int, seed: int, cycles: int, warmup: int, autotune: bool) -> dict:
+def run_ensemble(shy, terrains: int, seed: int, cycles: int, warmup: int, autotune) -> dict:
     """The batched ensemble: ``warmup`` graph-replayed cycles, then ``cycles``
     timed cycles with HIP events between kernels (after the autotuner picked
-    the launch shape, when ``autotune``)."""
+    the launch shape when ``autotune`` is true; a dict sets the shape)."""
     specs = ensemble_specs(shy, terrains, seed)
     e = shy.Ensemble(specs)
     for t, s in enumerate(specs):
         e.generate_fbm(t, s["seed"], 8)
     tuned, tuned_ms = None, None
-    if autotune:
+    if isinstance(autotune, dict):
+        e.set_tuning(**autotune)
+        tuned = autotune
+    elif autotune:
         tuned, tuned_ms = e.autotune(2)
     if warmup:
         e.erode(warmup, graph=True)
@@ -198,30 +201,61 @@ def run_ensemble(shy, terrains: int, seed: int, cycles: int, warmup: int, autotu
 
 
 def surrogate_leg(seed: int, size: int) -> dict:
-    """Time-to-steady discharge, warm-started (trained network, committed
-    weights) vs cold plain GPU run on a fixed fBm terrain of side ``size``;
-    plus the MFMA rate of the network's convolutions (forward and one
-    training step on a batch of 16 windows of 256x256)."""
+    """The second half of the metric on the stated workload: a size x size
+    fBm terrain, default parameters (erosion on), size^2/16 drops per cycle,
+    1000 cycles with the cascade.  The cold run's error (trailing W-cycle
+    mean of erf(0.4 d) against the mean over its last 200 cycles) for W in
+    {10, 50, 100, 200}; the warm start (committed network, K = 20
+    verification cycles, gate at 2% relative L1); time-to-steady at 2% only
+    where some window reaches it.  A second block repeats the earlier
+    measurement on a fixed terrain (deposition rate 0) at matched error,
+    labelled as not meeting the stated target.  Plus the network's MFMA rate
+    (forward and one training step on a batch of 16 windows of 256x256)."""
     from simplehydrology_amd import surrogate as sg
     from simplehydrology_amd import train_surrogate as ts
     net = ts.load_weights(sg.UNet(seed=0), ts.WEIGHTS)
     tt = ts.train_timing(sg.UNet(seed=0), batch=16)
+    prof = ts.steady_profile(n=size, seed=seed, cycles=1000, net=net, gate_l1=0.02,
+                             log=lambda *_: None)
+    floors = {f"W{w}": v["reachable_min_l1"] for w, v in prof["windows"].items()}
+    first = {f"W{w}": v["first_below_2pct"] for w, v in prof["windows"].items()}
+    steady = {f"W{w}": v["first_steady"] for w, v in prof["windows"].items()}
+    reached = [c for c in first.values() if c is not None]
+    ws = prof["warm_start"]
+    met = bool(reached) and ws["gate"]["accepted"]
     r = ts.evaluate(net, n=size, seed=seed, c0=10, K=20, W=10, ref=(200, 400), warm_cycles=0,
                     log=lambda *_: None)
     fwd = tt["forward_tflops"]
     return {
         "time_to_steady": {
-            "definition": "wall time until the trailing 10-cycle mean of erf(0.4 d) reaches "
-                          "the warm run's relative L1 error (vs the cold run's mean over "
-                          "cycles 200-400) after its 20 verification cycles; warm = 10 plain "
-                          "cycles + window-wise inference + 20 verification cycles",
-            "terrain": f"{size}x{size} fBm seed {seed}, deposition 0, {size * size // 16} "
-                       "drops/cycle",
-            "matched_l1": r["e_warm"][30], "cold_cycles": r["cold_cycles_to_match"],
-            "cold_ms": r["cold_ms_to_match"], "warm_ms": r["warm_ms"],
-            "speedup": r["speedup"], "gate": r["gate"], "cycle_ms": r["cycle_ms"]},
+            "workload": f"{size}x{size} fBm seed {seed}, default parameters (erosion on), "
+                        f"{size * size // 16} drops/cycle, 1000 cycles, cascade every cycle",
+            "target": "discharge L1 error <= 2% vs the long cold-start field, warm start "
+                      "rejected above 2% after K=20 verification cycles, >= 3x shorter "
+                      "time-to-steady than the cold plain GPU run",
+            "error": "relative L1 of the trailing W-cycle mean of erf(0.4 d) against the "
+                     "mean over cycles 800-1000 of the cold run",
+            "reachable_min_l1": floors, "first_cycle_below_2pct": first,
+            "first_cycle_steady": steady,
+            "steady_definition": "trailing W-cycle mean changes by < 1e-3 relative L1 per cycle",
+            "meets_target": met,
+            "warm_start": {"accepted": ws["gate"]["accepted"], "gate_l1": ws["gate"]["gate_l1"],
+                           "gate_threshold": ws["gate"]["gate_threshold"],
+                           "error_after_verification": ws["error_after_verification"],
+                           "cold_error_same_cycle": ws["cold_error_same_cycle"],
+                           "infer_ms": ws["gate"]["infer_ms"], "K": ws["gate"]["K"]},
+            "speedup": None,
+            "cycle_ms": prof["ms_per_cycle"], "height_l1_change": prof["height_l1_change"],
+            "fixed_terrain_matched_error": {
+                "note": "not the stated target: deposition rate 0 (heights fixed), error "
+                        "matched at the warm run's level (~0.21, not <= 0.02), gate 0.35",
+                "terrain": f"{size}x{size} fBm seed {seed}, deposition 0, "
+                           f"{size * size // 16} drops/cycle",
+                "matched_l1": r["e_warm"][30], "cold_cycles": r["cold_cycles_to_match"],
+                "cold_ms": r["cold_ms_to_match"], "warm_ms": r["warm_ms"],
+                "speedup": r["speedup"], "gate": r["gate"], "cycle_ms": r["cycle_ms"]}},
         "surrogate_roofline": {
-            "bound": "mfma", "kernel": "conv_lds_kernel (U-Net forward, batch 16 x 256^2)",
+            "bound": "mfma", "kernel": "conv_pf_kernel (U-Net forward, batch 16 x 256^2)",
             "achieved": fwd, "peak": 2500.0, "unit": "TFLOP/s", "frac": fwd / 2500.0,
             "train_step_tflops": tt["train_tflops"], "forward_gflop": tt["forward_gflop"]},
     }
@@ -247,6 +281,9 @@ def main():
                     help="leave the ensemble block out of the default line")
     ap.add_argument("--ensemble-steps", type=int, default=20,
                     help="timed cycles of the ensemble block")
+    ap.add_argument("--tuning", default="",
+                    help="ensemble launch shape walk_block,walk_grid,tile instead of the "
+                         "autotuner's (for profiling runs)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the time-to-steady (surrogate vs plain) leg")
     ap.add_argument("--surrogate-size", type=int, default=2048)
@@ -273,9 +310,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    tune = not args.no_autotune
+    if args.tuning:
+        wb, wg, tl = (int(v) for v in args.tuning.split(","))
+        tune = dict(walk_block=wb, walk_grid=wg, tile=tl)
     if args.workload == "ensemble":
         print(json.dumps(run_ensemble(shy, args.terrains, args.seed, args.steps, args.warmup,
-                                      not args.no_autotune)))
+                                      tune)))
         return
 
     n, drops = args.size, args.drops
@@ -319,8 +360,7 @@ def main():
     }
     w.close()
     if not args.no_ensemble:
-        ens = run_ensemble(shy, args.terrains, args.seed, args.ensemble_steps, 5,
-                           not args.no_autotune)
+        ens = run_ensemble(shy, args.terrains, args.seed, args.ensemble_steps, 5, tune)
         out["ensemble"] = {k: ens[k] for k in (
             "value", "unit", "steps", "warmup", "ms_per_step", "cycles_per_s", "particle_steps",
             "config", "kernel_ms_per_cycle", "roofline")}

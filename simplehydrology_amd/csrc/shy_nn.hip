@@ -237,6 +237,151 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(ConvArgs a) {
   }
 }
 
+// The same convolution with the next 32-channel chunk's patch and weights
+// loaded into registers while the current chunk's MFMAs run (the global
+// loads' latency hides behind them; LDS holds one chunk), and each patch row
+// read from LDS once per column shift dx and used for the three row shifts
+// dy (a wave's NT output rows need NT + 2 patch rows): per chunk and wave
+// 3 (NT + 2) B reads and 9 MT A reads feed 9 MT NT MFMAs.
+template <int TAPS, int MT, int NT>
+__global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
+  constexpr int kTH = 4 * NT;
+  constexpr int HALO = TAPS == 9 ? 1 : 0;
+  constexpr int PW = kTW + 2 * HALO, PH = kTH + 2 * HALO;
+  constexpr int WS = TAPS * 32 + 8;  // LDS bf16 per output channel of the weight chunk
+  constexpr int NPV = PH * PW * 4;             // 16-byte patch vectors per chunk
+  constexpr int NWV = MT * 16 * TAPS * 4;      // 16-byte weight vectors per chunk
+  constexpr int RP = (NPV + 255) / 256, RW = (NWV + 255) / 256;
+  constexpr int NS = 2 * HALO + 1;             // column (and row) shifts
+  __shared__ __bf16 patch[PH * PW * kPS];
+  __shared__ __bf16 wl[MT * 16 * WS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+  int b = blockIdx.x;
+  const int tx = b % tiles_x;
+  b /= tiles_x;
+  const int ty = b % tiles_y;
+  const int n = b / tiles_y;
+  const int x0 = tx * kTW, y0 = ty * kTH;
+  const int cb = blockIdx.y * MT * 16;
+  const int Cin = a.Cin, K = a.K;
+  bf16x8 rp[RP], rw[RW];
+  auto gload = [&](int ci0) {
+#pragma unroll
+    for (int j = 0; j < RP; ++j) {
+      const int v = threadIdx.x + 256 * j;
+      bf16x8 val = zero8();
+      if (v < NPV) {
+        const int pix = v >> 2, q = v & 3;
+        const int py = pix / PW, px = pix % PW;
+        const int gy = y0 + py - HALO, gx = x0 + px - HALO;
+        if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+          val = load8(a.x + ((size_t)(n * a.H + gy) * a.W + gx) * Cin + ci0 + q * 8);
+      }
+      rp[j] = val;
+    }
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+      const int v = threadIdx.x + 256 * j;
+      bf16x8 val = zero8();
+      if (v < NWV) {
+        const int co = v / (TAPS * 4), r = v % (TAPS * 4);
+        const int tap = r >> 2, q = r & 3;
+        if (cb + co < a.Cout) val = load8(a.w + (size_t)(cb + co) * K + tap * Cin + ci0 + q * 8);
+      }
+      rw[j] = val;
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < RP; ++j) {
+      const int v = threadIdx.x + 256 * j;
+      if (v < NPV)
+        *reinterpret_cast<bf16x8*>(&patch[(v >> 2) * kPS + (v & 3) * 8]) = rp[j];
+    }
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+      const int v = threadIdx.x + 256 * j;
+      if (v < NWV) {
+        const int co = v / (TAPS * 4), r = v % (TAPS * 4);
+        *reinterpret_cast<bf16x8*>(&wl[co * WS + (r >> 2) * 32 + (r & 3) * 8]) = rw[j];
+      }
+    }
+  };
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  gload(0);
+  lstore();
+  __syncthreads();
+  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+    const bool more = ci0 + 32 < Cin;
+    if (more) gload(ci0 + 32);
+#pragma unroll
+    for (int dx = 0; dx < NS; ++dx) {
+      bf16x8 B[NT + 2 * HALO];
+#pragma unroll
+      for (int r = 0; r < NT + 2 * HALO; ++r) {
+        const int row = wave * NT + r, col = (lane & 15) + dx;
+        B[r] = *reinterpret_cast<const bf16x8*>(&patch[(row * PW + col) * kPS + (lane >> 4) * 8]);
+      }
+#pragma unroll
+      for (int dy = 0; dy < NS; ++dy) {
+        const int tap = dy * NS + dx;
+        bf16x8 A[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          A[m] = *reinterpret_cast<const bf16x8*>(&wl[(m * 16 + (lane & 15)) * WS + tap * 32 +
+                                                      (lane >> 4) * 8]);
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[m][t] = mfma(A[m], B[t + dy], acc[m][t]);
+      }
+    }
+    __syncthreads();
+    if (more) {
+      lstore();
+      __syncthreads();
+    }
+  }
+  // epilogue: lane holds channels cb + 16m + 4(lane/16) + i of pixel (row, lane%16)
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
+    if (gy >= a.H || gx >= a.W) continue;
+    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int co = cb + m * 16 + (lane >> 4) * 4;
+      if (co >= a.Cout) continue;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
+      const size_t o = p * a.Cout + co;
+      if (a.res) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
+      }
+      if (a.relu) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+      }
+      if (a.y_f) {
+        *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 q;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
+        *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
+      }
+    }
+  }
+}
+
 // dgrad weights: Wf[ci][tap'*Cout + co] = W[co][(TAPS-1-tap')*Cin + ci]
 __global__ void flip_weights_kernel(const __bf16* __restrict__ w, __bf16* wf, int Cin, int Cout,
                                     int taps) {
@@ -776,6 +921,7 @@ int nn_check(const char* what) {
   return 0;
 }
 inline unsigned blocks(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
 }  // namespace
 
 extern "C" {
@@ -812,19 +958,19 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
     const dim3 g((unsigned)(big ? tiles16 : tiles8), gy);
     if (mt == 4) {
       if (big) {
-        if (taps == 9) conv_lds_kernel<9, 4, 4><<<g, 256, 0, s>>>(a);
-        else conv_lds_kernel<1, 4, 4><<<g, 256, 0, s>>>(a);
+        if (taps == 9) conv_pf_kernel<9, 4, 4><<<g, 256, 0, s>>>(a);
+        else conv_pf_kernel<1, 4, 4><<<g, 256, 0, s>>>(a);
       } else {
-        if (taps == 9) conv_lds_kernel<9, 4, 2><<<g, 256, 0, s>>>(a);
-        else conv_lds_kernel<1, 4, 2><<<g, 256, 0, s>>>(a);
+        if (taps == 9) conv_pf_kernel<9, 4, 2><<<g, 256, 0, s>>>(a);
+        else conv_pf_kernel<1, 4, 2><<<g, 256, 0, s>>>(a);
       }
     } else {
       if (big) {
-        if (taps == 9) conv_lds_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
-        else conv_lds_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
+        if (taps == 9) conv_pf_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
+        else conv_pf_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
       } else {
-        if (taps == 9) conv_lds_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
-        else conv_lds_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
+        if (taps == 9) conv_pf_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
+        else conv_pf_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
       }
     }
     return nn_check("conv_fwd");

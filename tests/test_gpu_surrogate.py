@@ -67,14 +67,20 @@ def test_trained_network_beats_the_state_it_starts_from(ts, trained):
 
 
 def test_gate_accepts_trained_and_rejects_untrained_with_restore(ts, trained):
+    """At a gate of 0.35 (the plain method's own spread against a smooth
+    field) the trained network passes and an untrained one fails; at the
+    stated 2% gate the trained one is rejected too.  A rejection restores the
+    plain run's state bit for bit."""
     from simplehydrology_amd import surrogate as sg
     drops = N * N // 16
-    for net, expect in ((trained, True), (sg.UNet(seed=3), False)):
-        print("warm start, trained" if expect else "warm start, untrained", flush=True)
+    assert ts.GATE_L1 == 0.02
+    for net, gate, expect in ((trained, 0.35, True), (sg.UNet(seed=3), 0.35, False),
+                              (trained, ts.GATE_L1, False)):
+        print("warm start", "trained" if net is trained else "untrained", gate, flush=True)
         w = ts.flow_world(N, SEED, base_freq=1.0 / 2048)
         w.erode_cycles(C0, drops, graph=True)
         before, cyc = w.cells(), w.cycle
-        r = ts.warm_start(net, w, drops, SEED, K=20, W=10)
+        r = ts.warm_start(net, w, drops, SEED, K=20, W=10, gate_l1=gate)
         print({k: v for k, v in r.items() if k != "window_mean"}, flush=True)
         assert r["accepted"] == expect
         if expect:
