@@ -37,6 +37,22 @@ int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, vo
    part holds part_splits*Cout*taps*Cin floats of scratch */
 int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int part_splits,
                       int N, int H, int W, int Cin, int Cout, int taps, void* stream);
+/* The same weight gradient with no workspace argument and no allocation.
+   No counterpart in the reference (nor in shy_nn_conv_wgrad's signature):
+   for callers that cannot allocate device memory, such as code under HIP
+   graph capture or a serving loop.  The split-K partials live in a
+   file-scope device array of SHY_NN_NOALLOC_FLOATS floats (16 MiB) that
+   every device the library runs on holds until the process ends; the call
+   itself never calls hipMalloc or hipFree.  Equals shy_nn_conv_wgrad with
+   part_splits = min(SHY_NN_NOALLOC_FLOATS / (Cout*taps*Cin),
+   SHY_NN_NOALLOC_MAX_SPLITS) bit for bit.  Fails (SHY_NN_ERR) when
+   Cout*taps*Cin exceeds SHY_NN_NOALLOC_FLOATS.  All calls on one device share
+   the array: calls that overlap on one GPU (other streams or threads) race,
+   so serialise them. */
+#define SHY_NN_NOALLOC_FLOATS 4194304
+#define SHY_NN_NOALLOC_MAX_SPLITS 1024
+int shy_nn_conv_wgrad_noalloc(const void* x, const void* g, float* dw, int N, int H, int W,
+                              int Cin, int Cout, int taps, void* stream);
 /* db[c] = sum over pixels of g[p][c]; part holds part_rows*C floats */
 int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P, int C,
                      void* stream);

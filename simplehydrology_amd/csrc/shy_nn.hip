@@ -24,9 +24,17 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/shy_nn.h"
+
+// Split-K partials of shy_nn_conv_wgrad_noalloc: SHY_NN_NOALLOC_FLOATS
+// floats (16 MiB) on every device the library runs on, held until the
+// process ends.  On the host this name is a stand-in, not a device address:
+// host code uses only the address hipGetSymbolAddress returns for the
+// current device (noalloc_scratch below).
+static __device__ __align__(256) float scratch[SHY_NN_NOALLOC_FLOATS];
 
 namespace shy_nn {
 
@@ -922,6 +930,29 @@ int nn_check(const char* what) {
 }
 inline unsigned blocks(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// Device address of `scratch` on the current device: looked up once per
+// device (on that device's first allocation-free call) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+int noalloc_scratch(float** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nn_fail(std::string("noalloc: hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return nn_fail("noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return nn_fail(std::string("noalloc: hipGetSymbolAddress(scratch): ") +
+                     hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -1046,6 +1077,21 @@ int shy_nn_conv_wgrad(const void* x, const void* g, float* dw, float* part, int 
   const int n = Cout * a.K;
   wgrad_reduce_kernel<<<(n + 31) / 32, 256, 0, s>>>(part, dw, n, splits);
   return nn_check("wgrad_reduce");
+}
+
+int shy_nn_conv_wgrad_noalloc(const void* x, const void* g, float* dw, int N, int H, int W,
+                              int Cin, int Cout, int taps, void* stream) {
+  if (Cin < 1 || Cout < 1 || (taps != 9 && taps != 1))
+    return nn_fail("conv_wgrad_noalloc: bad shape");
+  const size_t per = (size_t)Cout * taps * Cin;
+  if (per > (size_t)SHY_NN_NOALLOC_FLOATS)
+    return nn_fail("conv_wgrad_noalloc: Cout*taps*Cin = " + std::to_string(per) +
+                   " floats exceeds the scratch of " + std::to_string(SHY_NN_NOALLOC_FLOATS));
+  float* part = nullptr;
+  if (int r = noalloc_scratch(&part)) return r;
+  const int splits = (int)std::min<size_t>((size_t)SHY_NN_NOALLOC_FLOATS / per,
+                                           (size_t)SHY_NN_NOALLOC_MAX_SPLITS);
+  return shy_nn_conv_wgrad(x, g, dw, part, splits, N, H, W, Cin, Cout, taps, stream);
 }
 
 int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P, int C,

@@ -43,6 +43,7 @@ def nn():
                                       vp]),
             "shy_nn_flip_weights": (i32, [vp, vp, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
+            "shy_nn_conv_wgrad_noalloc": (i32, [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_bias_grad": (i32, [vp, vp, vp, i32, i32, i32, vp]),
             "shy_nn_relu_bwd": (i32, [vp, vp, vp, sz, vp]),
             "shy_nn_pool2": (i32, [vp, vp, i32, i32, i32, i32, i32, vp]),
