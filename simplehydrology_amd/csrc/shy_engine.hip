@@ -69,7 +69,8 @@ struct shy_ctx {
   int32_t* qb = nullptr;     // [drop][step] height commits of the replay
   size_t qb_cap = 0;         // ints qb holds
   int qstride = 0;           // qb's stride per drop (log rows rounded up to kReplayDepth)
-  int4* ent = nullptr;       // partition entries in tile order: {slot, cell, tracks}
+  int4* ent = nullptr;       // partition entries in tile order: wave_cap start cells, then
+                             // wave_cap {slot, tracks} (16 B per log slot in all)
   int32_t* vq = nullptr;     // [terrain][row] quantised volume track per step
   size_t vq_cap = 0;
   int vq_rows = 0;
@@ -515,7 +516,8 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   pa.ter = c->d_ter;
   pa.nter = nter;
   pa.hist = c->hist;
-  pa.ent = c->ent;
+  pa.ecell = reinterpret_cast<int32_t*>(c->ent);
+  pa.erest = reinterpret_cast<EntRest*>(reinterpret_cast<int32_t*>(c->ent) + c->wave_cap);
   pa.rows = walk ? c->part_rows : 0;
   pa.drops = std::max(c->total_drops, 1);
   pa.tb = kTileBits;
@@ -545,12 +547,12 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
       c->d_ter, nter, c->ntiles, nh > 0 ? c->hist : nullptr, (long long)nh, c->tstart);
   if (c->tile_split == kTileSplit) {
     tile_count_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
-        c->tstart, c->ent, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter,
+        c->tstart, pa.ecell, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter,
         c->tcount);
     tile_count_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
         c->tcount, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter);
   } else {
-    tile_count_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(c->tstart, c->ent, n, c->sr,
+    tile_count_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(c->tstart, pa.ecell, n, c->sr,
                                                             c->first, c->visit[p], c->fld,
                                                             c->root, c->d_ter, nter, nullptr);
   }
@@ -569,13 +571,13 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   if (c->tile_split == kTileSplit) {
     tile_reduce_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
-        c->tstart, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
+        c->tstart, pa.ecell, pa.erest, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
         c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, c->tsum);
     tile_reduce_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
         c->tsum, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   } else {
     tile_reduce_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(
-        c->tstart, c->ent, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
+        c->tstart, pa.ecell, pa.erest, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
         c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, nullptr);
   }
   HIPCHK(c, hipGetLastError());

@@ -731,6 +731,12 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(int32_t* v,
 // terrains ends the first terrain's matrix and opens the next one's, so its
 // entries stay contiguous.  Order inside a tile does not matter: every
 // consumer sums integers.
+// The rest of a partition entry beside its start cell: the log slot and the
+// step's quantised momentum tracks.
+struct EntRest {
+  int slot, tx, ty;
+};
+
 struct PartArgs {
   const int2* __restrict__ slog;      // [slot] {start cell, c_eq}
   const int2* __restrict__ strk;      // [slot] {qt(vol sx), qt(vol sy)}
@@ -738,7 +744,10 @@ struct PartArgs {
   const Terrain* __restrict__ ter;
   int nter;
   int32_t* hist;   // per terrain [tile][block] counts, then exclusive offsets; [total] = entries
-  int4* ent;       // [entry] {slot, start cell, qt(vol sx), qt(vol sy)}, tile order
+  // [entry] in tile order: the start cell (all tile_count_kernel reads) and
+  // {slot, qt(vol sx), qt(vol sy)}
+  int32_t* ecell;
+  EntRest* erest;
   int rows;        // log rows (0: no walk this cycle)
   int drops;       // log stride = all drops (padded)
   int tb;          // log2 cells per tile
@@ -822,7 +831,8 @@ __global__ void __launch_bounds__(1024) part_scatter_kernel(PartArgs a) {
       const int c = a.slog[k].x;
       const int2 tk = a.strk[k];
       const int e = atomicAdd(&cur[(c >> a.tb) - col.wlo], 1);
-      a.ent[e] = make_int4(k, c, tk.x, tk.y);
+      a.ecell[e] = c;
+      a.erest[e] = EntRest{k, tk.x, tk.y};
     }
   }
 }
@@ -1055,7 +1065,9 @@ __global__ void __launch_bounds__(1024, RS <= 4096 ? 8 : 4) part_scatter_staged_
     for (int i = tid; i < tot; i += 1024) {
       const int4 v = stage[i];
       const int t = (v.y >> a.tb) - wlo;
-      a.ent[cur[t] + (i - off[t])] = v;
+      const int e = cur[t] + (i - off[t]);
+      a.ecell[e] = v.y;
+      a.erest[e] = EntRest{v.x, v.z, v.w};
     }
     __syncthreads();
     for (int t = tid; t < wn; t += 1024) cur[t] += cnt[t];
@@ -1104,7 +1116,7 @@ __device__ inline void cell_count_finish(size_t i, int k, float* sr, int32_t* fi
 
 template <int S>
 __global__ void __launch_bounds__(1024) tile_count_kernel(
-    const int32_t* __restrict__ start, const int4* __restrict__ ent, size_t n,
+    const int32_t* __restrict__ start, const int32_t* __restrict__ ecell, size_t n,
     float* sr, int32_t* first, uint8_t* visit, const float4* __restrict__ fld,
     const float* __restrict__ root, const Terrain* __restrict__ ter, int nter, int32_t* part) {
   __shared__ int cnt[kTileCells];
@@ -1120,7 +1132,7 @@ __global__ void __launch_bounds__(1024) tile_count_kernel(
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int x = e + u * (int)blockDim.x;
-      c[u] = x < e1 ? ent[x].y - base : -1;
+      c[u] = x < e1 ? ecell[x] - base : -1;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -1176,7 +1188,8 @@ __global__ void volq_kernel(const Terrain* __restrict__ ter, int nter, int rows,
 // in tile_count_kernel and tile_reduce_finish_kernel adds the parts.
 template <int S>
 __global__ void __launch_bounds__(1024) tile_reduce_kernel(
-    const int32_t* __restrict__ start, const int4* __restrict__ ent,
+    const int32_t* __restrict__ start, const int32_t* __restrict__ ecell,
+    const EntRest* __restrict__ erest,
     const int32_t* __restrict__ qb, int qstride, const int32_t* __restrict__ vq, int rows,
     int drops, const Terrain* __restrict__ ter, int nter, int32_t* deposit, size_t n,
     int32_t* dh, long long* td, long long* tmx, long long* tmy, long long* part) {
@@ -1204,7 +1217,12 @@ __global__ void __launch_bounds__(1024) tile_reduce_kernel(
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int x = e + u * (int)blockDim.x;
-      p[u] = x < e1 ? ent[x] : make_int4(-1, 0, 0, 0);
+      if (x < e1) {
+        const EntRest r = erest[x];
+        p[u] = make_int4(r.slot, ecell[x], r.tx, r.ty);
+      } else {
+        p[u] = make_int4(-1, 0, 0, 0);
+      }
     }
     int q[4], v[4];
 #pragma unroll
