@@ -98,8 +98,9 @@ struct CellView {
   float h, xm, xp, ym, yp;  // h(x,y), h(x-1,y), h(x+1,y), h(x,y-1), h(x,y+1)
   float mx, my, d, erfd;    // momentum, discharge, erf(0.4 d)
   float root;               // rootdensity
-  float n0, n2;             // the cell's normal (x, z), when read from a packed record
-  float rf;                 // 1/|(mx, my)| when |(mx, my)| > 0, from a packed record
+  float n0, n2;             // from a packed record: the gravity terms lod*gravity*normal (x, z)
+  float rf;                 // from a packed record: 1/|(mx, my)| when |(mx, my)| > 0, its sign
+                            // set where a gravity term is outside markstein_num_ok
 };
 
 // quad::_normal (cellpool.h:181-204) from a loaded stencil.  Plane k is
@@ -146,29 +147,29 @@ __device__ inline float inv_len(float fx, float fy) {
 // within an ulp of a/b, the remainder r = a - b q is exact in one FMA, and
 // RN(q + r y) is the correctly rounded quotient (Markstein's theorem).  The
 // theorem needs q, r and y in the normal range: with 2^-60 <= |a| <= 2^60 and
-// 2^-60 <= b <= 2^60 they are (|q| <= 2^120, |r| >= 2^-84).  Outside that
-// range (a tiny slope, or a volume driven to the extremes by minVol <= 0 or
-// a negative evapRate) the plain division runs instead.  r = 0 means q is
-// exact, which keeps the sign of a zero.
-__host__ __device__ inline float div_rcp(float a, float b, float y) {
+// 2^-60 <= b <= 2^60 they are (|q| <= 2^120, |r| >= 2^-84); markstein_ok
+// tests that.  r = 0 means q is exact, which keeps the sign of a zero.
+__host__ __device__ inline bool markstein_num_ok(float a) {
   const float aa = __builtin_fabsf(a);
-  const bool in_range = (aa >= 0x1p-60f || aa == 0.0f) && aa <= 0x1p60f && b >= 0x1p-60f &&
-                        b <= 0x1p60f;
-  if (!in_range) return a / b;
+  return (aa >= 0x1p-60f || aa == 0.0f) && aa <= 0x1p60f;
+}
+__host__ __device__ inline bool markstein_den_ok(float b) { return b >= 0x1p-60f && b <= 0x1p60f; }
+__host__ __device__ inline float markstein_div(float a, float b, float y) {
   const float q = a * y;
   const float r = __builtin_fmaf(-q, b, a);
   return r == 0.0f ? q : __builtin_fmaf(r, y, q);
 }
+// Outside the theorem's range (a tiny slope, or a volume driven to the
+// extremes by minVol <= 0 or a negative evapRate) the plain division runs.
+__host__ __device__ inline float div_rcp(float a, float b, float y) {
+  if (!(markstein_num_ok(a) && markstein_den_ok(b))) return a / b;
+  return markstein_div(a, b, y);
+}
 
-// water.h:95-111: gravity, momentum coupling, fixed step of length sqrt(2).
-// rf = inv_len(fx, fy), used only when |f| > 0 (the count walk reads it from
-// the cell's packed record; the other callers compute it); yv = 1.0f / vol.
-__device__ inline void step_motion_rf(const shy_params& P, float n0, float n2, float vol,
-                                      float yv, float d, float fx, float fy, float rf,
-                                      float& sx, float& sy, float& px, float& py) {
-  const float k = kLod * P.gravity;
-  sx = sx + div_rcp(k * n0, vol, yv);  // (k n0) / vol
-  sy = sy + div_rcp(k * n2, vol, yv);
+// water.h:95-111 after the gravity term: momentum coupling, fixed step of
+// length sqrt(2).  rf = inv_len(fx, fy), used only when |f| > 0.
+__device__ inline void step_coupling(const shy_params& P, float vol, float d, float fx, float fy,
+                                     float rf, float& sx, float& sy, float& px, float& py) {
   if (__builtin_sqrtf(fx * fx + fy * fy) > 0.0f && __builtin_sqrtf(sx * sx + sy * sy) > 0.0f) {
     float rs = 1.0f / __builtin_sqrtf(sx * sx + sy * sy);
     float dt = (fx * rf) * (sx * rs) + (fy * rf) * (sy * rs);
@@ -186,10 +187,14 @@ __device__ inline void step_motion_rf(const shy_params& P, float n0, float n2, f
   py = py + sy;
 }
 
+// water.h:95-111: gravity (k n) / vol, momentum coupling, fixed step.
 __device__ inline void step_motion(const shy_params& P, float n0, float n2, float vol, float d,
                                    float fx, float fy, float& sx, float& sy, float& px,
                                    float& py) {
-  step_motion_rf(P, n0, n2, vol, 1.0f / vol, d, fx, fy, inv_len(fx, fy), sx, sy, px, py);
+  const float k = kLod * P.gravity, yv = 1.0f / vol;
+  sx = sx + div_rcp(k * n0, vol, yv);  // (k n0) / vol
+  sy = sy + div_rcp(k * n2, vol, yv);
+  step_coupling(P, vol, d, fx, fy, inv_len(fx, fy), sx, sy, px, py);
 }
 
 // water.h:135-136, evaluated in double as the reference's 1.0-evapRate is.

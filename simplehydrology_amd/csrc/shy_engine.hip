@@ -51,6 +51,7 @@ struct shy_ctx {
   int device = 0;
   std::vector<Terrain> ter;  // host copy of the table
   Terrain* d_ter = nullptr;
+  int* gflag = nullptr;      // [terrain] a gravity term out of the walk's division range (pack)
   size_t n = 0;              // cells, all terrains
   int total_drops = 0;
   int total_tiles = 0;
@@ -205,10 +206,23 @@ void drop_graph(shy_ctx* c) {
 constexpr int kPartMaxTiles = 32768;
 
 // Offsets of every terrain's cells, drops and cascade tiles; uploads the table.
+// Whether every volume a drop carries into a step (vol_0 = 1, vol_k+1 =
+// evap_vol(vol_k), while age <= maxAge and vol >= minVol: water.h:74-82, 136)
+// lies where the walk's reciprocal-form division is exact.
+bool volumes_in_division_range(const shy_params& P) {
+  float vol = 1.0f;
+  for (int age = 0;; ++age) {
+    if ((float)age > P.maxAge || vol < P.minVol) return true;
+    if (!markstein_den_ok(vol)) return false;
+    vol = evap_vol(vol, P.evapRate);
+  }
+}
+
 int layout(shy_ctx* c) {
   long long cells = 0, drops = 0, tiles = 0;
   const int ty = kTileY[c->tune.tile], tx = kTileX[c->tune.tile];
   for (Terrain& t : c->ter) {
+    t.vol_ok = volumes_in_division_range(t.P) ? 1 : 0;
     t.cell_off = cells;
     t.drop_off = (int)drops;
     t.tiles_y = (t.ny + ty - 1) / ty;
@@ -463,6 +477,7 @@ WaveArgs wave_args(shy_ctx* c, int p) {
   w.deposit = c->deposit;
   w.visit = c->visit[p];
   w.state = c->state;
+  w.gflag = c->gflag;
   w.tsteps = c->tsteps;
   return w;
 }
@@ -490,7 +505,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   const bool walk = c->total_drops > 0;
   if (walk) {
     pack_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, n, c->h[p],
-                                                                     c->fld, c->pack);
+                                                                     c->fld, c->pack, c->gflag);
     if (ordered(c))
       order_kernel<<<c->order_chunks, 1024, 0, c->stream>>>(c->d_ter, nter, c->total_drops, bs, c->state,
                                                  c->dperm);
@@ -596,6 +611,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   a.fld = c->fld;
   a.erf_a = c->erf_a;
   a.state = c->state;
+  a.gflag = c->gflag;
   return launch_cascade(c, a);
 }
 
@@ -777,7 +793,7 @@ int shy_destroy(shy_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   void* ptrs[] = {c->h[0], c->h[1], c->fld, c->root, c->visit[0], c->visit[1], c->first,
-                  c->sr, c->dh, c->trk, c->deposit, c->slog, c->strk, c->qb, c->ent,
+                  c->sr, c->dh, c->trk, c->deposit, c->slog, c->strk, c->qb, c->ent, c->gflag,
                   c->vq, c->tcount, c->tsum, c->hist, c->hbsum, c->tstart, c->dlog, c->wsteps, c->dperm, c->tsteps, c->veg_px, c->veg_py, c->veg_alive, c->veg_cflag,
                   c->veg_cx, c->veg_cy, c->veg_units, c->veg_st, c->veg_size, c->pack, c->d_ter, c->state, c->erf_a, c->part, c->perms, c->trace_buf,
                   c->trace_n, c->drop_buf};
@@ -847,6 +863,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");
   if ((e = hipMalloc(&c->tsteps, nter * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc tsteps");
   if ((e = hipMalloc(&c->d_ter, nter * sizeof(Terrain))) != hipSuccess) return bail(e, "hipMalloc terrains");
+  if ((e = hipMalloc(&c->gflag, nter * sizeof(int))) != hipSuccess) return bail(e, "hipMalloc gflag");
   if ((e = hipMalloc(&c->state, sizeof(DeviceState))) != hipSuccess) return bail(e, "hipMalloc state");
   if ((e = hipMalloc(&c->erf_a, kErfTerms * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc erf");
   if ((e = hipMalloc(&c->part, kPartBlocks * sizeof(float2))) != hipSuccess) return bail(e, "hipMalloc part");
@@ -864,6 +881,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   hipMemsetAsync(c->trk, 0, 3 * n * sizeof(long long), c->stream);
   hipMemsetAsync(c->deposit, 0, n * sizeof(int32_t), c->stream);
   hipMemsetAsync(c->tsteps, 0, nter * sizeof(long long), c->stream);
+  hipMemsetAsync(c->gflag, 0, nter * sizeof(int), c->stream);
   DeviceState st{};
   st.cycle = 0;
   double coef[kErfTerms];

@@ -87,6 +87,9 @@ struct Terrain {
   unsigned long long seed;
   unsigned int id;  // terrain number in the spawn hash
   int has_root;
+  // every volume a drop of this terrain can carry into a step lies where the
+  // walk's reciprocal-form division is exact (markstein_den_ok; host)
+  int vol_ok;
   shy_params P;
 };
 
@@ -100,6 +103,10 @@ constexpr int kFuseMaxTiles = 4096;       // most tiles of a terrain whose walk 
 struct WaveArgs {
   const Terrain* __restrict__ ter;
   int nter;
+  // [terrain] set by pack_kernel when some cell's gravity term lies outside
+  // the range of the walk's reciprocal-form division this cycle (cleared by
+  // the stencil at the end of the cycle)
+  const int* __restrict__ gflag;
   int total_drops;
   const float* __restrict__ h;
   const float4* __restrict__ fld;
@@ -167,6 +174,7 @@ struct CascArgs {
   float4* fld;
   const double* __restrict__ erf_a;
   DeviceState* state;
+  int* gflag;  // cleared for the next cycle (pack_kernel sets it)
 };
 
 // ------------------------------------------------------------ reductions
@@ -250,7 +258,7 @@ struct NoStepHook {
   __device__ void operator()(int) const {}
 };
 
-template <int MODE, class Hook = NoStepHook, bool PREFETCH = false>
+template <int MODE, class Hook = NoStepHook, bool PREFETCH = false, bool SAFE = false>
 __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, int* end,
                          int* end_cell, shy_drop_step* tr, int max_tr, int* tr_n,
                          Hook step_hook = Hook(), uint32_t* pf_sink = nullptr) {
@@ -289,8 +297,9 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   // a wait, and it keeps the load alive.
   uint32_t pf_acc = 0;
   float pf = 0.0f;
-  // 1 / vol for the gravity term, computed a step ahead, off the chain that
-  // leads from a record to the next one
+  // 1 / vol for the gravity term, and whether vol lies where the
+  // reciprocal-form division is exact, computed a step ahead, off the chain
+  // that leads from a record to the next one
   float yv = 1.0f / vol;
   // The first record is waited for here, before the loop: left pending, it
   // makes the wait at the top of the loop count the previous step's log
@@ -304,7 +313,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       pf = a.pack[2 * (size_t)(qx * ny + qy)].x;
     }
     float n0, n1, n2;
-    if constexpr (COUNT) {  // the packed record holds the cell's normal (water.h:70)
+    if constexpr (COUNT) {  // the packed record holds the cell's gravity terms (water.h:70, 95)
       n0 = c.n0;
       n2 = c.n2;
     } else {
@@ -320,12 +329,37 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
     ++steps;
     float effD = P.depositionRate * (1.0f - c.root);  // water.h:86-87
     if (effD < 0) effD = 0;
-    if constexpr (COUNT)  // water.h:95-111
-      step_motion_rf(P, n0, n2, vol, yv, c.d, c.mx, c.my, c.rf, sx, sy, px, py);
-    else
+    if constexpr (COUNT) {  // water.h:95-111
+      // the record holds the gravity terms k n (x, z).  SAFE: every gravity
+      // term of the terrain and every volume lies where the reciprocal-form
+      // division is exact; else each division checks its operands
+      if constexpr (SAFE) {
+        sx = sx + markstein_div(n0, vol, yv);
+        sy = sy + markstein_div(n2, vol, yv);
+      } else {
+        sx = sx + div_rcp(n0, vol, yv);
+        sy = sy + div_rcp(n2, vol, yv);
+      }
+      step_coupling(P, vol, c.d, c.mx, c.my, c.rf, sx, sy, px, py);
+    } else {
       step_motion(P, n0, n2, vol, c.d, c.mx, c.my, sx, sy, px, py);
-    const int jx = cell_coord(px), jy = cell_coord(py);
-    const bool ex = off_map(jx, jy, nx, ny);
+    }
+    // ivec2(pos) (truncation toward zero) and the exit test: (int)p lies in
+    // [0, n) iff -1 < p < n, and a NaN coordinate fails the test (a speed made
+    // NaN by a volume that underflowed to 0 with minVol <= 0: the reference's
+    // x86 build sends it off the map too); the exit test does not wait for the
+    // conversion, and off the map the cell index is not used
+    int jx, jy;
+    bool ex;
+    if constexpr (COUNT) {
+      jx = (int)px;
+      jy = (int)py;
+      ex = !(px > -1.0f && px < (float)nx && py > -1.0f && py < (float)ny);
+    } else {
+      jx = cell_coord(px);
+      jy = cell_coord(py);
+      ex = off_map(jx, jy, nx, ny);
+    }
     const int j = jx * ny + jy;
     // off the map the record is not used; loading the current cell's keeps
     // the load unconditional, so its wait is counted exactly
@@ -394,7 +428,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
 // step: a shorter chain per step.
 __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
                             const float* __restrict__ h, const float4* __restrict__ fld,
-                            float4* pack) {
+                            float4* pack, int* gflag) {
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const int t = find_terrain(nter, (long long)g, [&](int q) { return ter[q].cell_off; });
@@ -410,9 +444,14 @@ __global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
   c.h = hc; c.xm = xm; c.xp = xp; c.ym = ym; c.yp = yp;
   float n0, n1, n2;
   normal_from(c, x, y, T.nx, ny, n0, n1, n2);  // water.h:70, once per cell instead of per step
-  // 1/|momentum| for the coupling term (read only where |momentum| > 0)
+  // the gravity terms k n (water.h:95's numerators; the terrain is flagged
+  // where one lies outside the range of the walk's reciprocal-form division),
+  // and 1/|momentum| for the coupling term (read only where |momentum| > 0)
+  const float k = kLod * T.P.gravity;
+  const float a0 = k * n0, a2 = k * n2;
+  if (!(markstein_num_ok(a0) && markstein_num_ok(a2))) gflag[t] = 1;
   const float rf = f.x * f.x + f.y * f.y > 0.0f ? inv_len(f.x, f.y) : 0.0f;
-  pack[2 * g] = make_float4(hc, n0, n2, f.w);
+  pack[2 * g] = make_float4(hc, a0, a2, f.w);
   pack[2 * g + 1] = make_float4(f.x, f.y, f.z, rf);
 }
 
@@ -475,9 +514,15 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       if ((double)a.h[cid] >= 0.1) {  // world.h:71-72
         int end = 0, end_cell = 0;
         uint32_t sink = 0;
-        dl.x = walk_drop<kCount, decltype(count_step), PREFETCH>(a, g, (float)(cid / (uint32_t)T.ny),
-                                 (float)(cid % (uint32_t)T.ny), &end, &end_cell, nullptr, 0,
-                                 nullptr, count_step, &sink);
+        const float x0 = (float)(cid / (uint32_t)T.ny), y0 = (float)(cid % (uint32_t)T.ny);
+        // uniform over the wave (one terrain per wave): the step loop without
+        // the division's operand checks wherever they cannot fail
+        if (T.vol_ok && !w.gflag[t])
+          dl.x = walk_drop<kCount, decltype(count_step), PREFETCH, true>(
+              a, g, x0, y0, &end, &end_cell, nullptr, 0, nullptr, count_step, &sink);
+        else
+          dl.x = walk_drop<kCount, decltype(count_step), PREFETCH, false>(
+              a, g, x0, y0, &end, &end_cell, nullptr, 0, nullptr, count_step, &sink);
         dl.w = (int)sink;  // not read: it keeps the record prefetches
         dl.y = end;
         dl.z = end_cell;
@@ -1385,6 +1430,8 @@ __global__ void __launch_bounds__(kCbY * kCbX) cascade_blend_kernel(CascArgs e) 
   __shared__ uint8_t Vs[kCbX + 6][kCbY + 6]; // cascade sites, 3-cell halo
   __shared__ uint8_t Wc[kCbX + 4][kCbY + 4]; // site windows covering a cell, 2-cell halo
   __shared__ int32_t Wd[(kCbX + 2) * (kCbY + 2)][9];
+  if (blockIdx.x == 0 && e.gflag)
+    for (int q = threadIdx.x; q < e.nter; q += kCbThreads) e.gflag[q] = 0;
   const int tt = find_terrain(e.nter, (long long)blockIdx.x,
                               [&](int q) { return (long long)e.ter[q].tile_off; });
   const Terrain& T = e.ter[tt];
