@@ -31,6 +31,14 @@ const char* shy_nn_last_error(void);
 int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void* res,
                     void* y_bf16, float* y_f32, int N, int H, int W, int Cin, int Cout,
                     int taps, int relu, void* stream);
+/* The same 3x3 convolution of the channel concatenation [up2(x_half), skip]
+   -- a nearest x2 upsample of x_half [N][H/2][W/2][C1] in channels [0, C1),
+   skip [N][H][W][C2] in [C1, C1+C2) -- read straight from the two tensors,
+   with the same bits as shy_nn_up2 + shy_nn_channels + shy_nn_conv_fwd.
+   Weights w[Cout][9*(C1+C2)]; H, W even; C1, C2, Cout multiples of 32. */
+int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, const float* bias,
+                          const void* res, void* y_bf16, float* y_f32, int N, int H, int W,
+                          int C1, int C2, int Cout, int relu, void* stream);
 /* weights for the data gradient: wf[Cin][taps*Cout] = w[Cout][(taps-1-t)*Cin] */
 int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, void* stream);
 /* dw[Cout][taps*Cin] (FP32) = sum over pixels of g[p][co] x[p+tap][ci];

@@ -63,6 +63,10 @@ struct ConvArgs {
   float* y_f;
   int N, H, W, Cin, Cout, K;
   int relu;
+  // When x2 is set (conv_pf_kernel only), the input is the concatenation of
+  // a nearest x2 upsample of x ([N][H/2][W/2][C1]) and x2 ([N][H][W][Cin-C1]).
+  const __bf16* __restrict__ x2;
+  int C1;
 };
 
 // Forward (and dgrad) convolution.  Block = 4 waves; each wave owns 32 pixels
@@ -136,122 +140,26 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvArgs a) {
     }
 }
 
+// Tile width of the LDS-staged convolutions, and LDS bf16 per patch pixel
+// (32 channels + 8: spreads the banks).
+constexpr int kTW = 16;
+constexpr int kPS = 40;
+
 // Forward (and dgrad) convolution for Cin and Cout multiples of 32, staged
 // through LDS.  A block owns a 16 (x) by 4*NT (y) pixel tile of one image and
 // 16*MT output channels.  Per 32-channel chunk of the input it stages the
 // tile's input patch with a 1-pixel halo (TAPS = 9) and the chunk's weights
-// for every tap, then each wave multiplies: output channels are the MFMA's
-// M operand (A = weights), pixels its N operand (B = patch), so a lane ends
-// with 4 consecutive channels of one pixel and stores them as one vector.
-// Wave w owns tile rows NT*w .. NT*w+NT-1 (one 16-pixel N tile each).
-constexpr int kTW = 16;
-constexpr int kPS = 40;  // LDS bf16 per patch pixel (32 + 8: spreads the banks)
-
-template <int TAPS, int MT, int NT>
-__global__ void __launch_bounds__(256) conv_lds_kernel(ConvArgs a) {
-  constexpr int kTH = 4 * NT;
-  constexpr int HALO = TAPS == 9 ? 1 : 0;
-  constexpr int PW = kTW + 2 * HALO, PH = kTH + 2 * HALO;
-  constexpr int WS = TAPS * 32 + 8;  // LDS bf16 per output channel of the weight chunk
-  __shared__ __bf16 patch[PH * PW * kPS];
-  __shared__ __bf16 wl[MT * 16 * WS];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
-  int b = blockIdx.x;
-  const int tx = b % tiles_x;
-  b /= tiles_x;
-  const int ty = b % tiles_y;
-  const int n = b / tiles_y;
-  const int x0 = tx * kTW, y0 = ty * kTH;
-  const int cb = blockIdx.y * MT * 16;
-  const int Cin = a.Cin, K = a.K;
-  f32x4 acc[MT][NT];
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
-    // patch: PH x PW pixels x 32 channels as 16-byte vectors (zero off the image)
-    for (int v = threadIdx.x; v < PH * PW * 4; v += 256) {
-      const int pix = v >> 2, q = v & 3;
-      const int py = pix / PW, px = pix % PW;
-      const int gy = y0 + py - HALO, gx = x0 + px - HALO;
-      bf16x8 val = zero8();
-      if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-        val = load8(a.x + ((size_t)(n * a.H + gy) * a.W + gx) * Cin + ci0 + q * 8);
-      *reinterpret_cast<bf16x8*>(&patch[pix * kPS + q * 8]) = val;
-    }
-    // weights: MT*16 channels x TAPS taps x 32 input channels
-    for (int v = threadIdx.x; v < MT * 16 * TAPS * 4; v += 256) {
-      const int co = v / (TAPS * 4), r = v % (TAPS * 4);
-      const int tap = r >> 2, q = r & 3;
-      bf16x8 val = zero8();
-      if (cb + co < a.Cout) val = load8(a.w + (size_t)(cb + co) * K + tap * Cin + ci0 + q * 8);
-      *reinterpret_cast<bf16x8*>(&wl[co * WS + tap * 32 + q * 8]) = val;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int tap = 0; tap < TAPS; ++tap) {
-      const int dy = TAPS == 9 ? tap / 3 : 0, dx = TAPS == 9 ? tap % 3 : 0;
-      bf16x8 B[NT], A[MT];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int row = wave * NT + t + dy, col = (lane & 15) + dx;
-        B[t] = *reinterpret_cast<const bf16x8*>(&patch[(row * PW + col) * kPS + (lane >> 4) * 8]);
-      }
-#pragma unroll
-      for (int m = 0; m < MT; ++m)
-        A[m] = *reinterpret_cast<const bf16x8*>(&wl[(m * 16 + (lane & 15)) * WS + tap * 32 +
-                                                    (lane >> 4) * 8]);
-#pragma unroll
-      for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = mfma(A[m], B[t], acc[m][t]);
-    }
-    __syncthreads();
-  }
-  // epilogue: lane holds channels cb + 16m + 4(lane/16) + i of pixel (row, lane%16)
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
-    if (gy >= a.H || gx >= a.W) continue;
-    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int co = cb + m * 16 + (lane >> 4) * 4;
-      if (co >= a.Cout) continue;
-      float v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
-      const size_t o = p * a.Cout + co;
-      if (a.res) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
-      }
-      if (a.relu) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
-      }
-      if (a.y_f) {
-        *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
-      } else {
-        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-        bf16x4 q;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
-        *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
-      }
-    }
-  }
-}
-
-// The same convolution with the next 32-channel chunk's patch and weights
+// for every tap; output channels are the MFMA's M operand (A = weights),
+// pixels its N operand (B = patch), so a lane ends with 4 consecutive
+// channels of one pixel and stores them as one vector.  Wave w owns tile
+// rows NT*w .. NT*w+NT-1 (one 16-pixel N tile each).  The next chunk's patch
+// and weights are
 // loaded into registers while the current chunk's MFMAs run (the global
 // loads' latency hides behind them; LDS holds one chunk), and each patch row
 // read from LDS once per column shift dx and used for the three row shifts
 // dy (a wave's NT output rows need NT + 2 patch rows): per chunk and wave
 // 3 (NT + 2) B reads and 9 MT A reads feed 9 MT NT MFMAs.
-template <int TAPS, int MT, int NT>
+template <int TAPS, int MT, int NT, bool UPCAT = false, bool PF = true>
 __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
   constexpr int kTH = 4 * NT;
   constexpr int HALO = TAPS == 9 ? 1 : 0;
@@ -283,8 +191,16 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
         const int pix = v >> 2, q = v & 3;
         const int py = pix / PW, px = pix % PW;
         const int gy = y0 + py - HALO, gx = x0 + px - HALO;
-        if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-          val = load8(a.x + ((size_t)(n * a.H + gy) * a.W + gx) * Cin + ci0 + q * 8);
+        if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+          const __bf16* src;
+          if (!UPCAT)
+            src = a.x + ((size_t)(n * a.H + gy) * a.W + gx) * Cin + ci0;
+          else if (ci0 < a.C1)  // upsampled half: pixel (gy/2, gx/2) of x
+            src = a.x + ((size_t)(n * (a.H >> 1) + (gy >> 1)) * (a.W >> 1) + (gx >> 1)) * a.C1 + ci0;
+          else
+            src = a.x2 + ((size_t)(n * a.H + gy) * a.W + gx) * (Cin - a.C1) + (ci0 - a.C1);
+          val = load8(src + q * 8);
+        }
       }
       rp[j] = val;
     }
@@ -321,11 +237,22 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
   for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  gload(0);
-  lstore();
-  __syncthreads();
+  // PF: the next chunk loads into registers during this chunk's MFMAs; else
+  // each chunk is loaded and staged before its MFMAs (fewer registers live
+  // across them, for single-chunk layers with larger tiles)
+  if constexpr (PF) {
+    gload(0);
+    lstore();
+    __syncthreads();
+  }
   for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
-    const bool more = ci0 + 32 < Cin;
+    if constexpr (!PF) {
+      if (ci0 > 0) __syncthreads();
+      gload(ci0);
+      lstore();
+      __syncthreads();
+    }
+    const bool more = PF && ci0 + 32 < Cin;
     if (more) gload(ci0 + 32);
 #pragma unroll
     for (int dx = 0; dx < NS; ++dx) {
@@ -349,10 +276,12 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
           for (int t = 0; t < NT; ++t) acc[m][t] = mfma(A[m], B[t + dy], acc[m][t]);
       }
     }
-    __syncthreads();
-    if (more) {
-      lstore();
+    if constexpr (PF) {
       __syncthreads();
+      if (more) {
+        lstore();
+        __syncthreads();
+      }
     }
   }
   // epilogue: lane holds channels cb + 16m + 4(lane/16) + i of pixel (row, lane%16)
@@ -385,6 +314,100 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
         *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
+      }
+    }
+  }
+}
+
+// 3x3 convolution of an 8-channel input (the network's first layer), staged
+// through LDS.  K = 9 taps x 8 channels = 72, padded to three MFMA k-steps of
+// 32: k-step j takes taps 4j .. 4j+3 (lane group q = lane/16 reads tap 4j+q,
+// zero past tap 8), each tap's 8 channels one 16-byte patch vector.  Output
+// channels are the M operand and pixels the N operand, as in conv_pf_kernel,
+// whose epilogue this shares: a lane stores 4 consecutive channels at once.
+template <int MT, int NT>
+__global__ void __launch_bounds__(256) conv_c8_kernel(ConvArgs a) {
+  constexpr int kTH = 4 * NT, PW = kTW + 2, PH = kTH + 2;
+  constexpr int KP = 96 + 8;  // LDS bf16 per output channel of the weights (72 used, padded)
+  __shared__ bf16x8 patch[PH * PW];
+  __shared__ __bf16 wl[MT * 16 * KP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+  int b = blockIdx.x;
+  const int tx = b % tiles_x;
+  b /= tiles_x;
+  const int ty = b % tiles_y;
+  const int n = b / tiles_y;
+  const int x0 = tx * kTW, y0 = ty * kTH;
+  const int cb = blockIdx.y * MT * 16;
+  for (int v = threadIdx.x; v < PH * PW; v += 256) {
+    const int py = v / PW, px = v % PW;
+    const int gy = y0 + py - 1, gx = x0 + px - 1;
+    bf16x8 val = zero8();
+    if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+      val = load8(a.x + ((size_t)(n * a.H + gy) * a.W + gx) * 8);
+    patch[v] = val;
+  }
+  for (int v = threadIdx.x; v < MT * 16 * 12; v += 256) {  // 12 taps of 8 per channel
+    const int co = v / 12, tap = v % 12;
+    bf16x8 val = zero8();
+    if (tap < 9 && cb + co < a.Cout) val = load8(a.w + (size_t)(cb + co) * 72 + tap * 8);
+    *reinterpret_cast<bf16x8*>(&wl[co * KP + tap * 8]) = val;
+  }
+  __syncthreads();
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int q = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int tap = 4 * j + q;
+    const int dy = tap < 9 ? tap / 3 : 0, dx = tap < 9 ? tap % 3 : 0;
+    bf16x8 A[MT], B[NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+      A[m] = *reinterpret_cast<const bf16x8*>(&wl[(m * 16 + (lane & 15)) * KP + tap * 8]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x8 v = patch[(wave * NT + t + dy) * PW + (lane & 15) + dx];
+      B[t] = tap < 9 ? v : zero8();
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[m][t] = mfma(A[m], B[t], acc[m][t]);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
+    if (gy >= a.H || gx >= a.W) continue;
+    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int co = cb + m * 16 + (lane >> 4) * 4;
+      if (co >= a.Cout) continue;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
+      const size_t o = p * a.Cout + co;
+      if (a.res) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
+      }
+      if (a.relu) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+      }
+      if (a.y_f) {
+        *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 qv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) qv[i] = (__bf16)v[i];
+        *reinterpret_cast<bf16x4*>(a.y_bf + o) = qv;
       }
     }
   }
@@ -930,6 +953,51 @@ int nn_check(const char* what) {
 }
 inline unsigned blocks(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// conv_pf_kernel for Cin and Cout multiples of 32: 16x16 tiles (4 rows per
+// wave) when they still give >= 1024 blocks, else 16x8
+int launch_pf(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
+  const int mt = a.Cout % 64 == 0 ? 4 : 2;
+  const unsigned gy = (unsigned)(a.Cout / (16 * mt));
+  const long long tiles16 = (long long)a.N * ((a.H + 15) / 16) * ((a.W + kTW - 1) / kTW);
+  const long long tiles8 = (long long)a.N * ((a.H + 7) / 8) * ((a.W + kTW - 1) / kTW);
+  if (tiles8 > INT32_MAX) return nn_fail(std::string(what) + ": too many tiles");
+  const bool big = tiles16 * gy >= 1024;
+  const dim3 g((unsigned)(big ? tiles16 : tiles8), gy);
+  const long long tiles32 = (long long)a.N * ((a.H + 31) / 32) * ((a.W + kTW - 1) / kTW);
+  if (!a.x2 && taps == 9 && mt == 2 && a.Cin == 32 && tiles32 * gy >= 1024) {
+    // one input chunk: 16x32 tiles staged without prefetch, so each block
+    // reloads the layer's weights for twice the pixels
+    conv_pf_kernel<9, 2, 8, false, false><<<dim3((unsigned)tiles32, gy), 256, 0, s>>>(a);
+    return nn_check(what);
+  }
+  if (a.x2) {  // the input is [up2(x), x2] (taps 9)
+    if (mt == 4) {
+      if (big) conv_pf_kernel<9, 4, 4, true><<<g, 256, 0, s>>>(a);
+      else conv_pf_kernel<9, 4, 2, true><<<g, 256, 0, s>>>(a);
+    } else {
+      if (big) conv_pf_kernel<9, 2, 4, true><<<g, 256, 0, s>>>(a);
+      else conv_pf_kernel<9, 2, 2, true><<<g, 256, 0, s>>>(a);
+    }
+  } else if (mt == 4) {
+    if (big) {
+      if (taps == 9) conv_pf_kernel<9, 4, 4><<<g, 256, 0, s>>>(a);
+      else conv_pf_kernel<1, 4, 4><<<g, 256, 0, s>>>(a);
+    } else {
+      if (taps == 9) conv_pf_kernel<9, 4, 2><<<g, 256, 0, s>>>(a);
+      else conv_pf_kernel<1, 4, 2><<<g, 256, 0, s>>>(a);
+    }
+  } else {
+    if (big) {
+      if (taps == 9) conv_pf_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
+      else conv_pf_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
+    } else {
+      if (taps == 9) conv_pf_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
+      else conv_pf_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
+    }
+  }
+  return nn_check(what);
+}
+
 // Device address of `scratch` on the current device: looked up once per
 // device (on that device's first allocation-free call) and kept.
 constexpr int kMaxDevices = 64;
@@ -974,36 +1042,19 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
   a.y_f = y_f32;
   a.N = N; a.H = H; a.W = W; a.Cin = Cin; a.Cout = Cout; a.K = taps * Cin;
   a.relu = relu;
+  a.x2 = nullptr;
+  a.C1 = 0;
   const long long P = (long long)N * H * W;
   if (P > INT32_MAX / 2) return nn_fail("conv_fwd: too many pixels");
   hipStream_t s = (hipStream_t)stream;
-  if (Cin % 32 == 0 && Cout % 32 == 0) {
-    // 16x16 tiles (4 rows per wave) when they still give >= 1024 blocks,
-    // else 16x8
+  if (Cin % 32 == 0 && Cout % 32 == 0) return launch_pf(a, taps, s, "conv_fwd");
+  if (Cin == 8 && taps == 9 && Cout % 32 == 0) {  // the first layer
     const int mt = Cout % 64 == 0 ? 4 : 2;
-    const unsigned gy = (unsigned)(Cout / (16 * mt));
-    const long long tiles16 = (long long)N * ((H + 15) / 16) * ((W + kTW - 1) / kTW);
-    const long long tiles8 = (long long)N * ((H + 7) / 8) * ((W + kTW - 1) / kTW);
-    if (tiles8 > INT32_MAX) return nn_fail("conv_fwd: too many tiles");
-    const bool big = tiles16 * gy >= 1024;
-    const dim3 g((unsigned)(big ? tiles16 : tiles8), gy);
-    if (mt == 4) {
-      if (big) {
-        if (taps == 9) conv_pf_kernel<9, 4, 4><<<g, 256, 0, s>>>(a);
-        else conv_pf_kernel<1, 4, 4><<<g, 256, 0, s>>>(a);
-      } else {
-        if (taps == 9) conv_pf_kernel<9, 4, 2><<<g, 256, 0, s>>>(a);
-        else conv_pf_kernel<1, 4, 2><<<g, 256, 0, s>>>(a);
-      }
-    } else {
-      if (big) {
-        if (taps == 9) conv_pf_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
-        else conv_pf_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
-      } else {
-        if (taps == 9) conv_pf_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
-        else conv_pf_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
-      }
-    }
+    const long long tiles = (long long)N * ((H + 15) / 16) * ((W + kTW - 1) / kTW);
+    if (tiles > INT32_MAX) return nn_fail("conv_fwd: too many tiles");
+    const dim3 g((unsigned)tiles, (unsigned)(Cout / (16 * mt)));
+    if (mt == 4) conv_c8_kernel<4, 4><<<g, 256, 0, s>>>(a);
+    else conv_c8_kernel<2, 4><<<g, 256, 0, s>>>(a);
     return nn_check("conv_fwd");
   }
   const unsigned gx = (unsigned)((P + 127) / 128);
@@ -1017,6 +1068,29 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
     else conv_fwd_kernel<1, 1><<<g, 256, 0, s>>>(a);
   }
   return nn_check("conv_fwd");
+}
+
+int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, const float* bias,
+                          const void* res, void* y_bf16, float* y_f32, int N, int H, int W,
+                          int C1, int C2, int Cout, int relu, void* stream) {
+  if (!x_half || !skip || !w || (!y_bf16 && !y_f32) || (y_bf16 && y_f32))
+    return nn_fail("conv_fwd_upcat: bad pointers");
+  if (N < 1 || H < 2 || W < 2 || H % 2 || W % 2 || C1 < 32 || C1 % 32 || C2 < 32 || C2 % 32 ||
+      Cout < 32 || Cout % 32)
+    return nn_fail("conv_fwd_upcat: need even H, W and C1, C2, Cout multiples of 32");
+  if ((long long)N * H * W > INT32_MAX / 2) return nn_fail("conv_fwd_upcat: too many pixels");
+  ConvArgs a;
+  a.x = (const __bf16*)x_half;
+  a.x2 = (const __bf16*)skip;
+  a.C1 = C1;
+  a.w = (const __bf16*)w;
+  a.bias = bias;
+  a.res = (const __bf16*)res;
+  a.y_bf = (__bf16*)y_bf16;
+  a.y_f = y_f32;
+  a.N = N; a.H = H; a.W = W; a.Cin = C1 + C2; a.Cout = Cout; a.K = 9 * (C1 + C2);
+  a.relu = relu;
+  return launch_pf(a, 9, (hipStream_t)stream, "conv_fwd_upcat");
 }
 
 int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, void* stream) {

@@ -41,6 +41,8 @@ def nn():
             "shy_nn_last_error": (C.c_char_p, []),
             "shy_nn_conv_fwd": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32,
                                       vp]),
+            "shy_nn_conv_fwd_upcat": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32,
+                                            i32, i32, vp]),
             "shy_nn_flip_weights": (i32, [vp, vp, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad_noalloc": (i32, [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
@@ -160,6 +162,19 @@ class Conv:
                                  _stream()))
         return y
 
+    def forward_upcat(self, xh, skip, shape, res=None, relu=True):
+        """The same convolution of [up2(xh), skip] along channels, read from
+        the two tensors (no concatenated copy)."""
+        import torch
+        N, H, W = shape
+        c1, c2 = xh.shape[-1], skip.shape[-1]
+        assert c1 + c2 == self.cin and self.taps == 9
+        y = torch.empty(N, H, W, self.cout, device=xh.device, dtype=torch.bfloat16)
+        _ck(nn().shy_nn_conv_fwd_upcat(_p(xh), _p(skip), _p(self.w_bf), _p(self.b), _p(res),
+                                       _p(y), None, N, H, W, c1, c2, self.cout,
+                                       1 if relu else 0, _stream()))
+        return y
+
     def backward(self, x, g, shape, scratch, need_dx=True):
         """g: gradient of the pre-activation output (bf16). Returns dx (bf16)."""
         import torch
@@ -228,11 +243,16 @@ class ResStage:
         self.b = Conv(c, c, 9, gen, device)
         self.c = Conv(c, c, 9, gen, device)
 
-    def forward(self, x, shape):
-        ya = self.a.forward(x, shape)
+    def forward(self, x, shape, skip=None):
+        """With ``skip`` the stage's input is [up2(x), skip] along channels,
+        read without building it (inference: nothing is kept for backward)."""
+        if skip is None:
+            ya = self.a.forward(x, shape)
+        else:
+            ya = self.a.forward_upcat(x, skip, shape)
         yb = self.b.forward(ya, shape)
         yc = self.c.forward(yb, shape, res=ya)
-        self.saved = (x, ya, yb, yc)
+        self.saved = (x, ya, yb, yc) if skip is None else None
         return yc
 
     def backward(self, g, shape, scratch, need_dx=True):
@@ -286,11 +306,15 @@ class UNet:
         f += 2 * self.head.cin * self.head.cout * sizes[0]
         return f * N
 
-    def forward(self, x):
-        """x: [N, H, W, 8] bf16 (H, W multiples of 8). Returns [N, H, W, 8] f32."""
+    def forward(self, x, keep=True):
+        """x: [N, H, W, 8] bf16 (H, W multiples of 8). Returns [N, H, W, 8] f32.
+        ``keep`` saves what :meth:`backward` needs; without it (inference) the
+        decoder stages read their upsampled-and-concatenated input straight
+        from the two tensors, with the same bits."""
         import torch
         N, H, W, _ = x.shape
         self.shapes = [(N, H >> k, W >> k) for k in range(4)]
+        self.kept = keep
         skips = []
         h = x
         for k in range(4):
@@ -306,6 +330,9 @@ class UNet:
         for k in (2, 1, 0):
             n, hh, ww = self.shapes[k]
             cu, cs = h.shape[-1], skips[k].shape[-1]
+            if not keep:
+                h = self.d[k].forward(h, self.shapes[k], skip=skips[k])
+                continue
             u = torch.empty(n, hh, ww, cu + cs, dtype=torch.bfloat16, device=x.device)
             _ck(nn().shy_nn_up2(_p(h), _p(u), n, hh // 2, ww // 2, cu, cu + cs, 0, 0, _stream()))
             _ck(nn().shy_nn_channels(_p(skips[k]), _p(u), n * hh * ww, cs, cu + cs, cu, 0,
@@ -318,6 +345,8 @@ class UNet:
     def backward(self, gy):
         """gy: [N, H, W, 8] bf16 gradient of the output."""
         import torch
+        if not self.kept:
+            raise RuntimeError("backward needs a forward(..., keep=True)")
         g = self.head.backward(self.h0, gy, self.shapes[0], self.scratch)
         gskip = [None, None, None]
         for k, (cu, cs) in zip((0, 1, 2), reversed(self.cat)):
@@ -400,7 +429,7 @@ def predict_tiled(net, h, d, mx, my, tile=256, halo=32, batch=16):
             for wy, cy, ly in _tile_starts(Y, tile, halo)]
     for b0 in range(0, len(jobs), batch):
         js = jobs[b0:b0 + batch]
-        y = net.forward(window_batch(feats, [(j[0], j[3]) for j in js], tile))
+        y = net.forward(window_batch(feats, [(j[0], j[3]) for j in js], tile), keep=False)
         for k, (wx, cx, lx, wy, cy, ly) in enumerate(js):
             out[cx:cx + lx, cy:cy + ly] = y[k, cx - wx:cx - wx + lx, cy - wy:cy - wy + ly, :3]
     return decode_flow(out)

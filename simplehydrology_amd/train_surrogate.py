@@ -272,8 +272,8 @@ def train_timing(net, batch=16, tile=256, reps=5):
     x = torch.zeros(batch, tile, tile, 8, dtype=torch.bfloat16, device=net.device)
     y = torch.zeros(batch, tile, tile, 3, dtype=torch.float32, device=net.device)
 
-    def fwd():
-        return net.forward(x)
+    def fwd():  # inference: nothing kept for a backward pass
+        return net.forward(x, keep=False)
 
     def step():
         out = net.forward(x)
@@ -317,7 +317,7 @@ def validate_maps(net, feats, targets, batch=16):
     pred_l1, start_l1 = [], []
     for b0 in range(0, feats.shape[0], batch):
         x = feats[b0:b0 + batch].contiguous()
-        y = net.forward(x)
+        y = net.forward(x, keep=False)
         for k in range(x.shape[0]):
             d, _, _ = sg.decode_flow(y[k, ..., :3])
             td, _, _ = sg.decode_flow(targets[b0 + k].float())

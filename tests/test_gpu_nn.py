@@ -42,7 +42,8 @@ def ref_conv(x_nhwc, w_rows, b, taps, cin, cout):
 @pytest.mark.parametrize("N,H,W,cin,cout,taps", [(2, 16, 24, 8, 32, 9), (1, 32, 32, 64, 96, 9),
                                                  (3, 8, 40, 224, 96, 9), (2, 16, 16, 32, 8, 1),
                                                  (1, 13, 7, 16, 8, 9), (1, 24, 20, 32, 64, 9),
-                                                 (2, 8, 16, 32, 64, 1), (1, 5, 9, 96, 128, 9)])
+                                                 (2, 8, 16, 32, 64, 1), (1, 5, 9, 96, 128, 9),
+                                                 (8, 256, 256, 32, 32, 9)])
 def test_conv_forward_dgrad_wgrad(sg, N, H, W, cin, cout, taps):
     g = torch.Generator().manual_seed(cin * 7 + cout)
     x = bf(torch.randn(N, H, W, cin, generator=g))
@@ -166,6 +167,48 @@ def test_unet_gradients_match_fp32_reference(sg):
         worst = max(worst, rel(cv.dw.cpu(), w.grad), rel(cv.db.cpu(), b.grad))
     print("worst relative gradient error", worst, flush=True)
     assert worst < 0.08
+
+
+@pytest.mark.parametrize("N,H,W,c1,c2,cout", [(2, 16, 24, 64, 32, 32), (1, 32, 16, 96, 64, 64),
+                                               (3, 8, 40, 128, 96, 96)])
+def test_conv_upcat_equals_concatenated_input(sg, N, H, W, c1, c2, cout):
+    """shy_nn_conv_fwd_upcat (the decoder's first convolution reading the
+    upsampled half and the skip straight from their tensors) gives the bits of
+    up2 + channel concatenation + the plain convolution, and agrees with an
+    FP32 torch convolution of the same input."""
+    g = torch.Generator().manual_seed(c1 + c2)
+    xh = bf(torch.randn(N, H // 2, W // 2, c1, generator=g))
+    sk = bf(torch.randn(N, H, W, c2, generator=g))
+    w = bf(torch.randn(cout, 9 * (c1 + c2), generator=g) * 0.05)
+    b = torch.randn(cout, generator=g) * 0.1
+    dev = "cuda:0"
+    conv = sg.Conv(c1 + c2, cout, 9, np.random.default_rng(0), dev)
+    conv.w_bf.copy_(w.to(dev).to(torch.bfloat16))
+    conv.b.copy_(b.to(dev))
+    xhd = xh.to(dev).to(torch.bfloat16).contiguous()
+    skd = sk.to(dev).to(torch.bfloat16).contiguous()
+    print("upcat", N, H, W, c1, c2, cout, flush=True)
+    y = conv.forward_upcat(xhd, skd, (N, H, W))
+    u = torch.cat([xhd.repeat_interleave(2, 1).repeat_interleave(2, 2), skd], -1).contiguous()
+    y0 = conv.forward(u, (N, H, W))
+    assert torch.equal(y.view(torch.int16), y0.view(torch.int16))
+    xr = torch.cat([xh.repeat_interleave(2, 1).repeat_interleave(2, 2), sk], -1)
+    yr = torch.relu(ref_conv(xr, w, b, 9, c1 + c2, cout))
+    assert rel(y.float().cpu(), yr) < 1e-2  # output rounded to bf16
+
+
+def test_unet_inference_forward_equals_training_forward(sg):
+    """The inference forward (decoder inputs read without concatenation)
+    returns the training forward's bits."""
+    dev = "cuda:0"
+    net = sg.UNet(seed=6, device=dev)
+    g = torch.Generator().manual_seed(11)
+    x = (torch.randn(2, 64, 48, 8, generator=g) * 0.5).to(dev).to(torch.bfloat16).contiguous()
+    y1 = net.forward(x, keep=True)
+    y2 = net.forward(x, keep=False)
+    assert torch.equal(y1.view(torch.int32), y2.view(torch.int32))
+    with pytest.raises(RuntimeError):
+        net.backward(torch.zeros(2, 64, 48, 8, dtype=torch.bfloat16, device=dev))
 
 
 def test_unet_training_reduces_loss(sg):

@@ -90,3 +90,41 @@ def test_gate_accepts_trained_and_rejects_untrained_with_restore(ts, trained):
             assert w.cycle == cyc
             assert np.array_equal(w.cells().view(np.uint32), before.view(np.uint32))
         w.close()
+
+
+def test_terrain_training_data_through_the_ensemble(ts):
+    """The C4-scale data path at a small size: terrains generated in one
+    ensemble hold, per terrain, what a single World of the same seed and
+    terrain number computes (terrain 0 checked against a World), and a short
+    training run on them lowers the loss."""
+    from simplehydrology_amd import surrogate as sg
+    import simplehydrology_amd as shy
+    n, m, c0, t0, t1, seed0 = 128, 6, 5, 15, 25, 4242
+    print("terrain data", n, m, flush=True)
+    feats, targets, info = ts.generate_terrains(m, n, c0, t0, t1, seed0=seed0, per_batch=4,
+                                                log=lambda *_: None)
+    assert feats.shape == (m, n, n, 8) and targets.shape == (m, n, n, 3)
+    assert info["particle_steps"] > 0
+    # terrain 0 of the first ensemble equals a World of its seed (terrain id 0)
+    p = shy.Params(depositionRate=0.0)
+    w = shy.World(n, n, seed=seed0, params=p)
+    w.generate_fbm(seed0, 8)
+    rec = ts.Recorder(w)
+    w.erode_cycles(c0, n * n // 16, graph=False)
+    st = rec.grab()
+    f = sg.window_batch(sg.features(st[..., 0], st[..., 1], st[..., 2], st[..., 3]), [(0, 0)], n)
+    assert torch.equal(f[0].view(torch.int16), feats[0].view(torch.int16))
+    w.erode_cycles(t0 - c0, n * n // 16, graph=False)
+    acc = torch.zeros(n, n, 3, device="cuda:0")
+    for _ in range(t1 - t0):
+        w.erode_cycles(1, n * n // 16, graph=False)
+        acc += rec.grab()[..., 1:4]
+    acc /= (t1 - t0)
+    want = sg.encode_flow(acc[..., 0], acc[..., 1], acc[..., 2]).half()
+    assert torch.equal(want.view(torch.int16), targets[0].view(torch.int16))
+    w.close()
+    net = sg.UNet(seed=0)
+    hist, _ = ts.train(net, feats, targets, steps=40, batch=4, lr=2e-3, log=lambda *_: None,
+                       log_every=10, whole_maps=True)
+    print("losses", [h["loss"] for h in hist], flush=True)
+    assert hist[-1]["loss"] < hist[0]["loss"]
