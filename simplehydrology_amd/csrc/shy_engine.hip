@@ -86,6 +86,7 @@ struct shy_ctx {
   size_t part_entries = 0;   // count-matrix entries over all terrains
   int part_rows = 0;         // log rows
   int part_window = 0;       // most tiles of a terrain
+  int part_maxw = 0;         // widest partition block (drops)
   bool walk_counts = false;  // the walk builds the partition's tile counts
   int ntiles = 0;            // cell tiles of kTileCells cells
   int32_t* tcount = nullptr;    // per-part tile counts (kTileSplit parts per tile)
@@ -347,6 +348,8 @@ int ensure_wave(shy_ctx* c) {
   c->part_entries = entries;
   c->part_rows = rows;
   c->part_window = window;
+  c->part_maxw = 0;
+  for (const Terrain& t : c->ter) c->part_maxw = std::max(c->part_maxw, t.part_dwidth);
   const size_t hist_need = entries + 1;
   const int qstride = (rows + kReplayDepth - 1) / kReplayDepth * kReplayDepth;
   const size_t qb_need = (size_t)c->total_drops * qstride;
@@ -549,13 +552,23 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     scan_sums_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, nh, c->hbsum);
     scan_top_kernel<<<1, kScanBlock, 0, c->stream>>>(c->hbsum, nbh);
     scan_apply_kernel<<<nbh, kScanBlock, 0, c->stream>>>(c->hist, c->hbsum, nh);
-    if (c->part_window <= 1024 && nb <= kBigRoundBlocks)
-      part_scatter_staged_kernel<1024, 8192><<<nb, 1024, 0, c->stream>>>(pa);
-    else if (c->part_window <= 1024)
-      part_scatter_staged_kernel<1024, 4096><<<nb, 1024, 0, c->stream>>>(pa);
-    else if (c->part_window <= 4096)
-      part_scatter_staged_kernel<4096, 4096><<<nb, 1024, 0, c->stream>>>(pa);
-    else
+    const int mw = c->part_maxw;
+    if (c->part_window <= 1024 && nb <= kBigRoundBlocks) {
+      if (mw <= scatter_lds_drops<1024, 8192>())
+        part_scatter_staged_kernel<1024, 8192, true><<<nb, 1024, 0, c->stream>>>(pa);
+      else
+        part_scatter_staged_kernel<1024, 8192, false><<<nb, 1024, 0, c->stream>>>(pa);
+    } else if (c->part_window <= 1024) {
+      if (mw <= scatter_lds_drops<1024, 4096>())
+        part_scatter_staged_kernel<1024, 4096, true><<<nb, 1024, 0, c->stream>>>(pa);
+      else
+        part_scatter_staged_kernel<1024, 4096, false><<<nb, 1024, 0, c->stream>>>(pa);
+    } else if (c->part_window <= 4096) {
+      if (mw <= scatter_lds_drops<4096, 4096>())
+        part_scatter_staged_kernel<4096, 4096, true><<<nb, 1024, 0, c->stream>>>(pa);
+      else
+        part_scatter_staged_kernel<4096, 4096, false><<<nb, 1024, 0, c->stream>>>(pa);
+    } else
       part_scatter_kernel<kPartMaxTiles><<<nb, 1024, 0, c->stream>>>(pa);
   }
   tile_start_kernel<<<(c->ntiles + 256) / 256, 256, 0, c->stream>>>(

@@ -1037,13 +1037,23 @@ __global__ void __launch_bounds__(1024) order_kernel(const Terrain* __restrict__
 // entries of one tile go to consecutive addresses.  LDS holds the block's
 // terrain's tiles (at most MAXT, which the host checks) and RS entries.
 template <int MAXT, int RS>
+constexpr int scatter_lds_drops() {
+  return (MAXT <= 1024 && RS <= 4096) ? 1008 : 4096;
+}
+
+// LDSW: the block's step counts are staged in LDS (every block width at most
+// scatter_lds_drops, which the host checks); else they are read from memory.
+// The choice is made at compile time: a pointer that may address either
+// makes every read of it a flat access, whose wait covers all memory loads
+// in flight, and so would serialise the next round's loads.
+template <int MAXT, int RS, bool LDSW>
 // (rounds of 4096: two blocks of 1024 threads per CU, at most 64 VGPRs;
 // rounds of 8192 fill the LDS of a CU with one block)
 __global__ void __launch_bounds__(1024, RS <= 4096 ? 8 : 4) part_scatter_staged_kernel(PartArgs a) {
   constexpr int PER = RS / 1024;
   // drops whose step counts the block keeps in LDS: with 1024 tiles and 4096
   // staged entries the block stays within 80 KB, two blocks per CU
-  constexpr int kMaxW = (MAXT <= 1024 && RS <= 4096) ? 1008 : 4096;
+  constexpr int kMaxW = LDSW ? scatter_lds_drops<MAXT, RS>() : 1;
   __shared__ int cur[MAXT];  // next entry of each tile of the block's terrain
   __shared__ int cnt[MAXT];  // the round's entries per tile
   __shared__ int off[MAXT];  // their exclusive offsets in the stage
@@ -1055,57 +1065,74 @@ __global__ void __launch_bounds__(1024, RS <= 4096 ? 8 : 4) part_scatter_staged_
   const int g0 = col.g0, width = col.width, ns = col.nslots;
   if (ns == 0) return;  // uniform over the block
   const int wlo = col.wlo, wn = col.wn;
-  const bool lds_steps = width <= kMaxW;
-  if (lds_steps)
+  if (LDSW)
     for (int d = tid; d < width; d += 1024) wst[d] = a.steps[g0 + d];
   for (int t = tid; t < wn; t += 1024) cur[t] = a.hist[col.base + (long long)t * col.stride];
   __syncthreads();
-  // log index of the block's slot x, or -1 (past the slots, or the drop took
-  // fewer steps)
-  auto slot = [&](int x) {
-    if (x >= ns) return -1;
-    const int row = x / width, d = x - row * width;
-    const int st = lds_steps ? wst[d] : a.steps[g0 + d];
-    return row < st ? row * a.drops + g0 + d : -1;
+  // A thread's slots are tid, tid + 1024, tid + 2048, ... (round r0 takes
+  // r0 + u * 1024 + tid for u < PER): slot x is row x / width, drop x %
+  // width, kept as a cursor advanced by 1024 slots at a time.  A slot is
+  // valid when its row lies below its drop's step count (steps <= rows, so
+  // that also bounds x < ns).
+  const int qa = 1024 / width, ra = 1024 - qa * width;
+  int crow = tid / width, cd = tid - crow * width;
+  auto next_slot = [&]() {
+    const int st = LDSW ? wst[cd] : a.steps[g0 + cd];
+    const int k = crow < st ? crow * a.drops + g0 + cd : -1;
+    cd += ra;
+    crow += qa;
+    if (cd >= width) {
+      cd -= width;
+      ++crow;
+    }
+    return k;
   };
-  // The next round's log records are loaded while this round is ranked,
-  // staged and written.  The loads are unconditional (an invalid slot reads
-  // record 0, unused) so that no wait inside the round covers them.
+  // The next round's log records are loaded while this round is ranked and
+  // staged, and taken into registers before this round's entries are
+  // written out: the wait for them then does not cover those stores.  The
+  // loads are unconditional (an invalid slot reads record 0, unused); the
+  // slots are found first, then their loads are issued together.
   int kq[PER], lc[PER];
   int2 lt[PER];
+  auto load_round = [&]() {
 #pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    kq[u] = slot(u * 1024 + tid);
-    const int k = kq[u] >= 0 ? kq[u] : 0;
-    lc[u] = a.slog[k].x;
-    lt[u] = a.strk[k];
-  }
-  for (int r0 = 0; r0 < ns; r0 += RS) {
-    for (int t = tid; t < wn; t += 1024) cnt[t] = 0;
-    int4 ev[PER];
-    int tl[PER], rk[PER];
+    for (int u = 0; u < PER; ++u) kq[u] = next_slot();
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      ev[u] = make_int4(kq[u], lc[u], lt[u].x, lt[u].y);
-      tl[u] = kq[u] >= 0 ? (lc[u] >> a.tb) - wlo : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      kq[u] = slot(r0 + RS + u * 1024 + tid);
       const int k = kq[u] >= 0 ? kq[u] : 0;
       lc[u] = a.slog[k].x;
       lt[u] = a.strk[k];
     }
+  };
+  int4 ev[PER];
+  int rk[PER];
+  auto take_round = [&]() {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) ev[u] = make_int4(kq[u], lc[u], lt[u].x, lt[u].y);
+  };
+  // tile of staged record u (-1: no step)
+  auto tile_of = [&](int u) { return ev[u].x >= 0 ? (ev[u].y >> a.tb) - wlo : -1; };
+  load_round();
+  take_round();
+  for (int r0 = 0; r0 < ns; r0 += RS) {
+    for (int t = tid; t < wn; t += 1024) cnt[t] = 0;
+    load_round();  // round r0 + RS
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < PER; ++u) rk[u] = tl[u] >= 0 ? atomicAdd(&cnt[tl[u]], 1) : 0;
+    for (int u = 0; u < PER; ++u) {
+      const int t = tile_of(u);
+      rk[u] = t >= 0 ? atomicAdd(&cnt[t], 1) : 0;
+    }
     __syncthreads();
     block_scan_runs(cnt, off, wn, wsum);
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < PER; ++u)
-      if (tl[u] >= 0) stage[off[tl[u]] + rk[u]] = ev[u];
+    for (int u = 0; u < PER; ++u) {
+      const int t = tile_of(u);
+      if (t >= 0) stage[off[t] + rk[u]] = ev[u];
+    }
     __syncthreads();
+    take_round();
     const int tot = off[wn - 1] + cnt[wn - 1];
     for (int i = tid; i < tot; i += 1024) {
       const int4 v = stage[i];
