@@ -18,7 +18,8 @@
 //   slog, strk  int2x2  [step][drop] step log of the count walk: {start cell,
 //                       c_eq} (what the replay reads) and {momentum tracks}
 //   qb          int32   [drop][step] height commit of each step (replay)
-//   ent         int4    the valid slots grouped by cell tile (partition)
+//   ent         int32 + {int32x3}  the valid slots grouped by cell tile
+//                       (partition): start cells, then {slot, tracks}
 //   vq          int32   [terrain][step] quantised volume track of a step
 //   dlog, wsteps int4, int32 [drop] how each walk ended, its steps
 //   hist        int32   per terrain a [tile][block] count matrix of the
