@@ -29,7 +29,7 @@
 //                       cycle's start (height, normal, flow), by pack_kernel
 //   sr          float   burst scale per cell (tile_count_kernel)
 //   tcount, tsum int32, int64x4  per-part tile counts and sums when a tile is
-//                       split over kTileSplit blocks
+//                       split over kCountSplit / kReduceSplit blocks
 // A device table of Terrain entries maps drops, cells and cascade tiles to
 // their terrain, so every kernel covers the whole ensemble in one launch.
 // Parity p = cycle & 1 selects h and visit; the cascade/blend kernel of cycle
@@ -90,9 +90,9 @@ struct shy_ctx {
   int part_maxw = 0;         // widest partition block (drops)
   bool walk_counts = false;  // the walk builds the partition's tile counts
   int ntiles = 0;            // cell tiles of kTileCells cells
-  int32_t* tcount = nullptr;    // per-part tile counts (kTileSplit parts per tile)
-  long long* tsum = nullptr;    // per-part tile sums
-  int tile_split = 1;           // kTileSplit when the part buffers are small, else 1
+  int32_t* tcount = nullptr;    // per-part tile counts (kCountSplit parts per tile)
+  long long* tsum = nullptr;    // per-part tile sums (kReduceSplit parts per tile)
+  bool tile_split = false;      // tiles split over several blocks (the part buffers are small)
   int4* dlog = nullptr;  // [drop] how each walk ended
   int32_t* dperm = nullptr;  // [drop] the terrain's drop each walk thread takes (order_kernel)
   int drop_order = -1;       // assign drops to walk threads by spawn position: 1, 0, -1 auto
@@ -261,7 +261,12 @@ size_t wave_steps(const shy_ctx* c) {
 // Partition blocks: at most kPartMaxBlocks, each a whole number of waves of
 // drops over all log rows (the [tile][block] matrix stays small).
 constexpr int kPartMaxBlocks = 512;
-constexpr int kTileSplit = 4;             // blocks per tile in the tile kernels
+// blocks per tile in the tile kernels: the largest tiles hold several times
+// the mean's entries; the counts split 4 ways, the sums (whose parts are
+// 32 B per cell) 2 ways (1024^2 cycle: 1.185 ms at 4 and 4, 1.171 at 2 and 2,
+// 1.165 at 4 and 2)
+constexpr int kCountSplit = 4;
+constexpr int kReduceSplit = 2;
 constexpr int kTileSplitMaxTiles = 2048;
 // partition blocks up to which each takes a CU's whole LDS (the scatter's
 // rounds of 8192 entries): no more blocks than CUs
@@ -382,22 +387,22 @@ int ensure_wave(shy_ctx* c) {
     }
   }
   {
-    // split each tile's entries over kTileSplit blocks while the per-part
-    // buffers stay modest (up to 2048 tiles: 1 GiB of partial sums)
-    const int split = c->ntiles <= kTileSplitMaxTiles ? kTileSplit : 1;
-    if (split != c->tile_split || (split > 1 && !c->tcount)) {
+    // split each tile's entries over several blocks while the per-part
+    // buffers stay modest (up to 2048 tiles: 640 MiB of partial sums)
+    const bool split = c->ntiles <= kTileSplitMaxTiles;
+    if (split != c->tile_split || (split && !c->tcount)) {
       drop_graph(c);
       HIPCHK(c, hipStreamSynchronize(c->stream));
       if (c->tcount) hipFree(c->tcount);
       if (c->tsum) hipFree(c->tsum);
       c->tcount = nullptr;
       c->tsum = nullptr;
-      c->tile_split = 1;
-      if (split > 1) {
-        const size_t parts = (size_t)c->ntiles * split * kTileCells;
-        HIPCHK(c, hipMalloc(&c->tcount, parts * sizeof(int32_t)));
-        HIPCHK(c, hipMalloc(&c->tsum, 4 * parts * sizeof(long long)));
-        c->tile_split = split;
+      c->tile_split = false;
+      if (split) {
+        const size_t cells = (size_t)c->ntiles * kTileCells;
+        HIPCHK(c, hipMalloc(&c->tcount, cells * kCountSplit * sizeof(int32_t)));
+        HIPCHK(c, hipMalloc(&c->tsum, 4 * cells * kReduceSplit * sizeof(long long)));
+        c->tile_split = true;
       }
     }
   }
@@ -574,11 +579,11 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   }
   tile_start_kernel<<<(c->ntiles + 256) / 256, 256, 0, c->stream>>>(
       c->d_ter, nter, c->ntiles, nh > 0 ? c->hist : nullptr, (long long)nh, c->tstart);
-  if (c->tile_split == kTileSplit) {
-    tile_count_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
+  if (c->tile_split) {
+    tile_count_kernel<kCountSplit><<<c->ntiles * kCountSplit, 1024, 0, c->stream>>>(
         c->tstart, pa.ecell, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter,
         c->tcount);
-    tile_count_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+    tile_count_finish_kernel<kCountSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
         c->tcount, n, c->sr, c->first, c->visit[p], c->fld, c->root, c->d_ter, nter);
   } else {
     tile_count_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(c->tstart, pa.ecell, n, c->sr,
@@ -598,11 +603,11 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     HIPCHK(c, hipGetLastError());
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
-  if (c->tile_split == kTileSplit) {
-    tile_reduce_kernel<kTileSplit><<<c->ntiles * kTileSplit, 1024, 0, c->stream>>>(
+  if (c->tile_split) {
+    tile_reduce_kernel<kReduceSplit><<<c->ntiles * kReduceSplit, 1024, 0, c->stream>>>(
         c->tstart, pa.ecell, pa.erest, c->qb, c->qstride, c->vq, c->vq_rows, pa.drops, c->d_ter, nter,
         c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n, c->tsum);
-    tile_reduce_finish_kernel<kTileSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+    tile_reduce_finish_kernel<kReduceSplit><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
         c->tsum, c->deposit, n, c->dh, c->trk, c->trk + n, c->trk + 2 * n);
   } else {
     tile_reduce_kernel<1><<<c->ntiles, 1024, 0, c->stream>>>(
