@@ -33,6 +33,10 @@ workload, else null.  CPU baseline: the oracle's sequential restatement of
 World::erode on the same terrain and spawn stream, one core, on a bounded
 sample of cycles.
 
+"walk_lifetimes": how the last timed cycle's drops ended and a histogram of
+their steps (shy_terrain_drop_log): the walk's time is set by its longest
+chains, the drops that reach maxAge + 1 = 501 steps.
+
 The default line also carries "ensemble": the batched ensemble of 64 uneven
 terrains (sides 128..1024, per-terrain parameters, 1 drop per 16 cells) in
 one launch per kernel, autotuned and timed the same way (--no-ensemble skips
@@ -145,6 +149,28 @@ def cpu_baseline(n: int, drops: int, seed: int, cycles: int) -> dict:
                       f"{n}x{n} fBm terrain (seed {seed}), {steps} particle-steps in {dt:.1f} s, "
                       f"1 thread on {cpu_model()}",
             "seconds": dt, "particle_steps": steps}
+
+
+def lifetimes(steps, ends) -> dict:
+    """How the last timed cycle's drops ended and how many steps they took:
+    the walk's time is set by its longest chains (maxAge + 1 = 501 steps)."""
+    import numpy as np
+    n = len(steps)
+    walked = ends != 3
+    tot = int(steps.sum())
+    full = steps == steps.max() if n else steps
+    edges = [0, 1, 11, 51, 101, 201, 301, 401, 501, 502]
+    hist = np.histogram(steps[walked], bins=edges)[0] if n else np.zeros(len(edges) - 1, int)
+    labels = ["0", "1-10", "11-50", "51-100", "101-200", "201-300", "301-400", "401-500", "501"]
+    return {
+        "drops": n, "particle_steps": tot, "max_steps": int(steps.max()) if n else 0,
+        "ends": {k: int((ends == v).sum()) for k, v in
+                 (("maxAge", 0), ("minVol", 1), ("left_map", 2), ("not_walked", 3))},
+        "steps_histogram": dict(zip(labels, (int(x) for x in hist))),
+        "mean_steps_walked": float(steps[walked].mean()) if walked.any() else 0.0,
+        "longest_drops_share": {"drops": float(full.mean()) if n else 0.0,
+                                "of_steps": float(steps[full].sum() / max(tot, 1))},
+    }
 
 
 def ensemble_specs(shy, n_terrains: int, seed: int):
@@ -332,6 +358,7 @@ def main():
     tm = w.time_cycles(args.steps, drops)  # synchronises at the end
     wall = time.perf_counter() - t0
     s1 = w.stats()
+    life = lifetimes(*w.drop_log())
     steps = s1["steps"] - s0["steps"]
     sec = tm["ms_total"] / 1e3
     value = steps / sec
@@ -357,6 +384,7 @@ def main():
         "host_wall_s": wall,
         "kernel_ms_per_cycle": {k[3:]: v / args.steps for k, v in tm.items() if k != "ms_total"},
         "roofline": roofline(tm, args.steps, steps, n * n, f"single{n}_drops{drops}"),
+        "walk_lifetimes": life,
     }
     w.close()
     if not args.no_ensemble:

@@ -170,6 +170,15 @@ int shy_terrain_export(shy_ctx* ctx, int terrain, float* dst4_device);
 int shy_terrain_import_flow(shy_ctx* ctx, int terrain, const float* src3_device);
 /* particle-steps per terrain since the last shy_reset_stats */
 int shy_terrain_steps(shy_ctx* ctx, int64_t* steps);
+/* The last cycle's walk of terrain t, one entry per drop (no reference
+   counterpart; for measuring the walk): its particle-steps and how it ended
+   (0 maxAge reached, water.h:74-76; 1 volume below minVol, water.h:78-82;
+   2 left the map, water.h:139-142; 3 not walked, spawn height below 0.1,
+   world.h:71-72), in walk-thread order.  Writes at most `capacity` entries
+   and the number written to *count; *count = 0 before the first cycle or
+   after the parameters or drop counts changed. */
+int shy_terrain_drop_log(shy_ctx* ctx, int terrain, int32_t* steps, int32_t* ends, int capacity,
+                         int* count);
 /* `cycles` World::erode calls of every terrain (each with its own drops);
    use_graph=1 replays two-cycle hipGraphs. */
 int shy_ensemble_erode(shy_ctx* ctx, int cycles, int use_graph);

@@ -231,6 +231,7 @@ def lib():
             "shy_terrain_generate_fbm": (i32, [vp, i32, u64, i32, C.c_float, C.c_float,
                                               C.c_float]),
             "shy_terrain_steps": (i32, [vp, vp]),
+            "shy_terrain_drop_log": (i32, [vp, i32, vp, vp, i32, C.POINTER(C.c_int)]),
             "shy_ensemble_erode": (i32, [vp, i32, i32]),
             "shy_set_cycle": (i32, [vp, u32]),
             "shy_set_tuning": (i32, [vp, vp]),
@@ -265,6 +266,21 @@ def _cycle_times(ms) -> dict:
     """Slots of shy_time_cycles: total, then the five kernel phases."""
     return {"ms_total": ms[0], "ms_pack": ms[1], "ms_count_walk": ms[2],
             "ms_partition": ms[3], "ms_replay": ms[4], "ms_reduce_cascade": ms[5]}
+
+
+def _drop_log(ctx, terrain: int):
+    """(steps, ends) of the last cycle's drops of one terrain (see
+    shy_terrain_drop_log; ends: 0 maxAge, 1 minVol, 2 left the map, 3 not
+    walked)."""
+    nx, ny, drops = C.c_int(), C.c_int(), C.c_int()
+    _check(ctx, lib().shy_terrain_size(ctx, terrain, C.byref(nx), C.byref(ny), C.byref(drops)))
+    cap = max(drops.value, 0)
+    steps = np.zeros(cap, np.int32)
+    ends = np.zeros(cap, np.int32)
+    n = C.c_int(0)
+    _check(ctx, lib().shy_terrain_drop_log(ctx, terrain, steps.ctypes.data, ends.ctypes.data, cap,
+                                           C.byref(n)))
+    return steps[:n.value], ends[:n.value]
 
 
 def _check(ctx, rc: int):
@@ -520,6 +536,10 @@ class World:
         _torch_sync()
         _check(self._ctx, lib().shy_terrain_import_flow(self._ctx, 0, src_ptr))
 
+    def drop_log(self):
+        """(steps, ends) of the last cycle's walk, one entry per drop."""
+        return _drop_log(self._ctx, 0)
+
     def time_cycles(self, cycles: int, drops: int) -> dict:
         """Run ``cycles`` erode calls with HIP events between kernels on the
         context's stream; returns milliseconds per kernel, summed."""
@@ -606,6 +626,10 @@ class Ensemble(World):
         """(d, mx, my) of one terrain from a device buffer (3 floats/cell)."""
         _torch_sync()
         _check(self._ctx, lib().shy_terrain_import_flow(self._ctx, terrain, src_ptr))
+
+    def drop_log(self, terrain: int):
+        """(steps, ends) of the last cycle's walk of one terrain's drops."""
+        return _drop_log(self._ctx, terrain)
 
     def terrain_steps(self) -> np.ndarray:
         out = np.zeros(self.nter, np.int64)

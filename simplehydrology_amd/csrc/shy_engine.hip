@@ -94,6 +94,7 @@ struct shy_ctx {
   long long* tsum = nullptr;    // per-part tile sums (kReduceSplit parts per tile)
   bool tile_split = false;      // tiles split over several blocks (the part buffers are small)
   int4* dlog = nullptr;  // [drop] how each walk ended
+  bool dlog_valid = false;  // dlog holds a cycle of the current wave geometry
   int32_t* dperm = nullptr;  // [drop] the terrain's drop each walk thread takes (order_kernel)
   int drop_order = -1;       // assign drops to walk threads by spawn position: 1, 0, -1 auto
   int order_chunks = 0;      // blocks of order_kernel
@@ -243,6 +244,7 @@ int layout(shy_ctx* c) {
   c->total_tiles = (int)tiles;
   c->ntiles = (int)((cells + kTileCells - 1) / kTileCells);
   c->vq_dirty = true;
+  c->dlog_valid = false;
   drop_graph(c);
   if (c->d_ter) {
     HIPCHK(c, hipMemcpyAsync(c->d_ter, c->ter.data(), c->ter.size() * sizeof(Terrain),
@@ -530,6 +532,7 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
     else
       (pf ? walk_kernel<false, true> : walk_kernel<false, false>)<<<grid, bs, 0, c->stream>>>(w);
     HIPCHK(c, hipGetLastError());
+    c->dlog_valid = true;
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   // partition of the valid log slots by cell tile (empty without drops)
@@ -1659,6 +1662,35 @@ int shy_terrain_steps(shy_ctx* c, int64_t* steps) {
   HIPCHK(c, hipMemcpyAsync(steps, c->tsteps, c->ter.size() * sizeof(long long),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SHY_OK;
+}
+
+int shy_terrain_drop_log(shy_ctx* c, int t, int32_t* steps, int32_t* ends, int capacity,
+                         int* count) {
+  RETURN_IF(check_terrain(c, t));
+  if (!count || capacity < 0 || (capacity > 0 && (!steps || !ends)))
+    return fail(c, SHY_ERR_ARG, "shy_terrain_drop_log: bad output arguments");
+  *count = 0;
+  if (!c->dlog_valid || !c->dlog) return SHY_OK;
+  hipSetDevice(c->device);
+  const Terrain& T = c->ter[t];
+  const int end = t + 1 < (int)c->ter.size() ? c->ter[t + 1].drop_off : c->total_drops;
+  const int padded = end - T.drop_off;
+  std::vector<int4> h((size_t)std::max(padded, 0));
+  if (padded > 0) {
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->dlog + T.drop_off, (size_t)padded * sizeof(int4),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  int k = 0;
+  for (const int4& d : h) {
+    if (d.y == kEndPad) continue;
+    if (k >= capacity) break;
+    steps[k] = d.x;
+    ends[k] = d.y;
+    ++k;
+  }
+  *count = k;
   return SHY_OK;
 }
 

@@ -163,3 +163,27 @@ def test_cpp_facade_demo_matches_python_path(shy):
             steps += 1
         assert out["drop_steps"] == steps
         w.close()
+
+
+def test_drop_log_matches_the_cycle_ledger(shy):
+    """shy_terrain_drop_log: one entry per drop of the last cycle, whose steps
+    and end kinds add up to that cycle's ledger; empty before a cycle and
+    after a parameter change."""
+    n, drops = 128, 2000
+    w = shy.World(n, n, seed=5)
+    w.generate_fbm(5, 8)
+    steps, ends = w.drop_log()
+    assert len(steps) == 0
+    w.erode_cycles(2, drops, graph=False)
+    w.reset_stats()
+    w.erode(drops)
+    st = w.stats()
+    steps, ends = w.drop_log()
+    assert len(steps) == drops and len(ends) == drops
+    assert int(steps.sum()) == st["steps"]
+    for code, key in ((0, "ends_age"), (1, "ends_vol"), (2, "exits"), (3, "skipped")):
+        assert int((ends == code).sum()) == st[key], key
+    assert steps.max() <= 501 and (steps[ends == 3] == 0).all()
+    w.params = shy.Params(evapRate=0.002)
+    assert len(w.drop_log()[0]) == 0
+    w.close()

@@ -148,3 +148,12 @@ def test_partition_more_blocks_than_cus(shy, orc):
     # walk blocks of 64: 20480 drops = 320 partition blocks, more than the CUs,
     # so the scatter runs rounds of 4096 (two blocks per CU) instead of 8192
     run_pair(shy, orc, 512, 6, [(20480, None), (20480, None)], walk_block=64)
+
+
+def test_partition_block_steps_read_from_memory(shy, orc):
+    # walk blocks of 1024 threads and 266240 drops: 260 partition blocks of
+    # 1024 drops, more than the CUs, so the scatter runs rounds of 4096, whose
+    # LDS holds the step counts of at most 1008 drops: it reads them from
+    # memory (evapRate 0.1 keeps the oracle's walks short)
+    P = orc.Params.default(evapRate=0.1)
+    run_pair(shy, orc, 512, 8, [(266240, P), (266240, None)], params=P, walk_block=1024)
