@@ -39,6 +39,16 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
 int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, const float* bias,
                           const void* res, void* y_bf16, float* y_f32, int N, int H, int W,
                           int C1, int C2, int Cout, int relu, void* stream);
+/* The data gradient dx [N][H][W][Cin] (BF16) of a convolution from g
+   [N][H][W][Cout] with the flipped weights wf of shy_nn_flip_weights, with
+   the backward pass's next two element-wise steps in its epilogue: gres
+   (optional, [N][H][W][Cin]) added to the rounded result, then the result
+   zeroed where mask (optional, the forward input [N][H][W][Cin]) is not
+   positive -- the same bits as shy_nn_conv_fwd, shy_nn_add and
+   shy_nn_relu_bwd in sequence.  Cin, Cout multiples of 32. */
+int shy_nn_conv_dgrad(const void* g, const void* wf, const void* gres, const void* mask,
+                      void* dx_bf16, int N, int H, int W, int Cin, int Cout, int taps,
+                      void* stream);
 /* weights for the data gradient: wf[Cin][taps*Cout] = w[Cout][(taps-1-t)*Cin] */
 int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, void* stream);
 /* dw[Cout][taps*Cin] (FP32) = sum over pixels of g[p][co] x[p+tap][ci];

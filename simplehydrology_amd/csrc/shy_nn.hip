@@ -68,7 +68,47 @@ struct ConvArgs {
   // a nearest x2 upsample of x ([N][H/2][W/2][C1]) and x2 ([N][H][W][Cin-C1]).
   const __bf16* __restrict__ x2;
   int C1;
+  // Backward-pass epilogue of the LDS-staged kernels (bf16 output): after
+  // rounding, gres ([P][Cout]) is added (in float, rounded again) and the
+  // result is zeroed where mask ([P][Cout]) is not positive: the bits of
+  // shy_nn_add then shy_nn_relu_bwd on the rounded output.
+  const __bf16* __restrict__ gres;
+  const __bf16* __restrict__ mask;
 };
+
+// One lane's 4 consecutive output channels co..co+3 of output element o:
+// bias, residual (before rounding), ReLU, then the store, FP32 or BF16 (with
+// the backward-pass residual and mask after rounding).
+__device__ inline void store_out4(const ConvArgs& a, size_t o, int co, const f32x4& acc) {
+  float v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = acc[i] + (a.bias ? a.bias[co + i] : 0.0f);
+  if (a.res) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
+  }
+  if (a.relu) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+  }
+  if (a.y_f) {
+    *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    bf16x4 q;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
+    if (a.gres) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[i] = (__bf16)((float)q[i] + (float)a.gres[o + i]);
+    }
+    if (a.mask) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[i] = (float)a.mask[o + i] > 0.0f ? q[i] : (__bf16)0.0f;
+    }
+    *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
+  }
+}
 
 // Forward (and dgrad) convolution.  Block = 4 waves; each wave owns 32 pixels
 // x 16*NT output channels as 2 x NT MFMA accumulators.
@@ -295,27 +335,7 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
     for (int m = 0; m < MT; ++m) {
       const int co = cb + m * 16 + (lane >> 4) * 4;
       if (co >= a.Cout) continue;
-      float v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
-      const size_t o = p * a.Cout + co;
-      if (a.res) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
-      }
-      if (a.relu) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
-      }
-      if (a.y_f) {
-        *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
-      } else {
-        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-        bf16x4 q;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
-        *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
-      }
+      store_out4(a, p * a.Cout + co, co, acc[m][t]);
     }
   }
 }
@@ -463,27 +483,7 @@ __global__ void __launch_bounds__(256, 2) conv_tiles_kernel(ConvArgs a) {
       for (int m = 0; m < MT; ++m) {
         const int co = cb + m * 16 + (lane >> 4) * 4;
         if (co >= a.Cout) continue;
-        float v[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
-        const size_t o = p * a.Cout + co;
-        if (a.res) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
-        }
-        if (a.relu) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
-        }
-        if (a.y_f) {
-          *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
-          typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-          bf16x4 q;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
-          *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
-        }
+        store_out4(a, p * a.Cout + co, co, acc[m][t]);
       }
     }
     tile += (int)gridDim.x;
@@ -560,27 +560,7 @@ __global__ void __launch_bounds__(256) conv_c8_kernel(ConvArgs a) {
     for (int m = 0; m < MT; ++m) {
       const int co = cb + m * 16 + (lane >> 4) * 4;
       if (co >= a.Cout) continue;
-      float v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = acc[m][t][i] + (a.bias ? a.bias[co + i] : 0.0f);
-      const size_t o = p * a.Cout + co;
-      if (a.res) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
-      }
-      if (a.relu) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
-      }
-      if (a.y_f) {
-        *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
-      } else {
-        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-        bf16x4 qv;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) qv[i] = (__bf16)v[i];
-        *reinterpret_cast<bf16x4*>(a.y_bf + o) = qv;
-      }
+      store_out4(a, p * a.Cout + co, co, acc[m][t]);
     }
   }
 }
@@ -1262,6 +1242,8 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
   a.relu = relu;
   a.x2 = nullptr;
   a.C1 = 0;
+  a.gres = nullptr;
+  a.mask = nullptr;
   const long long P = (long long)N * H * W;
   if (P > INT32_MAX / 2) return nn_fail("conv_fwd: too many pixels");
   hipStream_t s = (hipStream_t)stream;
@@ -1308,7 +1290,35 @@ int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, c
   a.y_f = y_f32;
   a.N = N; a.H = H; a.W = W; a.Cin = C1 + C2; a.Cout = Cout; a.K = 9 * (C1 + C2);
   a.relu = relu;
+  a.gres = nullptr;
+  a.mask = nullptr;
   return launch_pf(a, 9, (hipStream_t)stream, "conv_fwd_upcat");
+}
+
+int shy_nn_conv_dgrad(const void* g, const void* wf, const void* gres, const void* mask,
+                      void* dx_bf16, int N, int H, int W, int Cin, int Cout, int taps,
+                      void* stream) {
+  if (!g || !wf || !dx_bf16) return nn_fail("conv_dgrad: bad pointers");
+  if (N < 1 || H < 1 || W < 1 || Cin < 32 || Cin % 32 || Cout < 32 || Cout % 32 ||
+      (taps != 9 && taps != 1))
+    return nn_fail("conv_dgrad: need Cin and Cout multiples of 32 and taps 9 or 1");
+  if ((long long)N * H * W > INT32_MAX / 2) return nn_fail("conv_dgrad: too many pixels");
+  ConvArgs a;
+  a.x = (const __bf16*)g;
+  a.w = (const __bf16*)wf;
+  a.bias = nullptr;
+  a.res = nullptr;
+  a.y_bf = (__bf16*)dx_bf16;
+  a.y_f = nullptr;
+  // the data gradient is the convolution of g (Cout channels) with the
+  // flipped weights into Cin channels
+  a.N = N; a.H = H; a.W = W; a.Cin = Cout; a.Cout = Cin; a.K = taps * Cout;
+  a.relu = 0;
+  a.x2 = nullptr;
+  a.C1 = 0;
+  a.gres = (const __bf16*)gres;
+  a.mask = (const __bf16*)mask;
+  return launch_pf(a, taps, (hipStream_t)stream, "conv_dgrad");
 }
 
 int shy_nn_flip_weights(const void* w, void* wf, int Cin, int Cout, int taps, void* stream) {

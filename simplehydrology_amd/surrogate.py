@@ -43,6 +43,7 @@ def nn():
                                       vp]),
             "shy_nn_conv_fwd_upcat": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32,
                                             i32, i32, vp]),
+            "shy_nn_conv_dgrad": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_flip_weights": (i32, [vp, vp, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad_noalloc": (i32, [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
@@ -175,8 +176,10 @@ class Conv:
                                        1 if relu else 0, _stream()))
         return y
 
-    def backward(self, x, g, shape, scratch, need_dx=True):
-        """g: gradient of the pre-activation output (bf16). Returns dx (bf16)."""
+    def backward(self, x, g, shape, scratch, need_dx=True, gres=None, mask=None):
+        """g: gradient of the pre-activation output (bf16). Returns dx (bf16);
+        with ``gres`` and ``mask`` (32-channel multiples) dx + gres, zeroed
+        where mask <= 0, from the data-gradient kernel's epilogue."""
         import torch
         N, H, W = shape
         P = N * H * W
@@ -190,6 +193,10 @@ class Conv:
         _ck(nn().shy_nn_flip_weights(_p(self.w_bf), _p(self.wf_bf), self.cin, self.cout,
                                      self.taps, _stream()))
         dx = torch.empty(N, H, W, self.cin, dtype=torch.bfloat16, device=g.device)
+        if gres is not None or mask is not None:
+            _ck(nn().shy_nn_conv_dgrad(_p(g), _p(self.wf_bf), _p(gres), _p(mask), _p(dx), N, H, W,
+                                       self.cin, self.cout, self.taps, _stream()))
+            return dx
         _ck(nn().shy_nn_conv_fwd(_p(g), _p(self.wf_bf), None, None, _p(dx), None, N, H, W,
                                  self.cout, self.cin, self.taps, 0, _stream()))
         return dx
@@ -258,10 +265,10 @@ class ResStage:
     def backward(self, g, shape, scratch, need_dx=True):
         x, ya, yb, yc = self.saved
         gc = _relu_bwd(yc, g)
-        gb = self.c.backward(yb, gc, shape, scratch)
-        gb = _relu_bwd(yb, gb)
-        ga = _add(gc, self.b.backward(ya, gb, shape, scratch))
-        ga = _relu_bwd(ya, ga)
+        # relu_bwd(yb, dgrad_c) and relu_bwd(ya, gc + dgrad_b) in the
+        # data-gradient epilogues (the same bits as the separate passes)
+        gb = self.c.backward(yb, gc, shape, scratch, mask=yb)
+        ga = self.b.backward(ya, gb, shape, scratch, gres=gc, mask=ya)
         return self.a.backward(x, ga, shape, scratch, need_dx=need_dx)
 
     def convs(self):

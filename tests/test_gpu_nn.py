@@ -197,6 +197,31 @@ def test_conv_upcat_equals_concatenated_input(sg, N, H, W, c1, c2, cout):
     assert rel(y.float().cpu(), yr) < 1e-2  # output rounded to bf16
 
 
+@pytest.mark.parametrize("N,H,W,cin,cout", [(2, 16, 24, 32, 32), (1, 40, 24, 64, 32),
+                                             (2, 8, 16, 32, 96)])
+def test_conv_dgrad_epilogue_equals_separate_passes(sg, N, H, W, cin, cout):
+    """shy_nn_conv_dgrad with gres and mask gives the bits of the plain data
+    gradient followed by shy_nn_add and shy_nn_relu_bwd (the backward pass of
+    a residual stage), with either option alone too."""
+    g = torch.Generator().manual_seed(cin + 3 * cout)
+    dev = "cuda:0"
+    conv = sg.Conv(cin, cout, 9, np.random.default_rng(1), dev)
+    x = torch.randn(N, H, W, cin, generator=g).to(dev).to(torch.bfloat16).contiguous()
+    gy = torch.randn(N, H, W, cout, generator=g).to(dev).to(torch.bfloat16).contiguous()
+    gres = torch.randn(N, H, W, cin, generator=g).to(dev).to(torch.bfloat16).contiguous()
+    scratch = sg.Scratch(dev)
+    shape = (N, H, W)
+    print("dgrad epilogue", N, H, W, cin, cout, flush=True)
+    d0 = conv.backward(x, gy, shape, scratch)
+    both = conv.backward(x, gy, shape, scratch, gres=gres, mask=x)
+    assert torch.equal(both.view(torch.int16), sg._relu_bwd(x, sg._add(gres, d0)).view(torch.int16))
+    only_mask = conv.backward(x, gy, shape, scratch, mask=x)
+    assert torch.equal(only_mask.view(torch.int16), sg._relu_bwd(x, d0).view(torch.int16))
+    only_res = conv.backward(x, gy, shape, scratch, gres=gres)
+    assert torch.equal(only_res.view(torch.int16), sg._add(gres, d0).view(torch.int16))
+    assert (both.float() == 0).float().mean() > 0.3  # the mask did zero about half
+
+
 def test_unet_inference_forward_equals_training_forward(sg):
     """The inference forward (decoder inputs read without concatenation)
     returns the training forward's bits."""
