@@ -110,6 +110,26 @@ __device__ inline void store_out4(const ConvArgs& a, size_t o, int co, const f32
   }
 }
 
+// The epilogue of a block's tile: lane holds channels cb + 16m + 4(lane/16)
+// + i of pixel (y0 + NT wave + t, x0 + lane%16).
+template <int MT, int NT>
+__device__ inline void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][NT], int n, int y0,
+                                     int x0, int cb, int wave, int lane) {
+  const int gx = x0 + (lane & 15);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int gy = y0 + wave * NT + t;
+    if (gy >= a.H || gx >= a.W) continue;
+    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int co = cb + m * 16 + (lane >> 4) * 4;
+      if (co >= a.Cout) continue;
+      store_out4(a, p * a.Cout + co, co, acc[m][t]);
+    }
+  }
+}
+
 // Forward (and dgrad) convolution.  Block = 4 waves; each wave owns 32 pixels
 // x 16*NT output channels as 2 x NT MFMA accumulators.
 template <int TAPS, int NT>
@@ -325,19 +345,7 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
       }
     }
   }
-  // epilogue: lane holds channels cb + 16m + 4(lane/16) + i of pixel (row, lane%16)
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
-    if (gy >= a.H || gx >= a.W) continue;
-    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int co = cb + m * 16 + (lane >> 4) * 4;
-      if (co >= a.Cout) continue;
-      store_out4(a, p * a.Cout + co, co, acc[m][t]);
-    }
-  }
+  conv_epilogue<MT, NT>(a, acc, n, y0, x0, cb, wave, lane);
 }
 
 // The same convolution with persistent blocks: block (b, co) takes the tiles
@@ -471,21 +479,8 @@ __global__ void __launch_bounds__(256, 2) conv_tiles_kernel(ConvArgs a) {
         __syncthreads();
       }
     }
-    // epilogue: lane holds channels cb + 16m + 4(lane/16) + i of pixel (row, lane%16)
     const int tx = tile % tiles_x, r = tile / tiles_x;
-    const int x0 = tx * kTW, y0 = (r % tiles_y) * kTH, n = r / tiles_y;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
-      if (gy >= a.H || gx >= a.W) continue;
-      const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
-#pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        const int co = cb + m * 16 + (lane >> 4) * 4;
-        if (co >= a.Cout) continue;
-        store_out4(a, p * a.Cout + co, co, acc[m][t]);
-      }
-    }
+    conv_epilogue<MT, NT>(a, acc, r / tiles_y, (r % tiles_y) * kTH, tx * kTW, cb, wave, lane);
     tile += (int)gridDim.x;
     if (tile >= total) break;
   }
@@ -551,18 +546,7 @@ __global__ void __launch_bounds__(256) conv_c8_kernel(ConvArgs a) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[m][t] = mfma(A[m], B[t], acc[m][t]);
   }
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int gy = y0 + wave * NT + t, gx = x0 + (lane & 15);
-    if (gy >= a.H || gx >= a.W) continue;
-    const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int co = cb + m * 16 + (lane >> 4) * 4;
-      if (co >= a.Cout) continue;
-      store_out4(a, p * a.Cout + co, co, acc[m][t]);
-    }
-  }
+  conv_epilogue<MT, NT>(a, acc, n, y0, x0, cb, wave, lane);
 }
 
 // dgrad weights: Wf[ci][tap'*Cout + co] = W[co][(TAPS-1-tap')*Cin + ci]
@@ -1105,8 +1089,6 @@ int nn_check(const char* what) {
 }
 inline unsigned blocks(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
-// conv_pf_kernel for Cin and Cout multiples of 32: 16x16 tiles (4 rows per
-// wave) when they still give >= 1024 blocks, else 16x8
 // Compute units of the current device (looked up once per device).
 int device_cus() {
   static std::atomic<int> cus[64];
@@ -1147,6 +1129,8 @@ int launch_tiles(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
   return nn_check(what);
 }
 
+// conv_pf_kernel for Cin and Cout multiples of 32: 16x16 tiles (4 rows per
+// wave) when they still give >= 1024 blocks, else 16x8
 int launch_pf(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
   // one output-channel block and one or two input chunks: the persistent
   // tile loop (at 16 x 256^2: 32->32 50.7 -> 48.2 us, 32->64 32.6 -> 28.7,
@@ -1294,6 +1278,7 @@ int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, c
   a.mask = nullptr;
   return launch_pf(a, 9, (hipStream_t)stream, "conv_fwd_upcat");
 }
+
 
 int shy_nn_conv_dgrad(const void* g, const void* wf, const void* gres, const void* mask,
                       void* dx_bf16, int N, int H, int W, int Cin, int Cout, int taps,
