@@ -93,6 +93,12 @@ int shy_nn_mse(const float* y, const float* t, const float* mask, void* g_bf16, 
 /* Adam on FP32 master weights, optional bf16 copy for the MFMA kernels */
 int shy_nn_adam(float* w, const float* g, float* m, float* v, void* w_bf16, size_t n, float lr,
                 float b1, float b2, float eps, int step, float weight_decay, void* stream);
+/* shy_nn_adam on a convolution's weights w[Cout][taps*Cin] that also writes
+   the rounded weights flipped for the data gradient (the layout of
+   shy_nn_flip_weights) into wf_bf16, so no separate flip runs per step */
+int shy_nn_adam_conv(float* w, const float* g, float* m, float* v, void* w_bf16, void* wf_bf16,
+                     int Cin, int Cout, int taps, float lr, float b1, float b2, float eps, int step,
+                     float weight_decay, void* stream);
 int shy_nn_cast(const void* src, void* dst, size_t n, int to_bf16, void* stream);
 
 #ifdef __cplusplus

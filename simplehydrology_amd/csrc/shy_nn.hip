@@ -841,6 +841,46 @@ __global__ void __launch_bounds__(256) bias_grad_kernel(const __bf16* __restrict
   }
 }
 
+// The same sums 8 channels per thread (C % 8 == 0, C <= 2048, g 16-byte
+// aligned): thread t covers channels 8 (t % C8) .. + 7 of every (256/C8)-th
+// row from row t / C8, each load 16 bytes, four rows in flight; then the
+// row-group sums are added in group order.
+__global__ void __launch_bounds__(256) bias_grad_v8_kernel(const __bf16* __restrict__ g,
+                                                           float* part, int P, int C,
+                                                           int rows_per_block) {
+  __shared__ float acc[256 * 8];
+  const int t = threadIdx.x, C8 = C / 8, groups = 256 / C8;
+  const int c8 = t % C8, r = t / C8;
+  const int p0 = blockIdx.x * rows_per_block;
+  const int p1 = min(P, p0 + rows_per_block);
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < groups) {
+    int p = p0 + r;
+    for (; p + 3 * groups < p1; p += 4 * groups) {
+      bf16x8 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = load8(g + (size_t)(p + u * groups) * C + 8 * c8);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += (float)v[u][k];
+    }
+    for (; p < p1; p += groups) {
+      const bf16x8 v = load8(g + (size_t)p * C + 8 * c8);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] += (float)v[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[t * 8 + k] = s[k];
+  __syncthreads();
+  for (int c = t; c < C; c += 256) {
+    float u = 0.0f;
+    for (int q = 0; q < groups; ++q) u += acc[(q * C8 + c / 8) * 8 + c % 8];
+    part[(size_t)blockIdx.x * C + c] = u;
+  }
+}
+
 __global__ void relu_bwd_kernel(const __bf16* __restrict__ y, const __bf16* __restrict__ g,
                                 __bf16* out, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1047,9 +1087,13 @@ __global__ void mse_kernel(const float* __restrict__ y, const float* __restrict_
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; optional bf16 copy
+// With wf, the rounded weight also goes to its place in the data gradient's
+// flipped weights (flip_weights_kernel's layout), so the backward pass needs
+// no flip of its own: w[co][t*Cin + ci] -> wf[ci][(taps-1-t)*Cout + co].
 __global__ void adam_kernel(float* w, const float* __restrict__ g, float* m, float* v,
                             __bf16* w_bf, size_t n, float lr, float b1, float b2, float eps,
-                            float c1, float c2, float wd) {
+                            float c1, float c2, float wd, __bf16* wf = nullptr, int Cin = 0,
+                            int Cout = 0, int taps = 0) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i] + wd * w[i];
@@ -1060,6 +1104,12 @@ __global__ void adam_kernel(float* w, const float* __restrict__ g, float* m, flo
   const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   w[i] = wi;
   if (w_bf) w_bf[i] = (__bf16)wi;
+  if (wf) {
+    const int k = taps * Cin;
+    const int co = (int)(i / k), rem = (int)(i % k);
+    const int t = rem / Cin, ci = rem % Cin;
+    wf[(size_t)ci * taps * Cout + (size_t)(taps - 1 - t) * Cout + co] = (__bf16)wi;
+  }
 }
 
 __global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* y, size_t n) {
@@ -1384,10 +1434,14 @@ int shy_nn_conv_wgrad_noalloc(const void* x, const void* g, float* dw, int N, in
 int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P, int C,
                      void* stream) {
   if (!g || !db || !part || C > 1024 || part_rows < 1) return nn_fail("bias_grad: bad argument");
-  const int rows = (P + part_rows - 1) / part_rows;
+  // at least 4096 pixels per block: fewer partials for the reduce to add
+  const int rows = std::max((P + part_rows - 1) / part_rows, 4096);
   const int nb = (P + rows - 1) / rows;
   hipStream_t s = (hipStream_t)stream;
-  bias_grad_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
+  if (C % 8 == 0 && ((uintptr_t)g & 15) == 0)
+    bias_grad_v8_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
+  else
+    bias_grad_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
   if (int r = nn_check("bias_grad")) return r;
   wgrad_reduce_kernel<<<(C + 31) / 32, 256, 0, s>>>(part, db, C, nb);
   return nn_check("bias_grad_reduce");
@@ -1480,6 +1534,22 @@ int shy_nn_adam(float* w, const float* g, float* m, float* v, void* w_bf16, size
   adam_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>(w, g, m, v, (__bf16*)w_bf16, n, lr, b1,
                                                            b2, eps, c1, c2, weight_decay);
   return nn_check("adam");
+}
+
+int shy_nn_adam_conv(float* w, const float* g, float* m, float* v, void* w_bf16, void* wf_bf16,
+                     int Cin, int Cout, int taps, float lr, float b1, float b2, float eps, int step,
+                     float weight_decay, void* stream) {
+  if (step < 1) return nn_fail("adam_conv: step must be >= 1");
+  if (!w || !g || !m || !v || !w_bf16 || !wf_bf16 || Cin < 1 || Cout < 1 ||
+      (taps != 9 && taps != 1))
+    return nn_fail("adam_conv: bad argument");
+  const size_t n = (size_t)Cout * taps * Cin;
+  if (n > (size_t)INT32_MAX) return nn_fail("adam_conv: too many weights");
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  adam_kernel<<<blocks(n), 256, 0, (hipStream_t)stream>>>(w, g, m, v, (__bf16*)w_bf16, n, lr, b1,
+                                                           b2, eps, c1, c2, weight_decay,
+                                                           (__bf16*)wf_bf16, Cin, Cout, taps);
+  return nn_check("adam_conv");
 }
 
 int shy_nn_cast(const void* src, void* dst, size_t n, int to_bf16, void* stream) {

@@ -55,6 +55,8 @@ def nn():
             "shy_nn_add": (i32, [vp, vp, vp, sz, vp]),
             "shy_nn_mse": (i32, [vp, vp, vp, vp, vp, sz, f32, vp]),
             "shy_nn_adam": (i32, [vp, vp, vp, vp, vp, sz, f32, f32, f32, f32, i32, f32, vp]),
+            "shy_nn_adam_conv": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, f32, f32, f32, f32,
+                                       i32, f32, vp]),
             "shy_nn_cast": (i32, [vp, vp, sz, i32, vp]),
         }
         for name, (res, args) in sig.items():
@@ -147,6 +149,7 @@ class Conv:
         self.b = torch.zeros(cout, dtype=torch.float32, device=device)
         self.w_bf = self.w.to(torch.bfloat16)
         self.wf_bf = torch.empty(cin * taps * cout, dtype=torch.bfloat16, device=device)
+        self._wf_of = None  # (w_bf's address, its torch version) that wf_bf was flipped from
         self.dw = torch.zeros_like(self.w)
         self.db = torch.zeros_like(self.b)
         self.m = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
@@ -190,8 +193,12 @@ class Conv:
                                   P, self.cout, _stream()))
         if not need_dx:
             return None
-        _ck(nn().shy_nn_flip_weights(_p(self.w_bf), _p(self.wf_bf), self.cin, self.cout,
-                                     self.taps, _stream()))
+        if self._wf_of != (self.w_bf.data_ptr(), self.w_bf._version):
+            # w_bf was replaced or written through torch since step() last
+            # wrote wf_bf (or step() has not run): flip it here
+            _ck(nn().shy_nn_flip_weights(_p(self.w_bf), _p(self.wf_bf), self.cin, self.cout,
+                                         self.taps, _stream()))
+            self._wf_of = (self.w_bf.data_ptr(), self.w_bf._version)
         dx = torch.empty(N, H, W, self.cin, dtype=torch.bfloat16, device=g.device)
         if gres is not None or mask is not None:
             _ck(nn().shy_nn_conv_dgrad(_p(g), _p(self.wf_bf), _p(gres), _p(mask), _p(dx), N, H, W,
@@ -203,8 +210,12 @@ class Conv:
 
     def step(self, lr, t, wd=0.0):
         s = _stream()
-        _ck(nn().shy_nn_adam(_p(self.w), _p(self.dw), _p(self.m[0]), _p(self.v[0]),
-                             _p(self.w_bf), self.w.numel(), lr, 0.9, 0.999, 1e-8, t, wd, s))
+        # the kernel writes w_bf and its flipped copy wf_bf (torch's version
+        # counter of w_bf does not move: it counts writes made through torch)
+        _ck(nn().shy_nn_adam_conv(_p(self.w), _p(self.dw), _p(self.m[0]), _p(self.v[0]),
+                                  _p(self.w_bf), _p(self.wf_bf), self.cin, self.cout, self.taps,
+                                  lr, 0.9, 0.999, 1e-8, t, wd, s))
+        self._wf_of = (self.w_bf.data_ptr(), self.w_bf._version)
         _ck(nn().shy_nn_adam(_p(self.b), _p(self.db), _p(self.m[1]), _p(self.v[1]), None,
                              self.b.numel(), lr, 0.9, 0.999, 1e-8, t, 0.0, s))
 

@@ -265,7 +265,7 @@ def train(net, feats, targets_enc, steps=2000, batch=16, tile=256, lr=2e-3, seed
     return hist, time.perf_counter() - t0
 
 
-def train_timing(net, batch=16, tile=256, reps=5):
+def train_timing(net, batch=16, tile=256, reps=20):
     """Forward and forward+backward+Adam time of one batch (HIP events) and
     the algorithmic FLOP rate (forward FLOPs x 3 for a training step)."""
     import torch

@@ -114,6 +114,37 @@ def test_adam_closed_form(sg):
     assert torch.allclose(w.cpu(), exp, atol=1e-6)
 
 
+@pytest.mark.parametrize("cin,cout,taps", [(32, 64, 9), (8, 32, 9), (96, 8, 1)])
+def test_adam_conv_writes_the_flipped_weights(sg, cin, cout, taps):
+    """shy_nn_adam_conv: the same update as shy_nn_adam, and wf_bf equal to
+    shy_nn_flip_weights of the updated bf16 weights."""
+    dev = "cuda:0"
+    g = torch.Generator().manual_seed(cin + 3 * cout + taps)
+    n = cout * taps * cin
+    w0 = torch.randn(n, generator=g)
+    gr = torch.randn(n, generator=g) * 0.01
+    m0 = torch.randn(n, generator=g).abs() * 0.01
+    v0 = torch.randn(n, generator=g).abs() * 1e-4
+    bufs = [[t.clone().to(dev) for t in (w0, m0, v0)] for _ in range(2)]
+    wb = [torch.empty(n, dtype=torch.bfloat16, device=dev) for _ in range(2)]
+    wf = torch.empty(n, dtype=torch.bfloat16, device=dev)
+    grd = gr.to(dev)
+    sg._ck(sg.nn().shy_nn_adam(sg._p(bufs[0][0]), sg._p(grd), sg._p(bufs[0][1]), sg._p(bufs[0][2]),
+                               sg._p(wb[0]), n, 1e-3, 0.9, 0.999, 1e-8, 3, 0.01, None))
+    sg._ck(sg.nn().shy_nn_adam_conv(sg._p(bufs[1][0]), sg._p(grd), sg._p(bufs[1][1]),
+                                    sg._p(bufs[1][2]), sg._p(wb[1]), sg._p(wf), cin, cout, taps,
+                                    1e-3, 0.9, 0.999, 1e-8, 3, 0.01, None))
+    for a, b in zip(bufs[0] + [wb[0]], bufs[1] + [wb[1]]):
+        assert torch.equal(a.view(torch.int16) if a.dtype == torch.bfloat16 else a.view(torch.int32),
+                           b.view(torch.int16) if b.dtype == torch.bfloat16 else b.view(torch.int32))
+    wf_ref = torch.empty_like(wf)
+    sg._ck(sg.nn().shy_nn_flip_weights(sg._p(wb[0]), sg._p(wf_ref), cin, cout, taps, None))
+    assert torch.equal(wf.view(torch.int16), wf_ref.view(torch.int16))
+    # and by hand: wf[ci][(taps-1-t)*cout + co] = w[co][t*cin + ci]
+    wr = wb[0].cpu().float().view(cout, taps, cin)
+    assert torch.equal(wf.cpu().float().view(cin, taps, cout), wr.flip(1).permute(2, 1, 0))
+
+
 def torch_unet(net, x):
     """The same U-Net in float32 torch on the CPU, sharing net's weights."""
     params = []
