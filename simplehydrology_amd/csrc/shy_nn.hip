@@ -356,8 +356,15 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
 // load latency behind MFMAs as a deep layer does, where one tile per block
 // leaves every block waiting for its patch.  A single-chunk layer loads its
 // weights once per block.
+// Blocks per compute unit of conv_tiles_kernel: three where a block computes
+// 32 output channels (MT = 2: at most 168 VGPRs, 45 KB of LDS), two where it
+// computes 64 (asked for three, the compiler moves the accumulators to AGPRs
+// and leaves one wave per SIMD).
+template <int MT>
+constexpr int tiles_per_cu() { return MT == 2 ? 3 : 2; }
+
 template <int TAPS, int MT, int NT, bool UPCAT = false>
-__global__ void __launch_bounds__(256, 2) conv_tiles_kernel(ConvArgs a) {
+__global__ void __launch_bounds__(256, tiles_per_cu<MT>()) conv_tiles_kernel(ConvArgs a) {
   constexpr int kTH = 4 * NT;
   constexpr int HALO = TAPS == 9 ? 1 : 0;
   constexpr int PW = kTW + 2 * HALO, PH = kTH + 2 * HALO;
@@ -1153,14 +1160,14 @@ int device_cus() {
   return c;
 }
 
-// Persistent tile loop (conv_tiles_kernel): two blocks per compute unit over
+// Persistent tile loop (conv_tiles_kernel): tiles_per_cu blocks per compute unit over
 // the layer's tiles of 16 x 4NT pixels, 16 MT output channels per block.
 template <int TAPS, int MT, int NT, bool UPCAT>
 int launch_tiles_mt(const ConvArgs& a, hipStream_t s) {
   const long long tiles = (long long)a.N * ((a.H + 4 * NT - 1) / (4 * NT)) * ((a.W + kTW - 1) / kTW);
   if (tiles > INT32_MAX) return nn_fail("conv: too many tiles");
   const int gy = a.Cout / (16 * MT);
-  const long long cap = std::max(1LL, 2LL * device_cus() / gy);
+  const long long cap = std::max(1LL, (long long)tiles_per_cu<MT>() * device_cus() / gy);
   conv_tiles_kernel<TAPS, MT, NT, UPCAT>
       <<<dim3((unsigned)std::min(tiles, cap), (unsigned)gy), 256, 0, s>>>(a);
   return 0;
