@@ -355,16 +355,13 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
 // tile is loaded.  A layer of one or two 32-channel chunks then hides its
 // load latency behind MFMAs as a deep layer does, where one tile per block
 // leaves every block waiting for its patch.  A single-chunk layer loads its
-// weights once per block.
-// Blocks per compute unit of conv_tiles_kernel: three where a block computes
-// 32 output channels (MT = 2: at most 168 VGPRs, 45 KB of LDS), two where it
-// computes 64 (asked for three, the compiler moves the accumulators to AGPRs
-// and leaves one wave per SIMD).
-template <int MT>
-constexpr int tiles_per_cu() { return MT == 2 ? 3 : 2; }
+// weights once per block.  Three blocks per compute unit with 32 output
+// channels per block (MT = 2): at most 168 VGPRs and 45 KB of LDS each.
+constexpr int kTileBlocksPerCU = 3;
 
 template <int TAPS, int MT, int NT, bool UPCAT = false>
-__global__ void __launch_bounds__(256, tiles_per_cu<MT>()) conv_tiles_kernel(ConvArgs a) {
+__global__ void __launch_bounds__(256, kTileBlocksPerCU) conv_tiles_kernel(ConvArgs a) {
+  static_assert(MT == 2, "three blocks per CU fit 32 output channels per block");
   constexpr int kTH = 4 * NT;
   constexpr int HALO = TAPS == 9 ? 1 : 0;
   constexpr int PW = kTW + 2 * HALO, PH = kTH + 2 * HALO;
@@ -1160,41 +1157,30 @@ int device_cus() {
   return c;
 }
 
-// Persistent tile loop (conv_tiles_kernel): tiles_per_cu blocks per compute unit over
-// the layer's tiles of 16 x 4NT pixels, 16 MT output channels per block.
-template <int TAPS, int MT, int NT, bool UPCAT>
-int launch_tiles_mt(const ConvArgs& a, hipStream_t s) {
+// Persistent tile loop (conv_tiles_kernel): three blocks per compute unit
+// over the layer's tiles of 16 x 16 pixels, 32 output channels per block.  A
+// 64-channel layer takes two such blocks per tile: at 16 x 128^2, 32->64
+// 31.1 -> 27.8 us and 64->64 41.7 -> 40.7 us against one 64-channel block
+// at two blocks per CU (the 64-channel block needs 255 VGPRs).
+int launch_tiles(const ConvArgs& a, hipStream_t s, const char* what) {
+  constexpr int MT = 2, NT = 4;
   const long long tiles = (long long)a.N * ((a.H + 4 * NT - 1) / (4 * NT)) * ((a.W + kTW - 1) / kTW);
-  if (tiles > INT32_MAX) return nn_fail("conv: too many tiles");
+  if (tiles > INT32_MAX) return nn_fail(std::string(what) + ": too many tiles");
   const int gy = a.Cout / (16 * MT);
-  const long long cap = std::max(1LL, (long long)tiles_per_cu<MT>() * device_cus() / gy);
-  conv_tiles_kernel<TAPS, MT, NT, UPCAT>
+  const long long cap = std::max(1LL, (long long)kTileBlocksPerCU * device_cus() / gy);
+  conv_tiles_kernel<9, MT, NT, false>
       <<<dim3((unsigned)std::min(tiles, cap), (unsigned)gy), 256, 0, s>>>(a);
-  return 0;
-}
-
-int launch_tiles(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
-  const bool m4 = a.Cout % 64 == 0;
-  int rc;
-  if (a.x2)
-    rc = m4 ? launch_tiles_mt<9, 4, 4, true>(a, s) : launch_tiles_mt<9, 2, 4, true>(a, s);
-  else if (taps == 9)
-    rc = m4 ? launch_tiles_mt<9, 4, 4, false>(a, s) : launch_tiles_mt<9, 2, 4, false>(a, s);
-  else
-    rc = m4 ? launch_tiles_mt<1, 4, 4, false>(a, s) : launch_tiles_mt<1, 2, 4, false>(a, s);
-  if (rc != 0) return rc;
   return nn_check(what);
 }
 
 // conv_pf_kernel for Cin and Cout multiples of 32: 16x16 tiles (4 rows per
 // wave) when they still give >= 1024 blocks, else 16x8
 int launch_pf(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
-  // one output-channel block and one or two input chunks: the persistent
-  // tile loop (at 16 x 256^2: 32->32 50.7 -> 48.2 us, 32->64 32.6 -> 28.7,
-  // 64->64 42.7 -> 39.3; with more channel blocks or chunks the one-tile
-  // blocks below are faster)
+  // 32 or 64 output channels and one or two input chunks: the persistent
+  // tile loop (32->32 at 16 x 256^2: 50.7 us in the one-tile blocks below,
+  // 43.4 us; with more channels or chunks the one-tile blocks are faster)
   if (!a.x2 && taps == 9 && a.Cin <= 64 && (a.Cout == 32 || a.Cout == 64))
-    return launch_tiles(a, taps, s, what);
+    return launch_tiles(a, s, what);
   const int mt = a.Cout % 64 == 0 ? 4 : 2;
   const unsigned gy = (unsigned)(a.Cout / (16 * mt));
   const long long tiles16 = (long long)a.N * ((a.H + 15) / 16) * ((a.W + kTW - 1) / kTW);
