@@ -1173,52 +1173,38 @@ int launch_tiles(const ConvArgs& a, hipStream_t s, const char* what) {
   return nn_check(what);
 }
 
-// conv_pf_kernel for Cin and Cout multiples of 32: 16x16 tiles (4 rows per
-// wave) when they still give >= 1024 blocks, else 16x8
+// conv_pf_kernel for Cin and Cout multiples of 32, 32 output channels per
+// block (MT = 2; 64-channel blocks at 32^2 and 128^2 ran 4-16% slower: half
+// the blocks, and 210-256 VGPRs), 16x16 tiles (4 rows per wave) when they
+// still give >= 1024 blocks, else 16x8.
 int launch_pf(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
   // 32 or 64 output channels and one or two input chunks: the persistent
   // tile loop (32->32 at 16 x 256^2: 50.7 us in the one-tile blocks below,
   // 43.4 us; with more channels or chunks the one-tile blocks are faster)
   if (!a.x2 && taps == 9 && a.Cin <= 64 && (a.Cout == 32 || a.Cout == 64))
     return launch_tiles(a, s, what);
-  const int mt = a.Cout % 64 == 0 ? 4 : 2;
-  const unsigned gy = (unsigned)(a.Cout / (16 * mt));
+  const unsigned gy = (unsigned)(a.Cout / 32);
   const long long tiles16 = (long long)a.N * ((a.H + 15) / 16) * ((a.W + kTW - 1) / kTW);
   const long long tiles8 = (long long)a.N * ((a.H + 7) / 8) * ((a.W + kTW - 1) / kTW);
   if (tiles8 > INT32_MAX) return nn_fail(std::string(what) + ": too many tiles");
   const bool big = tiles16 * gy >= 1024;
   const dim3 g((unsigned)(big ? tiles16 : tiles8), gy);
   const long long tiles32 = (long long)a.N * ((a.H + 31) / 32) * ((a.W + kTW - 1) / kTW);
-  if (!a.x2 && taps == 9 && mt == 2 && a.Cin == 32 && tiles32 * gy >= 1024) {
+  if (!a.x2 && taps == 9 && a.Cin == 32 && tiles32 * gy >= 1024) {
     // one input chunk: 16x32 tiles staged without prefetch, so each block
     // reloads the layer's weights for twice the pixels
     conv_pf_kernel<9, 2, 8, false, false><<<dim3((unsigned)tiles32, gy), 256, 0, s>>>(a);
     return nn_check(what);
   }
   if (a.x2) {  // the input is [up2(x), x2] (taps 9)
-    if (mt == 4) {
-      if (big) conv_pf_kernel<9, 4, 4, true><<<g, 256, 0, s>>>(a);
-      else conv_pf_kernel<9, 4, 2, true><<<g, 256, 0, s>>>(a);
-    } else {
-      if (big) conv_pf_kernel<9, 2, 4, true><<<g, 256, 0, s>>>(a);
-      else conv_pf_kernel<9, 2, 2, true><<<g, 256, 0, s>>>(a);
-    }
-  } else if (mt == 4) {
-    if (big) {
-      if (taps == 9) conv_pf_kernel<9, 4, 4><<<g, 256, 0, s>>>(a);
-      else conv_pf_kernel<1, 4, 4><<<g, 256, 0, s>>>(a);
-    } else {
-      if (taps == 9) conv_pf_kernel<9, 4, 2><<<g, 256, 0, s>>>(a);
-      else conv_pf_kernel<1, 4, 2><<<g, 256, 0, s>>>(a);
-    }
+    if (big) conv_pf_kernel<9, 2, 4, true><<<g, 256, 0, s>>>(a);
+    else conv_pf_kernel<9, 2, 2, true><<<g, 256, 0, s>>>(a);
+  } else if (big) {
+    if (taps == 9) conv_pf_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
+    else conv_pf_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
   } else {
-    if (big) {
-      if (taps == 9) conv_pf_kernel<9, 2, 4><<<g, 256, 0, s>>>(a);
-      else conv_pf_kernel<1, 2, 4><<<g, 256, 0, s>>>(a);
-    } else {
-      if (taps == 9) conv_pf_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
-      else conv_pf_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
-    }
+    if (taps == 9) conv_pf_kernel<9, 2, 2><<<g, 256, 0, s>>>(a);
+    else conv_pf_kernel<1, 2, 2><<<g, 256, 0, s>>>(a);
   }
   return nn_check(what);
 }
