@@ -74,12 +74,25 @@ struct ConvArgs {
   // shy_nn_add then shy_nn_relu_bwd on the rounded output.
   const __bf16* __restrict__ gres;
   const __bf16* __restrict__ mask;
+  // When head_w is set (conv_tiles_kernel, 32 output channels, BF16 output):
+  // the epilogue also applies a 1x1 convolution to the rounded output,
+  // head_y [P][head_c] (FP32) = head_b + head_w [head_c][32] . y, with one
+  // MFMA per row of pixels; y_bf may then be null (y not stored).
+  const __bf16* __restrict__ head_w = nullptr;
+  const float* __restrict__ head_b = nullptr;
+  float* head_y = nullptr;
+  int head_c = 0;
 };
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // One lane's 4 consecutive output channels co..co+3 of output element o:
 // bias, residual (before rounding), ReLU, then the store, FP32 or BF16 (with
-// the backward-pass residual and mask after rounding).
-__device__ inline void store_out4(const ConvArgs& a, size_t o, int co, const f32x4& acc) {
+// the backward-pass residual and mask after rounding).  Returns the stored
+// BF16 values (zeros for an FP32 output); with HEAD a null y_bf stores
+// nothing.
+template <bool HEAD>
+__device__ inline bf16x4 store_out4(const ConvArgs& a, size_t o, int co, const f32x4& acc) {
   float v[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) v[i] = acc[i] + (a.bias ? a.bias[co + i] : 0.0f);
@@ -91,11 +104,12 @@ __device__ inline void store_out4(const ConvArgs& a, size_t o, int co, const f32
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
   }
+  bf16x4 q;
   if (a.y_f) {
     *reinterpret_cast<float4*>(a.y_f + o) = make_float4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = (__bf16)0.0f;
   } else {
-    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-    bf16x4 q;
 #pragma unroll
     for (int i = 0; i < 4; ++i) q[i] = (__bf16)v[i];
     if (a.gres) {
@@ -106,28 +120,72 @@ __device__ inline void store_out4(const ConvArgs& a, size_t o, int co, const f32
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = (float)a.mask[o + i] > 0.0f ? q[i] : (__bf16)0.0f;
     }
-    *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
+    if (!HEAD || a.y_bf) *reinterpret_cast<bf16x4*>(a.y_bf + o) = q;
+  }
+  return q;
+}
+
+// The 1x1 head on a tile's rounded 32-channel output q (ConvArgs::head_w),
+// one MFMA per row of 16 pixels: the output channels are the M operand and
+// the pixels the N operand, and the K operand takes each lane's own 8
+// channels in the order it holds them, 4g..4g+3 then 16+4g..16+4g+3 for
+// lane group g = lane/16, so the rounded output needs no exchange; the
+// head's weights are read in that order.  Lane holds head channels
+// 4(lane/16) + i of pixel lane%16.
+template <int NT>
+__device__ inline void conv_head(const ConvArgs& a, const bf16x4 (&q)[2][NT], int n, int y0,
+                                 int x0, int wave, int lane) {
+  const int hc = a.head_c, o = lane & 15, g = lane >> 4;
+  bf16x8 A;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    A[j] = o < hc ? a.head_w[o * 32 + 4 * g + j] : (__bf16)0.0f;
+    A[4 + j] = o < hc ? a.head_w[o * 32 + 16 + 4 * g + j] : (__bf16)0.0f;
+  }
+  const int gx = x0 + (lane & 15), oc = 4 * g;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    bf16x8 B;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      B[j] = q[0][t][j];
+      B[4 + j] = q[1][t][j];
+    }
+    const f32x4 h = mfma(A, B, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    const int gy = y0 + wave * NT + t;
+    if (gy < a.H && gx < a.W && oc < hc) {
+      const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = h[i] + (a.head_b ? a.head_b[oc + i] : 0.0f);
+      *reinterpret_cast<float4*>(a.head_y + p * hc + oc) = make_float4(v[0], v[1], v[2], v[3]);
+    }
   }
 }
 
 // The epilogue of a block's tile: lane holds channels cb + 16m + 4(lane/16)
-// + i of pixel (y0 + NT wave + t, x0 + lane%16).
-template <int MT, int NT>
+// + i of pixel (y0 + NT wave + t, x0 + lane%16).  HEAD: then the 1x1 head
+// (its own instantiation: the rounded outputs it keeps cost registers).
+template <int MT, int NT, bool HEAD = false>
 __device__ inline void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][NT], int n, int y0,
                                      int x0, int cb, int wave, int lane) {
+  static_assert(!HEAD || MT == 2, "the head reads a block's 32 output channels");
   const int gx = x0 + (lane & 15);
+  bf16x4 q[MT][NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int gy = y0 + wave * NT + t;
-    if (gy >= a.H || gx >= a.W) continue;
     const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
       const int co = cb + m * 16 + (lane >> 4) * 4;
-      if (co >= a.Cout) continue;
-      store_out4(a, p * a.Cout + co, co, acc[m][t]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[m][t][i] = (__bf16)0.0f;
+      if (gy < a.H && gx < a.W && co < a.Cout)
+        q[m][t] = store_out4<HEAD>(a, p * a.Cout + co, co, acc[m][t]);
     }
   }
+  if constexpr (HEAD) conv_head<NT>(a, q, n, y0, x0, wave, lane);
 }
 
 // Forward (and dgrad) convolution.  Block = 4 waves; each wave owns 32 pixels
@@ -359,7 +417,7 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
 // channels per block (MT = 2): at most 168 VGPRs and 45 KB of LDS each.
 constexpr int kTileBlocksPerCU = 3;
 
-template <int TAPS, int MT, int NT, bool UPCAT = false>
+template <int TAPS, int MT, int NT, bool UPCAT = false, bool HEAD = false>
 __global__ void __launch_bounds__(256, kTileBlocksPerCU) conv_tiles_kernel(ConvArgs a) {
   static_assert(MT == 2, "three blocks per CU fit 32 output channels per block");
   constexpr int kTH = 4 * NT;
@@ -484,7 +542,8 @@ __global__ void __launch_bounds__(256, kTileBlocksPerCU) conv_tiles_kernel(ConvA
       }
     }
     const int tx = tile % tiles_x, r = tile / tiles_x;
-    conv_epilogue<MT, NT>(a, acc, r / tiles_y, (r % tiles_y) * kTH, tx * kTW, cb, wave, lane);
+    conv_epilogue<MT, NT, HEAD>(a, acc, r / tiles_y, (r % tiles_y) * kTH, tx * kTW, cb, wave,
+                                lane);
     tile += (int)gridDim.x;
     if (tile >= total) break;
   }
@@ -1168,8 +1227,9 @@ int launch_tiles(const ConvArgs& a, hipStream_t s, const char* what) {
   if (tiles > INT32_MAX) return nn_fail(std::string(what) + ": too many tiles");
   const int gy = a.Cout / (16 * MT);
   const long long cap = std::max(1LL, (long long)kTileBlocksPerCU * device_cus() / gy);
-  conv_tiles_kernel<9, MT, NT, false>
-      <<<dim3((unsigned)std::min(tiles, cap), (unsigned)gy), 256, 0, s>>>(a);
+  const dim3 g((unsigned)std::min(tiles, cap), (unsigned)gy);
+  if (a.head_w) conv_tiles_kernel<9, MT, NT, false, true><<<g, 256, 0, s>>>(a);
+  else conv_tiles_kernel<9, MT, NT, false, false><<<g, 256, 0, s>>>(a);
   return nn_check(what);
 }
 
@@ -1281,6 +1341,34 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
     else conv_fwd_kernel<1, 1><<<g, 256, 0, s>>>(a);
   }
   return nn_check("conv_fwd");
+}
+
+int shy_nn_conv_fwd_head(const void* x, const void* w, const float* bias, const void* res,
+                         void* y_bf16, const void* head_w, const float* head_b, float* head_y,
+                         int N, int H, int W, int Cin, int head_c, int relu, void* stream) {
+  if (!x || !w || !head_w || !head_y) return nn_fail("conv_fwd_head: bad pointers");
+  if (N < 1 || H < 1 || W < 1 || (Cin != 32 && Cin != 64) || head_c < 4 || head_c > 16 ||
+      head_c % 4)
+    return nn_fail("conv_fwd_head: need Cin 32 or 64 and head_c in {4, 8, 12, 16}");
+  if ((long long)N * H * W > INT32_MAX / 2) return nn_fail("conv_fwd_head: too many pixels");
+  ConvArgs a;
+  a.x = (const __bf16*)x;
+  a.w = (const __bf16*)w;
+  a.bias = bias;
+  a.res = (const __bf16*)res;
+  a.y_bf = (__bf16*)y_bf16;
+  a.y_f = nullptr;
+  a.N = N; a.H = H; a.W = W; a.Cin = Cin; a.Cout = 32; a.K = 9 * Cin;
+  a.relu = relu;
+  a.x2 = nullptr;
+  a.C1 = 0;
+  a.gres = nullptr;
+  a.mask = nullptr;
+  a.head_w = (const __bf16*)head_w;
+  a.head_b = head_b;
+  a.head_y = head_y;
+  a.head_c = head_c;
+  return launch_tiles(a, (hipStream_t)stream, "conv_fwd_head");
 }
 
 int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, const float* bias,

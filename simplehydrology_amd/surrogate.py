@@ -44,6 +44,8 @@ def nn():
             "shy_nn_conv_fwd_upcat": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32,
                                             i32, i32, vp]),
             "shy_nn_conv_dgrad": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
+            "shy_nn_conv_fwd_head": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32,
+                                           i32, i32, vp]),
             "shy_nn_flip_weights": (i32, [vp, vp, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad_noalloc": (i32, [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
@@ -166,6 +168,21 @@ class Conv:
                                  _stream()))
         return y
 
+    def forward_head(self, x, shape, head, res=None, relu=True, keep=True):
+        """This 3x3 convolution (32 output channels) and the 1x1 convolution
+        ``head`` (bias, no ReLU, FP32 output) applied to its rounded output,
+        in one pass.  Returns (y or None without ``keep``, head output)."""
+        import torch
+        N, H, W = shape
+        assert self.cout == 32 and self.taps == 9 and head.cin == 32 and head.taps == 1
+        y = (torch.empty(N, H, W, self.cout, device=x.device, dtype=torch.bfloat16)
+             if keep else None)
+        out = torch.empty(N, H, W, head.cout, device=x.device, dtype=torch.float32)
+        _ck(nn().shy_nn_conv_fwd_head(_p(x), _p(self.w_bf), _p(self.b), _p(res), _p(y),
+                                      _p(head.w_bf), _p(head.b), _p(out), N, H, W, self.cin,
+                                      head.cout, 1 if relu else 0, _stream()))
+        return y, out
+
     def forward_upcat(self, xh, skip, shape, res=None, relu=True):
         """The same convolution of [up2(xh), skip] along channels, read from
         the two tensors (no concatenated copy)."""
@@ -261,17 +278,22 @@ class ResStage:
         self.b = Conv(c, c, 9, gen, device)
         self.c = Conv(c, c, 9, gen, device)
 
-    def forward(self, x, shape, skip=None):
+    def forward(self, x, shape, skip=None, head=None):
         """With ``skip`` the stage's input is [up2(x), skip] along channels,
-        read without building it (inference: nothing is kept for backward)."""
+        read without building it (inference: nothing is kept for backward).
+        With ``head`` (a 1x1 Conv) returns (output or None with ``skip``,
+        head applied to it) from the last convolution's one pass."""
         if skip is None:
             ya = self.a.forward(x, shape)
         else:
             ya = self.a.forward_upcat(x, skip, shape)
         yb = self.b.forward(ya, shape)
-        yc = self.c.forward(yb, shape, res=ya)
+        if head is None:
+            yc = self.c.forward(yb, shape, res=ya)
+        else:
+            yc, out = self.c.forward_head(yb, shape, head, res=ya, keep=skip is None)
         self.saved = (x, ya, yb, yc) if skip is None else None
-        return yc
+        return yc if head is None else (yc, out)
 
     def backward(self, g, shape, scratch, need_dx=True):
         x, ya, yb, yc = self.saved
@@ -348,17 +370,19 @@ class UNet:
         for k in (2, 1, 0):
             n, hh, ww = self.shapes[k]
             cu, cs = h.shape[-1], skips[k].shape[-1]
+            # the last stage's last convolution applies the head in its own pass
+            head = self.head if k == 0 else None
             if not keep:
-                h = self.d[k].forward(h, self.shapes[k], skip=skips[k])
+                h = self.d[k].forward(h, self.shapes[k], skip=skips[k], head=head)
                 continue
             u = torch.empty(n, hh, ww, cu + cs, dtype=torch.bfloat16, device=x.device)
             _ck(nn().shy_nn_up2(_p(h), _p(u), n, hh // 2, ww // 2, cu, cu + cs, 0, 0, _stream()))
             _ck(nn().shy_nn_channels(_p(skips[k]), _p(u), n * hh * ww, cs, cu + cs, cu, 0,
                                      _stream()))
             self.cat.append((cu, cs))
-            h = self.d[k].forward(u, self.shapes[k])
-        self.h0 = h
-        return self.head.forward(h, self.shapes[0], relu=False, out_f32=True)
+            h = self.d[k].forward(u, self.shapes[k], head=head)
+        self.h0, out = h
+        return out
 
     def backward(self, gy):
         """gy: [N, H, W, 8] bf16 gradient of the output."""

@@ -253,6 +253,35 @@ def test_conv_dgrad_epilogue_equals_separate_passes(sg, N, H, W, cin, cout):
     assert (both.float() == 0).float().mean() > 0.3  # the mask did zero about half
 
 
+@pytest.mark.parametrize("N,H,W,cin,hc,with_res", [(2, 20, 36, 32, 8, True), (1, 16, 48, 64, 4, False),
+                                                   (1, 13, 21, 32, 16, True)])
+def test_conv_head_equals_conv_then_head(sg, N, H, W, cin, hc, with_res):
+    """shy_nn_conv_fwd_head: the 32-channel output is shy_nn_conv_fwd's bit
+    for bit, and the head applied to it matches the separate 1x1
+    convolution (both are FP32 sums of the same BF16 products, in different
+    orders), partial tiles included; without y the head is the same."""
+    g = torch.Generator().manual_seed(17 * cin + hc)
+    dev = "cuda:0"
+    conv = sg.Conv(cin, 32, 9, np.random.default_rng(4), dev)
+    head = sg.Conv(32, hc, 1, np.random.default_rng(5), dev)
+    conv.b.copy_((torch.randn(32, generator=g) * 0.1).to(dev))
+    head.b.copy_((torch.randn(hc, generator=g) * 0.1).to(dev))
+    x = torch.randn(N, H, W, cin, generator=g).to(dev).to(torch.bfloat16).contiguous()
+    res = (torch.randn(N, H, W, 32, generator=g).to(dev).to(torch.bfloat16).contiguous()
+           if with_res else None)
+    print("conv+head", N, H, W, cin, hc, with_res, flush=True)
+    y0 = conv.forward(x, (N, H, W), res=res)
+    o0 = head.forward(y0, (N, H, W), relu=False, out_f32=True)
+    y1, o1 = conv.forward_head(x, (N, H, W), head, res=res)
+    assert torch.equal(y1.view(torch.int16), y0.view(torch.int16))
+    assert rel(o1.cpu(), o0.cpu()) < 1e-6
+    # and against float32 on the CPU from the rounded y
+    ref = y0.float().cpu() @ head.w_bf.float().cpu().view(hc, 32).t() + head.b.cpu()
+    assert rel(o1.cpu(), ref) < 1e-6
+    _, o2 = conv.forward_head(x, (N, H, W), head, res=res, keep=False)
+    assert torch.equal(o2.view(torch.int32), o1.view(torch.int32))
+
+
 def test_unet_inference_forward_equals_training_forward(sg):
     """The inference forward (decoder inputs read without concatenation)
     returns the training forward's bits."""
