@@ -281,7 +281,7 @@ def surrogate_leg(seed: int, size: int) -> dict:
                 "cold_ms": r["cold_ms_to_match"], "warm_ms": r["warm_ms"],
                 "speedup": r["speedup"], "gate": r["gate"], "cycle_ms": r["cycle_ms"]}},
         "surrogate_roofline": {
-            "bound": "mfma", "kernel": "conv_pf_kernel (U-Net forward, batch 16 x 256^2)",
+            "bound": "mfma", "kernel": "conv_tiles_kernel + conv_pf_kernel (U-Net forward, batch 16 x 256^2)",
             "achieved": fwd, "peak": 2500.0, "unit": "TFLOP/s", "frac": fwd / 2500.0,
             "train_step_tflops": tt["train_tflops"], "forward_gflop": tt["forward_gflop"]},
     }
