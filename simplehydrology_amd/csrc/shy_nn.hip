@@ -8,14 +8,28 @@
 // that instruction (checked on the device): lane l holds A[l%16][(l/16)*8+i],
 // B[(l/16)*8+i][l%16] for i < 8 and C[(l/16)*4+i][l%16] for i < 4.
 //
-//   conv_fwd_kernel    forward; epilogue adds bias, an optional residual and
-//                      an optional ReLU, stores bf16 or f32
-//   (dgrad)            the same kernel on the flipped, transposed weights
-//                      built by flip_weights_kernel
-//   conv_wgrad_kernel  dW[co][k] = sum_p g[p][co] x[p + tap][ci]: K = pixels,
-//                      staged through LDS (both operands need 8 consecutive
-//                      pixels per lane), split over pixel ranges into FP32
-//                      partials that wgrad_reduce_kernel sums in a fixed order
+//   conv_tiles_kernel  3x3 forward / data gradient of layers with 32 or 64
+//                      output channels and one or two 32-channel input
+//                      chunks: persistent blocks (three per CU, 32 output
+//                      channels each) over 16x16 pixel tiles, patch and
+//                      weights staged in LDS, the next tile's patch loaded
+//                      into registers during the current tile's MFMAs;
+//                      optionally a 1x1 head on the rounded output
+//   conv_pf_kernel     the same arithmetic with one tile per block for the
+//                      other layers (the decoder inputs read as [up2(x),
+//                      skip] from two tensors)
+//   conv_c8_kernel     the 8-channel first layer, four taps per MFMA k-step
+//   conv_fwd_kernel    direct-load convolution of the remaining shapes (the
+//                      head's data gradient)
+//   epilogue           bias, an optional residual and ReLU, bf16 or f32
+//                      store; for the data gradient (the forward kernels on
+//                      weights flipped by flip_weights_kernel or written so
+//                      by Adam) the residual gradient and the ReLU mask
+//   wgrad_lds_kernel   dW[co][k] = sum_p g[p][co] x[p + tap][ci]: K = pixels,
+//   conv_wgrad_kernel  staged through LDS (both operands need 8 consecutive
+//                      pixels per lane; the second loads directly), split
+//                      over pixel ranges into FP32 partials that
+//                      wgrad_reduce_kernel sums in a fixed order
 //   elementwise        ReLU backward, 2x2 average pool, nearest upsample,
 //                      channel concat/split, bias gradient, MSE loss, Adam
 #include <hip/hip_runtime.h>
