@@ -918,27 +918,41 @@ __global__ void __launch_bounds__(256) bias_grad_kernel(const __bf16* __restrict
   }
 }
 
-// The same sums 8 channels per thread (C % 8 == 0, C <= 2048, g 16-byte
-// aligned): thread t covers channels 8 (t % C8) .. + 7 of every (256/C8)-th
-// row from row t / C8, each load 16 bytes, four rows in flight; then the
-// row-group sums are added in group order.
-__global__ void __launch_bounds__(256) bias_grad_v8_kernel(const __bf16* __restrict__ g,
-                                                           float* part, int P, int C,
-                                                           int rows_per_block) {
-  __shared__ float acc[256 * 8];
-  const int t = threadIdx.x, C8 = C / 8, groups = 256 / C8;
+// The same sums 8 channels per thread (C % 8 == 0, C <= 1024, g 16-byte
+// aligned), 1024 threads per block: thread t covers channels 8 (t % C8) ..
+// + 7 of every G-th row from row t / C8 (G = 1024 / C8 row groups), each
+// load 16 bytes, eight rows in flight; then the G group sums are folded in
+// LDS by halves (group r + h onto group r), a fixed order.
+__device__ inline void fold_groups(float* acc, int t, int C8, int groups) {
+  const int c8 = t % C8, r = t / C8;
+  for (int n = groups; n > 1;) {
+    const int h = (n + 1) / 2;  // groups [h, n) onto [0, n - h)
+    if (r < n - h) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[(r * C8 + c8) * 8 + k] += acc[((r + h) * C8 + c8) * 8 + k];
+    }
+    __syncthreads();
+    n = h;
+  }
+}
+
+__global__ void __launch_bounds__(1024) bias_grad_v8_kernel(const __bf16* __restrict__ g,
+                                                            float* part, int P, int C,
+                                                            int rows_per_block) {
+  __shared__ float acc[1024 * 8];
+  const int t = threadIdx.x, C8 = C / 8, groups = 1024 / C8;
   const int c8 = t % C8, r = t / C8;
   const int p0 = blockIdx.x * rows_per_block;
   const int p1 = min(P, p0 + rows_per_block);
   float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (r < groups) {
     int p = p0 + r;
-    for (; p + 3 * groups < p1; p += 4 * groups) {
-      bf16x8 v[4];
+    for (; p + 7 * groups < p1; p += 8 * groups) {
+      bf16x8 v[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = load8(g + (size_t)(p + u * groups) * C + 8 * c8);
+      for (int u = 0; u < 8; ++u) v[u] = load8(g + (size_t)(p + u * groups) * C + 8 * c8);
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 8; ++u)
 #pragma unroll
         for (int k = 0; k < 8; ++k) s[k] += (float)v[u][k];
     }
@@ -951,11 +965,47 @@ __global__ void __launch_bounds__(256) bias_grad_v8_kernel(const __bf16* __restr
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[t * 8 + k] = s[k];
   __syncthreads();
-  for (int c = t; c < C; c += 256) {
-    float u = 0.0f;
-    for (int q = 0; q < groups; ++q) u += acc[(q * C8 + c / 8) * 8 + c % 8];
-    part[(size_t)blockIdx.x * C + c] = u;
+  fold_groups(acc, t, C8, groups);
+  for (int c = t; c < C; c += 1024) part[(size_t)blockIdx.x * C + c] = acc[c];
+}
+
+// db[c] = sum over the nb block partials part[b][c] (C % 8 == 0, C <= 1024):
+// one block of 1024 threads, thread t adds the partials b = t / C8, + G, ...
+// of 8 channels (four in flight), then the groups fold as above.
+__global__ void __launch_bounds__(1024) bias_reduce_kernel(const float* __restrict__ part,
+                                                           float* db, int C, int nb) {
+  __shared__ float acc[1024 * 8];
+  const int t = threadIdx.x, C8 = C / 8, groups = 1024 / C8;
+  const int c8 = t % C8, r = t / C8;
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < groups) {
+    int b = r;
+    for (; b + 3 * groups < nb; b += 4 * groups) {
+      float4 v[4][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4* q = reinterpret_cast<const float4*>(part + (size_t)(b + u * groups) * C + 8 * c8);
+        v[u][0] = q[0];
+        v[u][1] = q[1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s[0] += v[u][0].x; s[1] += v[u][0].y; s[2] += v[u][0].z; s[3] += v[u][0].w;
+        s[4] += v[u][1].x; s[5] += v[u][1].y; s[6] += v[u][1].z; s[7] += v[u][1].w;
+      }
+    }
+    for (; b < nb; b += groups) {
+      const float4* q = reinterpret_cast<const float4*>(part + (size_t)b * C + 8 * c8);
+      const float4 v0 = q[0], v1 = q[1];
+      s[0] += v0.x; s[1] += v0.y; s[2] += v0.z; s[3] += v0.w;
+      s[4] += v1.x; s[5] += v1.y; s[6] += v1.z; s[7] += v1.w;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[t * 8 + k] = s[k];
+  __syncthreads();
+  fold_groups(acc, t, C8, groups);
+  for (int c = t; c < C; c += 1024) db[c] = acc[c];
 }
 
 __global__ void relu_bwd_kernel(const __bf16* __restrict__ y, const __bf16* __restrict__ g,
@@ -1514,15 +1564,22 @@ int shy_nn_conv_wgrad_noalloc(const void* x, const void* g, float* dw, int N, in
 
 int shy_nn_bias_grad(const void* g, float* db, float* part, int part_rows, int P, int C,
                      void* stream) {
-  if (!g || !db || !part || C > 1024 || part_rows < 1) return nn_fail("bias_grad: bad argument");
-  // at least 4096 pixels per block: fewer partials for the reduce to add
-  const int rows = std::max((P + part_rows - 1) / part_rows, 4096);
-  const int nb = (P + rows - 1) / rows;
+  if (!g || !db || !part || C < 1 || C > 1024 || P < 1 || part_rows < 1)
+    return nn_fail("bias_grad: bad argument");
   hipStream_t s = (hipStream_t)stream;
-  if (C % 8 == 0 && ((uintptr_t)g & 15) == 0)
-    bias_grad_v8_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
-  else
-    bias_grad_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
+  if (C % 8 == 0 && ((uintptr_t)g & 15) == 0 && ((uintptr_t)part & 15) == 0) {
+    // at least 2048 pixels per block (two 1024-thread blocks per CU on the
+    // largest layers, eight rows in flight per thread)
+    const int rows = std::max((P + part_rows - 1) / part_rows, 2048);
+    const int nb = (P + rows - 1) / rows;
+    bias_grad_v8_kernel<<<nb, 1024, 0, s>>>((const __bf16*)g, part, P, C, rows);
+    if (int r = nn_check("bias_grad")) return r;
+    bias_reduce_kernel<<<1, 1024, 0, s>>>(part, db, C, nb);
+    return nn_check("bias_grad_reduce");
+  }
+  const int rows = (P + part_rows - 1) / part_rows;
+  const int nb = (P + rows - 1) / rows;
+  bias_grad_kernel<<<nb, 256, 0, s>>>((const __bf16*)g, part, P, C, rows);
   if (int r = nn_check("bias_grad")) return r;
   wgrad_reduce_kernel<<<(C + 31) / 32, 256, 0, s>>>(part, db, C, nb);
   return nn_check("bias_grad_reduce");
