@@ -96,6 +96,7 @@ struct shy_ctx {
   int4* dlog = nullptr;  // [drop] how each walk ended
   bool dlog_valid = false;  // dlog holds a cycle of the current wave geometry
   int32_t* dperm = nullptr;  // [drop] the terrain's drop each walk thread takes (order_kernel)
+  long long pack_recs = 0;   // packed walk records over all terrains (pack_records)
   int drop_order = -1;       // assign drops to walk threads by spawn position: 1, 0, -1 auto
   int order_chunks = 0;      // blocks of order_kernel
   int32_t* wsteps = nullptr;  // [drop] steps of each walk
@@ -222,11 +223,13 @@ bool volumes_in_division_range(const shy_params& P) {
 }
 
 int layout(shy_ctx* c) {
-  long long cells = 0, drops = 0, tiles = 0;
+  long long cells = 0, drops = 0, tiles = 0, recs = 0;
   const int ty = kTileY[c->tune.tile], tx = kTileX[c->tune.tile];
   for (Terrain& t : c->ter) {
     t.vol_ok = volumes_in_division_range(t.P) ? 1 : 0;
     t.cell_off = cells;
+    t.pack_off = recs;
+    recs += pack_records(t.nx, t.ny);
     t.drop_off = (int)drops;
     t.tiles_y = (t.ny + ty - 1) / ty;
     t.tile_off = (int)tiles;
@@ -515,8 +518,10 @@ int launch_cycle(shy_ctx* c, int p, hipEvent_t* ev = nullptr) {
   const int nter = (int)c->ter.size();
   const bool walk = c->total_drops > 0;
   if (walk) {
-    pack_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, n, c->h[p],
-                                                                     c->fld, c->pack, c->gflag);
+    const size_t nrec = (size_t)c->pack_recs;
+    pack_kernel<<<(unsigned)((nrec + 255) / 256), 256, 0, c->stream>>>(c->d_ter, nter, nrec,
+                                                                        c->h[p], c->fld, c->pack,
+                                                                        c->gflag);
     if (ordered(c))
       order_kernel<<<c->order_chunks, 1024, 0, c->stream>>>(c->d_ter, nter, c->total_drops, bs, c->state,
                                                  c->dperm);
@@ -830,7 +835,7 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if (!out) return fail(nullptr, SHY_ERR_ARG, "shy_ensemble_create: out is NULL");
   *out = nullptr;
   if (!desc || nter < 1) return fail(nullptr, SHY_ERR_ARG, "shy_ensemble_create: need >= 1 terrain");
-  long long cells = 0;
+  long long cells = 0, recs = 0;  // cells and packed walk records
   for (int t = 0; t < nter; ++t) {
     const shy_terrain_desc& d = desc[t];
     if (d.nx < 1 || d.ny < 1 || (long long)d.nx * d.ny > (1LL << 31) - 1 || d.drops < 0)
@@ -838,8 +843,9 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
                   "shy_create: nx, ny must be >= 1 with nx*ny < 2^31, drops >= 0");
     if (!params_ok(&d.params)) return fail(nullptr, SHY_ERR_ARG, std::string("shy_create: ") + kParamsMsg);
     cells += (long long)d.nx * d.ny;
+    recs += pack_records(d.nx, d.ny);
   }
-  if (cells > (1LL << 31) - 1)
+  if (cells > (1LL << 31) - 1 || recs > (1LL << 31) - 1)
     return fail(nullptr, SHY_ERR_ARG, "shy_ensemble_create: total cells must stay below 2^31");
   if (shy_device_count() < 1) return fail(nullptr, SHY_ERR_NODEV, "shy_create: no GPU visible");
   shy_ctx* c = new shy_ctx();
@@ -879,7 +885,9 @@ int shy_ensemble_create(const shy_terrain_desc* desc, int nter, shy_ctx** out) {
   if ((e = hipMalloc(&c->root, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc root");
   if ((e = hipMalloc(&c->first, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc first");
   if ((e = hipMalloc(&c->sr, n * sizeof(float))) != hipSuccess) return bail(e, "hipMalloc sr");
-  if ((e = hipMalloc(&c->pack, 2 * n * sizeof(float4))) != hipSuccess) return bail(e, "hipMalloc pack");
+  c->pack_recs = recs;
+  if ((e = hipMalloc(&c->pack, 2 * (size_t)recs * sizeof(float4))) != hipSuccess)
+    return bail(e, "hipMalloc pack");
   if ((e = hipMalloc(&c->dh, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc dh");
   if ((e = hipMalloc(&c->trk, 3 * n * sizeof(long long))) != hipSuccess) return bail(e, "hipMalloc trk");
   if ((e = hipMalloc(&c->deposit, n * sizeof(int32_t))) != hipSuccess) return bail(e, "hipMalloc deposit");

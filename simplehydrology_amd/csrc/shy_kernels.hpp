@@ -4,7 +4,8 @@
 // drops that all read the map as it stood when the cycle began:
 //
 //  pack_kernel           one 32-byte record per cell of what a walk step reads
-//                        there, the cell's normal precomputed
+//                        there, the cell's normal precomputed, in 2x2 cell
+//                        blocks (four records per 128-byte line)
 //  order_kernel          (many drops) which drop each walk thread takes
 //  walk_kernel           one drop per lane walks its trajectory; for every
 //                        step it logs, coalesced as [step][drop], 16 bytes
@@ -93,6 +94,7 @@ struct DeviceState {
 // A single-terrain context is an ensemble of one.
 struct Terrain {
   long long cell_off;
+  long long pack_off;  // the terrain's first packed walk record (pack_rec)
   int nx, ny;
   int drop_off, drops;
   int tile_off, tiles_y;
@@ -140,7 +142,7 @@ struct WaveArgs {
   int32_t* steps;     // [drop] steps of the walk (the log rows it filled)
   int32_t* hist;      // per-terrain [tile][walk block] step counts, when the walk counts them
   const int32_t* __restrict__ perm;  // [drop] the terrain's drop a thread walks (order_kernel), or null
-  const float4* __restrict__ pack;  // per cell 32 B: what one walk step reads (pack_kernel)
+  const float4* __restrict__ pack;  // per cell 32 B: what one walk step reads (pack_kernel), 2x2 cell blocks (pack_rec)
   int32_t* deposit;
   uint8_t* visit;
   DeviceState* state;
@@ -301,7 +303,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
   int2 ps = make_int2(a.cell_off + i0, 0), pt = make_int2(0, 0);
   // the count walk reads the cycle's packed records, the trace the map arrays
   auto load_cell = [&](int x, int y, int i) {
-    if constexpr (COUNT) return load_packed(a.pack, i);
+    if constexpr (COUNT) return load_packed(a.pack, pack_rec(x, y, a.ny));
     else return load_unpacked(a, x, y, i);
   };
   CellView c = load_cell(ix, iy, i0);
@@ -329,7 +331,7 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
       int qx = (int)(px + sx), qy = (int)(py + sy);
       qx = qx < 0 ? 0 : (qx >= nx ? nx - 1 : qx);
       qy = qy < 0 ? 0 : (qy >= ny ? ny - 1 : qy);
-      pf = a.pack[2 * (size_t)(qx * ny + qy)].x;
+      pf = a.pack[2 * (size_t)pack_rec(qx, qy, ny)].x;
     }
     float n0, n1, n2;
     if constexpr (COUNT) {  // the packed record holds the cell's gravity terms (water.h:70, 95)
@@ -445,24 +447,31 @@ __device__ int walk_drop(const WalkArgs& a, int lane_drop, float x0, float y0, i
 // (water.h:101-104) depend only on the cell's snapshot, so they are computed
 // here once per cell, with the walk's own arithmetic, instead of at every
 // step: a shorter chain per step.
-__global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t n,
+// One thread per record, numbered over all terrains in record order
+// (Terrain::pack_off), so that four consecutive threads write one 128-byte
+// line; padding records (odd nx or ny) are skipped and never read.
+__global__ void pack_kernel(const Terrain* __restrict__ ter, int nter, size_t nrec,
                             const float* __restrict__ h, const float4* __restrict__ fld,
                             float4* pack, int* gflag) {
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const int t = find_terrain(nter, (long long)g, [&](int q) { return ter[q].cell_off; });
+  if (g >= nrec) return;
+  const int t = find_terrain(nter, (long long)g, [&](int q) { return ter[q].pack_off; });
   const Terrain& T = ter[t];
-  const size_t i = g - (size_t)T.cell_off;
-  const int ny = T.ny, x = (int)(i / (size_t)ny), y = (int)(i % (size_t)ny);
+  const int nx = T.nx, ny = T.ny, nyb = (ny + 1) >> 1;
+  const int rr = (int)(g - (size_t)T.pack_off), b = rr >> 2, q = rr & 3;
+  const int bx = b / nyb, by = b - bx * nyb;
+  const int x = 2 * bx + (q >> 1), y = 2 * by + (q & 1);  // pack_rec(x, y, ny) = rr
+  if (x >= nx || y >= ny) return;
+  const size_t i = (size_t)x * ny + y;
   const float* hh = h + T.cell_off;
   const float hc = hh[i];
-  const float xm = hh[x > 0 ? i - ny : i], xp = hh[x + 1 < T.nx ? i + ny : i];
+  const float xm = hh[x > 0 ? i - ny : i], xp = hh[x + 1 < nx ? i + ny : i];
   const float ym = hh[y > 0 ? i - 1 : i], yp = hh[y + 1 < ny ? i + 1 : i];
-  const float4 f = fld[g];
+  const float4 f = fld[(size_t)T.cell_off + i];
   CellView c;
   c.h = hc; c.xm = xm; c.xp = xp; c.ym = ym; c.yp = yp;
   float n0, n1, n2;
-  normal_from(c, x, y, T.nx, ny, n0, n1, n2);  // water.h:70, once per cell instead of per step
+  normal_from(c, x, y, nx, ny, n0, n1, n2);  // water.h:70, once per cell instead of per step
   // the gravity terms k n (water.h:95's numerators; the terrain is flagged
   // where one lies outside the range of the walk's reciprocal-form division),
   // and 1/|momentum| for the coupling term (read only where |momentum| > 0)
@@ -521,7 +530,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WaveArgs w) {
       a.cell_off = (int)T.cell_off;
       a.slog = w.slog;
       a.strk = w.strk;
-      a.pack = w.pack ? w.pack + 2 * T.cell_off : nullptr;
+      a.pack = w.pack ? w.pack + 2 * T.pack_off : nullptr;
       a.nx = T.nx;
       a.ny = T.ny;
       a.drops = w.total_drops;  // log stride
