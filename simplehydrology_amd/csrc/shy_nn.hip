@@ -8,16 +8,16 @@
 // that instruction (checked on the device): lane l holds A[l%16][(l/16)*8+i],
 // B[(l/16)*8+i][l%16] for i < 8 and C[(l/16)*4+i][l%16] for i < 4.
 //
-//   conv_tiles_kernel  3x3 forward / data gradient of layers with 32 or 64
-//                      output channels and one or two 32-channel input
-//                      chunks: persistent blocks (three per CU, 32 output
-//                      channels each) over 16x16 pixel tiles, patch and
-//                      weights staged in LDS, the next tile's patch loaded
-//                      into registers during the current tile's MFMAs;
-//                      optionally a 1x1 head on the rounded output
+//   conv_tiles_kernel  3x3 forward / data gradient of every layer whose
+//                      16x16 pixel tiles give each CU at least one:
+//                      persistent blocks (three per CU, 32 output channels
+//                      each) over the tiles, patch and weights staged in LDS,
+//                      the next chunk's patch loaded into registers during
+//                      the current chunk's MFMAs; decoder inputs read as
+//                      [up2(x), skip] from two tensors; optionally a 1x1
+//                      head on the rounded output
 //   conv_pf_kernel     the same arithmetic with one tile per block for the
-//                      other layers (the decoder inputs read as [up2(x),
-//                      skip] from two tensors)
+//                      smaller layers (16x8 tiles: more blocks)
 //   conv_c8_kernel     the 8-channel first layer, four taps per MFMA k-step
 //   conv_fwd_kernel    direct-load convolution of the remaining shapes (the
 //                      head's data gradient)
@@ -1292,7 +1292,8 @@ int launch_tiles(const ConvArgs& a, hipStream_t s, const char* what) {
   const int gy = a.Cout / (16 * MT);
   const long long cap = std::max(1LL, (long long)kTileBlocksPerCU * device_cus() / gy);
   const dim3 g((unsigned)std::min(tiles, cap), (unsigned)gy);
-  if (a.head_w) conv_tiles_kernel<9, MT, NT, false, true><<<g, 256, 0, s>>>(a);
+  if (a.x2) conv_tiles_kernel<9, MT, NT, true, false><<<g, 256, 0, s>>>(a);
+  else if (a.head_w) conv_tiles_kernel<9, MT, NT, false, true><<<g, 256, 0, s>>>(a);
   else conv_tiles_kernel<9, MT, NT, false, false><<<g, 256, 0, s>>>(a);
   return nn_check(what);
 }
@@ -1302,13 +1303,14 @@ int launch_tiles(const ConvArgs& a, hipStream_t s, const char* what) {
 // the blocks, and 210-256 VGPRs), 16x16 tiles (4 rows per wave) when they
 // still give >= 1024 blocks, else 16x8.
 int launch_pf(const ConvArgs& a, int taps, hipStream_t s, const char* what) {
-  // 32 or 64 output channels and one or two input chunks: the persistent
-  // tile loop (32->32 at 16 x 256^2: 50.7 us in the one-tile blocks below,
-  // 43.4 us; with more channels or chunks the one-tile blocks are faster)
-  if (!a.x2 && taps == 9 && a.Cin <= 64 && (a.Cout == 32 || a.Cout == 64))
-    return launch_tiles(a, s, what);
-  const unsigned gy = (unsigned)(a.Cout / 32);
   const long long tiles16 = (long long)a.N * ((a.H + 15) / 16) * ((a.W + kTW - 1) / kTW);
+  // 3x3 layers whose 16x16 tiles give every CU at least one: the persistent
+  // tile loop, decoder inputs included (at batch 16, against the one-tile
+  // blocks below: 32->32 at 256^2 50.7 -> 43.4 us, [up2, skip] 96->32 at
+  // 256^2 105 -> 83 us, 96->96 at 64^2 23.0 -> 21.5 us; at 32^2, 64 tiles,
+  // the one-tile blocks' 16x8 tiles are faster)
+  if (taps == 9 && tiles16 >= device_cus()) return launch_tiles(a, s, what);
+  const unsigned gy = (unsigned)(a.Cout / 32);
   const long long tiles8 = (long long)a.N * ((a.H + 7) / 8) * ((a.W + kTW - 1) / kTW);
   if (tiles8 > INT32_MAX) return nn_fail(std::string(what) + ": too many tiles");
   const bool big = tiles16 * gy >= 1024;
