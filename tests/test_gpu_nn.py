@@ -201,7 +201,9 @@ def test_unet_gradients_match_fp32_reference(sg):
 
 
 @pytest.mark.parametrize("N,H,W,c1,c2,cout", [(2, 16, 24, 64, 32, 32), (1, 32, 16, 96, 64, 64),
-                                               (3, 8, 40, 128, 96, 96)])
+                                               (3, 8, 40, 128, 96, 96),
+                                               # enough 16x16 tiles for the persistent tile loop
+                                               (1, 256, 256, 64, 32, 32), (2, 200, 232, 96, 64, 64)])
 def test_conv_upcat_equals_concatenated_input(sg, N, H, W, c1, c2, cout):
     """shy_nn_conv_fwd_upcat (the decoder's first convolution reading the
     upsampled half and the skip straight from their tensors) gives the bits of
