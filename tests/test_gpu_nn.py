@@ -231,7 +231,9 @@ def test_conv_upcat_equals_concatenated_input(sg, N, H, W, c1, c2, cout):
 
 
 @pytest.mark.parametrize("N,H,W,cin,cout", [(2, 16, 24, 32, 32), (1, 40, 24, 64, 32),
-                                             (2, 8, 16, 32, 96)])
+                                             (2, 8, 16, 32, 96),
+                                             # enough 16x16 tiles for the persistent tile loop
+                                             (1, 256, 256, 32, 64), (2, 200, 232, 64, 96)])
 def test_conv_dgrad_epilogue_equals_separate_passes(sg, N, H, W, cin, cout):
     """shy_nn_conv_dgrad with gres and mask gives the bits of the plain data
     gradient followed by shy_nn_add and shy_nn_relu_bwd (the backward pass of
