@@ -8,7 +8,8 @@
  *
  * Tensors are device pointers the caller owns; activations are NHWC bf16;
  * weights are bf16 Wt[Cout][taps*Cin] (tap-major, tap = 3*(dy+1)+(dx+1));
- * biases and gradients FP32.  All products use v_mfma_f32_16x16x32_bf16 with
+ * biases and gradients FP32 (a convolution's bias 16-byte aligned: the
+ * epilogue reads 4 at once).  All products use v_mfma_f32_16x16x32_bf16 with
  * FP32 accumulation.  Every call is asynchronous on `stream` (a hipStream_t;
  * NULL = the default stream) and returns 0, or SHY_NN_ERR with
  * shy_nn_last_error() describing the problem.

@@ -108,8 +108,14 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 template <bool HEAD>
 __device__ inline bf16x4 store_out4(const ConvArgs& a, size_t o, int co, const f32x4& acc) {
   float v[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = acc[i] + (a.bias ? a.bias[co + i] : 0.0f);
+  // the lane's 4 biases in one load: co is a multiple of 4 and the entry
+  // points require a 16-byte aligned bias
+  const float4 b = a.bias ? *reinterpret_cast<const float4*>(a.bias + co)
+                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  v[0] = acc[0] + b.x;
+  v[1] = acc[1] + b.y;
+  v[2] = acc[2] + b.z;
+  v[3] = acc[3] + b.w;
   if (a.res) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] += (float)a.res[o + i];
@@ -1368,6 +1374,7 @@ int shy_nn_conv_fwd(const void* x, const void* w, const float* bias, const void*
                     float* y_f32, int N, int H, int W, int Cin, int Cout, int taps, int relu,
                     void* stream) {
   if (!x || !w || (!y_bf16 && !y_f32) || (y_bf16 && y_f32)) return nn_fail("conv_fwd: bad pointers");
+  if ((uintptr_t)bias & 15) return nn_fail("conv_fwd: bias must be 16-byte aligned");
   if (N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (taps != 9 && taps != 1))
     return nn_fail("conv_fwd: need Cin a multiple of 8 and taps 9 or 1");
   ConvArgs a;
@@ -1413,6 +1420,7 @@ int shy_nn_conv_fwd_head(const void* x, const void* w, const float* bias, const 
                          void* y_bf16, const void* head_w, const float* head_b, float* head_y,
                          int N, int H, int W, int Cin, int head_c, int relu, void* stream) {
   if (!x || !w || !head_w || !head_y) return nn_fail("conv_fwd_head: bad pointers");
+  if ((uintptr_t)bias & 15) return nn_fail("conv_fwd_head: bias must be 16-byte aligned");
   if (N < 1 || H < 1 || W < 1 || (Cin != 32 && Cin != 64) || head_c < 4 || head_c > 16 ||
       head_c % 4)
     return nn_fail("conv_fwd_head: need Cin 32 or 64 and head_c in {4, 8, 12, 16}");
@@ -1442,6 +1450,7 @@ int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, c
                           int C1, int C2, int Cout, int relu, void* stream) {
   if (!x_half || !skip || !w || (!y_bf16 && !y_f32) || (y_bf16 && y_f32))
     return nn_fail("conv_fwd_upcat: bad pointers");
+  if ((uintptr_t)bias & 15) return nn_fail("conv_fwd_upcat: bias must be 16-byte aligned");
   if (N < 1 || H < 2 || W < 2 || H % 2 || W % 2 || C1 < 32 || C1 % 32 || C2 < 32 || C2 % 32 ||
       Cout < 32 || Cout % 32)
     return nn_fail("conv_fwd_upcat: need even H, W and C1, C2, Cout multiples of 32");

@@ -286,6 +286,22 @@ def test_conv_head_equals_conv_then_head(sg, N, H, W, cin, hc, with_res):
     assert torch.equal(o2.view(torch.int32), o1.view(torch.int32))
 
 
+def test_conv_rejects_unaligned_bias(sg):
+    """The convolution entry points read a lane's four biases in one 16-byte
+    load, so they reject a bias that is not 16-byte aligned (before any
+    launch)."""
+    dev = "cuda:0"
+    x = torch.zeros(1, 16, 16, 32, dtype=torch.bfloat16, device=dev)
+    w = torch.zeros(32, 9 * 32, dtype=torch.bfloat16, device=dev)
+    b = torch.zeros(33, dtype=torch.float32, device=dev)
+    y = torch.empty(1, 16, 16, 32, dtype=torch.bfloat16, device=dev)
+    rc = sg.nn().shy_nn_conv_fwd(sg._p(x), sg._p(w), b.data_ptr() + 4, None, sg._p(y), None,
+                                 1, 16, 16, 32, 32, 9, 1, sg._stream())
+    assert rc != 0 and b"aligned" in sg.nn().shy_nn_last_error()
+    assert sg.nn().shy_nn_conv_fwd(sg._p(x), sg._p(w), b.data_ptr(), None, sg._p(y), None,
+                                   1, 16, 16, 32, 32, 9, 1, sg._stream()) == 0
+
+
 def test_unet_inference_forward_equals_training_forward(sg):
     """The inference forward (decoder inputs read without concatenation)
     returns the training forward's bits."""
