@@ -43,8 +43,9 @@ int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, c
 /* shy_nn_conv_fwd (3x3, Cout = 32, BF16 output y, which may be null: not
    stored) with a 1x1 convolution applied to the rounded output in the same
    pass: head_y [N][H][W][head_c] (FP32) = head_b + head_w [head_c][32] . y,
-   head_c in {4, 8, 12, 16}, Cin 32 or 64.  A network's last convolution and
-   its output head without the 32-channel tensor round trip. */
+   head_c in {4, 8, 12, 16}, Cin 32 or 64, head_w and head_b 16-byte
+   aligned.  A network's last convolution and its output head without the
+   32-channel tensor round trip. */
 int shy_nn_conv_fwd_head(const void* x, const void* w, const float* bias, const void* res,
                          void* y_bf16, const void* head_w, const float* head_b, float* head_y,
                          int N, int H, int W, int Cin, int head_c, int relu, void* stream);

@@ -156,11 +156,19 @@ template <int NT>
 __device__ inline void conv_head(const ConvArgs& a, const bf16x4 (&q)[2][NT], int n, int y0,
                                  int x0, int wave, int lane) {
   const int hc = a.head_c, o = lane & 15, g = lane >> 4;
+  // two 8-byte loads (head_w is 16-byte aligned, rows of 32)
+  bf16x4 w0, w1;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) w0[j] = w1[j] = (__bf16)0.0f;
+  if (o < hc) {
+    w0 = *reinterpret_cast<const bf16x4*>(a.head_w + o * 32 + 4 * g);
+    w1 = *reinterpret_cast<const bf16x4*>(a.head_w + o * 32 + 16 + 4 * g);
+  }
   bf16x8 A;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    A[j] = o < hc ? a.head_w[o * 32 + 4 * g + j] : (__bf16)0.0f;
-    A[4 + j] = o < hc ? a.head_w[o * 32 + 16 + 4 * g + j] : (__bf16)0.0f;
+    A[j] = w0[j];
+    A[4 + j] = w1[j];
   }
   const int gx = x0 + (lane & 15), oc = 4 * g;
 #pragma unroll
@@ -175,10 +183,10 @@ __device__ inline void conv_head(const ConvArgs& a, const bf16x4 (&q)[2][NT], in
     const int gy = y0 + wave * NT + t;
     if (gy < a.H && gx < a.W && oc < hc) {
       const size_t p = (size_t)(n * a.H + gy) * a.W + gx;
-      float v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = h[i] + (a.head_b ? a.head_b[oc + i] : 0.0f);
-      *reinterpret_cast<float4*>(a.head_y + p * hc + oc) = make_float4(v[0], v[1], v[2], v[3]);
+      const float4 b = a.head_b ? *reinterpret_cast<const float4*>(a.head_b + oc)
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // 16-byte aligned
+      *reinterpret_cast<float4*>(a.head_y + p * hc + oc) =
+          make_float4(h[0] + b.x, h[1] + b.y, h[2] + b.z, h[3] + b.w);
     }
   }
 }
@@ -1420,7 +1428,8 @@ int shy_nn_conv_fwd_head(const void* x, const void* w, const float* bias, const 
                          void* y_bf16, const void* head_w, const float* head_b, float* head_y,
                          int N, int H, int W, int Cin, int head_c, int relu, void* stream) {
   if (!x || !w || !head_w || !head_y) return nn_fail("conv_fwd_head: bad pointers");
-  if ((uintptr_t)bias & 15) return nn_fail("conv_fwd_head: bias must be 16-byte aligned");
+  if (((uintptr_t)bias | (uintptr_t)head_b | (uintptr_t)head_w) & 15)
+    return nn_fail("conv_fwd_head: bias, head_w and head_b must be 16-byte aligned");
   if (N < 1 || H < 1 || W < 1 || (Cin != 32 && Cin != 64) || head_c < 4 || head_c > 16 ||
       head_c % 4)
     return nn_fail("conv_fwd_head: need Cin 32 or 64 and head_c in {4, 8, 12, 16}");

@@ -300,6 +300,18 @@ def test_conv_rejects_unaligned_bias(sg):
     assert rc != 0 and b"aligned" in sg.nn().shy_nn_last_error()
     assert sg.nn().shy_nn_conv_fwd(sg._p(x), sg._p(w), b.data_ptr(), None, sg._p(y), None,
                                    1, 16, 16, 32, 32, 9, 1, sg._stream()) == 0
+    # the fused head also reads its weights and biases 16 bytes at a time
+    hw = torch.zeros(8 * 32 + 8, dtype=torch.bfloat16, device=dev)
+    hb = torch.zeros(12, dtype=torch.float32, device=dev)
+    hy = torch.empty(1, 16, 16, 8, dtype=torch.float32, device=dev)
+    for wofs, bofs in ((2, 0), (0, 4)):
+        rc = sg.nn().shy_nn_conv_fwd_head(sg._p(x), sg._p(w), b.data_ptr(), None, sg._p(y),
+                                          hw.data_ptr() + wofs, hb.data_ptr() + bofs,
+                                          sg._p(hy), 1, 16, 16, 32, 8, 1, sg._stream())
+        assert rc != 0 and b"aligned" in sg.nn().shy_nn_last_error()
+    assert sg.nn().shy_nn_conv_fwd_head(sg._p(x), sg._p(w), b.data_ptr(), None, sg._p(y),
+                                        sg._p(hw), sg._p(hb), sg._p(hy), 1, 16, 16, 32, 8, 1,
+                                        sg._stream()) == 0
 
 
 def test_unet_inference_forward_equals_training_forward(sg):
