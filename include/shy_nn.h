@@ -49,6 +49,13 @@ int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, c
 int shy_nn_conv_fwd_head(const void* x, const void* w, const float* bias, const void* res,
                          void* y_bf16, const void* head_w, const float* head_b, float* head_y,
                          int N, int H, int W, int Cin, int head_c, int relu, void* stream);
+/* shy_nn_conv_fwd (3x3) that also writes the 2x2 average pool of its
+   output, pool_y [N][H/2][W/2][Cout] (8-byte aligned), with the bits of
+   shy_nn_pool2 on y: an encoder stage's last convolution without reading its
+   output back for the pool.  H, W even; Cin, Cout multiples of 32. */
+int shy_nn_conv_fwd_pool(const void* x, const void* w, const float* bias, const void* res,
+                         void* y_bf16, void* pool_y, int N, int H, int W, int Cin, int Cout,
+                         int relu, void* stream);
 /* The data gradient dx [N][H][W][Cin] (BF16) of a convolution from g
    [N][H][W][Cout] with the flipped weights wf of shy_nn_flip_weights, with
    the backward pass's next two element-wise steps in its epilogue: gres

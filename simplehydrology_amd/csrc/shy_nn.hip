@@ -15,7 +15,9 @@
 //                      the next chunk's patch loaded into registers during
 //                      the current chunk's MFMAs; decoder inputs read as
 //                      [up2(x), skip] from two tensors; optionally a 1x1
-//                      head on the rounded output
+//                      head on the rounded output, or its 2x2 average pool
+//                      (pairs of rows in a lane, pairs of columns across
+//                      lanes l and l ^ 1)
 //   conv_pf_kernel     the same arithmetic with one tile per block for the
 //                      smaller layers (16x8 tiles: more blocks)
 //   conv_c8_kernel     the 8-channel first layer, four taps per MFMA k-step
@@ -96,6 +98,10 @@ struct ConvArgs {
   const float* __restrict__ head_b = nullptr;
   float* head_y = nullptr;
   int head_c = 0;
+  // When pool_y is set (conv_tiles_kernel, H and W even): also the 2x2
+  // average of the rounded output, [N][H/2][W/2][Cout], with the bits of
+  // pool2_fwd_v8_kernel on y.
+  __bf16* pool_y = nullptr;
 };
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -194,9 +200,10 @@ __device__ inline void conv_head(const ConvArgs& a, const bf16x4 (&q)[2][NT], in
 // The epilogue of a block's tile: lane holds channels cb + 16m + 4(lane/16)
 // + i of pixel (y0 + NT wave + t, x0 + lane%16).  HEAD: then the 1x1 head
 // (its own instantiation: the rounded outputs it keeps cost registers).
-template <int MT, int NT, bool HEAD = false>
+template <int MT, int NT, bool HEAD = false, bool POOL = false>
 __device__ inline void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][NT], int n, int y0,
                                      int x0, int cb, int wave, int lane) {
+  static_assert(!POOL || NT % 2 == 0, "the pool pairs a wave's rows");
   static_assert(!HEAD || MT == 2, "the head reads a block's 32 output channels");
   const int gx = x0 + (lane & 15);
   bf16x4 q[MT][NT];
@@ -214,6 +221,41 @@ __device__ inline void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][N
     }
   }
   if constexpr (HEAD) conv_head<NT>(a, q, n, y0, x0, wave, lane);
+  if constexpr (POOL) {
+    // Rows y0 + NT wave + t pair as (t, t + 1) for even t (y0 and NT wave
+    // are even); columns pair as lanes (l, l ^ 1).  The even lane of a pair
+    // sums (top left, top right, bottom left, bottom right) in the order
+    // of pool2_fwd_v8_kernel.  H and W are even, so a pooled pixel lies in
+    // bounds exactly when its top-left input pixel does.
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int co = cb + m * 16 + (lane >> 4) * 4;
+#pragma unroll
+      for (int t = 0; t < NT; t += 2) {
+        bf16x4 r0, r1;
+        {
+          int2 u0 = *reinterpret_cast<const int2*>(&q[m][t]);
+          int2 u1 = *reinterpret_cast<const int2*>(&q[m][t + 1]);
+          u0.x = __shfl_xor(u0.x, 1);
+          u0.y = __shfl_xor(u0.y, 1);
+          u1.x = __shfl_xor(u1.x, 1);
+          u1.y = __shfl_xor(u1.y, 1);
+          r0 = *reinterpret_cast<const bf16x4*>(&u0);
+          r1 = *reinterpret_cast<const bf16x4*>(&u1);
+        }
+        const int gy = y0 + wave * NT + t;
+        if ((lane & 1) == 0 && gy < a.H && gx < a.W && co < a.Cout) {
+          bf16x4 o;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            o[i] = (__bf16)(0.25f * ((float)q[m][t][i] + (float)r0[i] + (float)q[m][t + 1][i] +
+                                     (float)r1[i]));
+          const size_t pp = ((size_t)(n * (a.H >> 1) + (gy >> 1)) * (a.W >> 1) + (gx >> 1)) * a.Cout;
+          *reinterpret_cast<bf16x4*>(a.pool_y + pp + co) = o;
+        }
+      }
+    }
+  }
 }
 
 // Forward (and dgrad) convolution.  Block = 4 waves; each wave owns 32 pixels
@@ -445,7 +487,7 @@ __global__ void __launch_bounds__(256, 2) conv_pf_kernel(ConvArgs a) {
 // channels per block (MT = 2): at most 168 VGPRs and 45 KB of LDS each.
 constexpr int kTileBlocksPerCU = 3;
 
-template <int TAPS, int MT, int NT, bool UPCAT = false, bool HEAD = false>
+template <int TAPS, int MT, int NT, bool UPCAT = false, bool HEAD = false, bool POOL = false>
 __global__ void __launch_bounds__(256, kTileBlocksPerCU) conv_tiles_kernel(ConvArgs a) {
   static_assert(MT == 2, "three blocks per CU fit 32 output channels per block");
   constexpr int kTH = 4 * NT;
@@ -570,8 +612,8 @@ __global__ void __launch_bounds__(256, kTileBlocksPerCU) conv_tiles_kernel(ConvA
       }
     }
     const int tx = tile % tiles_x, r = tile / tiles_x;
-    conv_epilogue<MT, NT, HEAD>(a, acc, r / tiles_y, (r % tiles_y) * kTH, tx * kTW, cb, wave,
-                                lane);
+    conv_epilogue<MT, NT, HEAD, POOL>(a, acc, r / tiles_y, (r % tiles_y) * kTH, tx * kTW, cb,
+                                      wave, lane);
     tile += (int)gridDim.x;
     if (tile >= total) break;
   }
@@ -1308,6 +1350,7 @@ int launch_tiles(const ConvArgs& a, hipStream_t s, const char* what) {
   const dim3 g((unsigned)std::min(tiles, cap), (unsigned)gy);
   if (a.x2) conv_tiles_kernel<9, MT, NT, true, false><<<g, 256, 0, s>>>(a);
   else if (a.head_w) conv_tiles_kernel<9, MT, NT, false, true><<<g, 256, 0, s>>>(a);
+  else if (a.pool_y) conv_tiles_kernel<9, MT, NT, false, false, true><<<g, 256, 0, s>>>(a);
   else conv_tiles_kernel<9, MT, NT, false, false><<<g, 256, 0, s>>>(a);
   return nn_check(what);
 }
@@ -1480,6 +1523,40 @@ int shy_nn_conv_fwd_upcat(const void* x_half, const void* skip, const void* w, c
   return launch_pf(a, 9, (hipStream_t)stream, "conv_fwd_upcat");
 }
 
+
+int shy_nn_conv_fwd_pool(const void* x, const void* w, const float* bias, const void* res,
+                         void* y_bf16, void* pool_y, int N, int H, int W, int Cin, int Cout,
+                         int relu, void* stream) {
+  if (!x || !w || !y_bf16 || !pool_y) return nn_fail("conv_fwd_pool: bad pointers");
+  if (((uintptr_t)bias & 15) || ((uintptr_t)pool_y & 7))
+    return nn_fail("conv_fwd_pool: bias must be 16-byte and pool_y 8-byte aligned");
+  if (N < 1 || H < 2 || W < 2 || H % 2 || W % 2 || Cin < 32 || Cin % 32 || Cout < 32 || Cout % 32)
+    return nn_fail("conv_fwd_pool: need H, W even and Cin, Cout multiples of 32");
+  if ((long long)N * H * W > INT32_MAX / 2) return nn_fail("conv_fwd_pool: too many pixels");
+  const long long tiles16 = (long long)N * ((H + 15) / 16) * ((W + kTW - 1) / kTW);
+  if (tiles16 < device_cus()) {
+    // layers the tile loop does not take: the convolution, then the pool
+    if (int r = shy_nn_conv_fwd(x, w, bias, res, y_bf16, nullptr, N, H, W, Cin, Cout, 9, relu,
+                                stream))
+      return r;
+    return shy_nn_pool2(y_bf16, pool_y, N, H, W, Cout, 0, stream);
+  }
+  ConvArgs a;
+  a.x = (const __bf16*)x;
+  a.w = (const __bf16*)w;
+  a.bias = bias;
+  a.res = (const __bf16*)res;
+  a.y_bf = (__bf16*)y_bf16;
+  a.y_f = nullptr;
+  a.N = N; a.H = H; a.W = W; a.Cin = Cin; a.Cout = Cout; a.K = 9 * Cin;
+  a.relu = relu;
+  a.x2 = nullptr;
+  a.C1 = 0;
+  a.gres = nullptr;
+  a.mask = nullptr;
+  a.pool_y = (__bf16*)pool_y;
+  return launch_tiles(a, (hipStream_t)stream, "conv_fwd_pool");
+}
 
 int shy_nn_conv_dgrad(const void* g, const void* wf, const void* gres, const void* mask,
                       void* dx_bf16, int N, int H, int W, int Cin, int Cout, int taps,

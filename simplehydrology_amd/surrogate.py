@@ -46,6 +46,8 @@ def nn():
             "shy_nn_conv_dgrad": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_conv_fwd_head": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32,
                                            i32, i32, vp]),
+            "shy_nn_conv_fwd_pool": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32,
+                                           vp]),
             "shy_nn_flip_weights": (i32, [vp, vp, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]),
             "shy_nn_conv_wgrad_noalloc": (i32, [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
@@ -183,6 +185,19 @@ class Conv:
                                       head.cout, 1 if relu else 0, _stream()))
         return y, out
 
+    def forward_pool(self, x, shape, res=None, relu=True):
+        """This 3x3 convolution and the 2x2 average pool of its output in one
+        pass (the bits of forward + shy_nn_pool2).  Returns (y, pooled y)."""
+        import torch
+        N, H, W = shape
+        assert self.taps == 9
+        y = torch.empty(N, H, W, self.cout, device=x.device, dtype=torch.bfloat16)
+        p = torch.empty(N, H // 2, W // 2, self.cout, device=x.device, dtype=torch.bfloat16)
+        _ck(nn().shy_nn_conv_fwd_pool(_p(x), _p(self.w_bf), _p(self.b), _p(res), _p(y), _p(p),
+                                      N, H, W, self.cin, self.cout, 1 if relu else 0,
+                                      _stream()))
+        return y, p
+
     def forward_upcat(self, xh, skip, shape, res=None, relu=True):
         """The same convolution of [up2(xh), skip] along channels, read from
         the two tensors (no concatenated copy)."""
@@ -278,21 +293,27 @@ class ResStage:
         self.b = Conv(c, c, 9, gen, device)
         self.c = Conv(c, c, 9, gen, device)
 
-    def forward(self, x, shape, skip=None, head=None):
+    def forward(self, x, shape, skip=None, head=None, pool=False):
         """With ``skip`` the stage's input is [up2(x), skip] along channels,
         read without building it (inference: nothing is kept for backward).
         With ``head`` (a 1x1 Conv) returns (output or None with ``skip``,
-        head applied to it) from the last convolution's one pass."""
+        head applied to it) from the last convolution's one pass.  With
+        ``pool`` returns (output, its 2x2 average pool), also from one pass."""
         if skip is None:
             ya = self.a.forward(x, shape)
         else:
             ya = self.a.forward_upcat(x, skip, shape)
         yb = self.b.forward(ya, shape)
-        if head is None:
+        pooled = None
+        if pool:
+            yc, pooled = self.c.forward_pool(yb, shape, res=ya)
+        elif head is None:
             yc = self.c.forward(yb, shape, res=ya)
         else:
             yc, out = self.c.forward_head(yb, shape, head, res=ya, keep=skip is None)
         self.saved = (x, ya, yb, yc) if skip is None else None
+        if pool:
+            return yc, pooled
         return yc if head is None else (yc, out)
 
     def backward(self, g, shape, scratch, need_dx=True):
@@ -358,14 +379,14 @@ class UNet:
         skips = []
         h = x
         for k in range(4):
-            h = self.e[k].forward(h, self.shapes[k])
             if k < 3:
+                # the stage's last convolution also writes the pooled input
+                # of the next stage
+                h, p = self.e[k].forward(h, self.shapes[k], pool=True)
                 skips.append(h)
-                n, hh, ww = self.shapes[k]
-                p = torch.empty(n, hh // 2, ww // 2, h.shape[-1], dtype=torch.bfloat16,
-                                device=x.device)
-                _ck(nn().shy_nn_pool2(_p(h), _p(p), n, hh, ww, h.shape[-1], 0, _stream()))
                 h = p
+            else:
+                h = self.e[k].forward(h, self.shapes[k])
         self.cat = []
         for k in (2, 1, 0):
             n, hh, ww = self.shapes[k]

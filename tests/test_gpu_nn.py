@@ -286,6 +286,35 @@ def test_conv_head_equals_conv_then_head(sg, N, H, W, cin, hc, with_res):
     assert torch.equal(o2.view(torch.int32), o1.view(torch.int32))
 
 
+@pytest.mark.parametrize("N,H,W,cin,cout,with_res", [
+    (1, 256, 256, 32, 32, True),    # the tile loop
+    (2, 200, 232, 32, 64, False),   # the tile loop, partial tiles, two channel blocks
+    (1, 32, 48, 64, 96, True),      # too few tiles: convolution, then pool
+])
+def test_conv_pool_equals_conv_then_pool(sg, N, H, W, cin, cout, with_res):
+    """shy_nn_conv_fwd_pool: the output is shy_nn_conv_fwd's bit for bit and
+    the pooled output shy_nn_pool2's on it, bit for bit; the pool also
+    against float32 on the CPU from the rounded output."""
+    g = torch.Generator().manual_seed(N * H + cout)
+    dev = "cuda:0"
+    conv = sg.Conv(cin, cout, 9, np.random.default_rng(6), dev)
+    conv.b.copy_((torch.randn(cout, generator=g) * 0.1).to(dev))
+    x = torch.randn(N, H, W, cin, generator=g).to(dev).to(torch.bfloat16).contiguous()
+    res = (torch.randn(N, H, W, cout, generator=g).to(dev).to(torch.bfloat16).contiguous()
+           if with_res else None)
+    print("conv+pool", N, H, W, cin, cout, with_res, flush=True)
+    y0 = conv.forward(x, (N, H, W), res=res)
+    p0 = torch.empty(N, H // 2, W // 2, cout, dtype=torch.bfloat16, device=dev)
+    assert sg.nn().shy_nn_pool2(sg._p(y0), sg._p(p0), N, H, W, cout, 0, sg._stream()) == 0
+    y1, p1 = conv.forward_pool(x, (N, H, W), res=res)
+    assert torch.equal(y1.view(torch.int16), y0.view(torch.int16))
+    assert torch.equal(p1.view(torch.int16), p0.view(torch.int16))
+    yc = y0.float().cpu().view(N, H // 2, 2, W // 2, 2, cout)
+    ref = (0.25 * (yc[:, :, 0, :, 0] + yc[:, :, 0, :, 1] + yc[:, :, 1, :, 0]
+                   + yc[:, :, 1, :, 1])).to(torch.bfloat16)
+    assert torch.equal(p1.cpu().view(torch.int16), ref.view(torch.int16))
+
+
 def test_conv_rejects_unaligned_bias(sg):
     """The convolution entry points read a lane's four biases in one 16-byte
     load, so they reject a bias that is not 16-byte aligned (before any
